@@ -1,0 +1,75 @@
+// Wavefront path-tracing kernels (declarations shared by renderer.hip).
+//
+// Path state lives in structure-of-arrays queues in HBM, one float4/uint4 per
+// field so every access is a coalesced 16-byte load or store per lane. Each
+// bounce runs closest-hit traversal over the ray queue, then shading, which
+// compacts surviving paths into the other half of a ping-pong queue (wave64
+// ballot + mbcnt prefix + one atomic per wave) and emits next-event shadow
+// rays into a compacted shadow queue, then any-hit traversal of the shadow
+// queue. Queue lengths stay in device memory, so a frame needs no host sync.
+#pragma once
+
+#include "bsdf.h"
+#include "trace.h"
+
+namespace aqd {
+
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+constexpr uint32_t kMaxBounces = 64;
+// counters[] layout
+constexpr uint32_t kCntRays = 0;        // [0, 65): rays in the queue of bounce b
+constexpr uint32_t kCntShadow = 80;     // [80, 144): shadow rays emitted at bounce b
+constexpr uint32_t kCntWords = 160;
+// stats64[] layout
+enum : uint32_t {
+  kStNodes = 0,
+  kStTris = 1,
+  kStShNodes = 2,
+  kStShTris = 3,
+  kStNonfinite = 4,
+  kStWords = 8
+};
+
+struct DevScene {
+  DevBvh bvh;
+  const float4* mats;     // 3 per material
+  const float4* lights;   // 4 per light: area (v0|area, v1, v2, Le); point (pos, -, -, I)
+  const float* light_cdf;
+  const float* light_pmf;
+  uint32_t n_area, n_lights;
+};
+
+struct CamParams {
+  float m[12];
+  float tan_half, aspect;
+  uint32_t width, height;
+};
+
+struct PassParams {
+  CamParams cam;
+  uint32_t npix;
+  uint32_t spp_in_pass;
+  uint32_t sample0;
+  uint32_t frame;
+  uint32_t seed;
+  uint32_t max_depth;
+  uint32_t rr_depth;
+  uint32_t dims;  // PCG32 draws reserved per sample
+};
+
+struct Queues {
+  float4* ro[2];   // origin, tmax
+  float4* rd[2];   // direction, -
+  float4* tp[2];   // throughput, pdf of the sampled direction
+  float4* L[2];    // radiance so far
+  uint4* rng[2];   // pcg state lo, hi, path id, depth
+  uint2* hit;      // triangle slot, t bits
+  float4* sho;     // shadow origin, tmax
+  float4* shd;     // shadow direction, destination bits
+  float4* shc;     // shadow contribution
+  float4* Lbuf;    // radiance of finished paths, by path id
+  uint32_t* counters;
+  unsigned long long* stats;
+};
+
+}  // namespace aqd

@@ -1,0 +1,287 @@
+#include "bvh.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+namespace aqua {
+namespace {
+
+struct Box {
+  float lo[3], hi[3];
+  void reset() {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = INFINITY;
+      hi[k] = -INFINITY;
+    }
+  }
+  void grow(const Box& b) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], b.lo[k]);
+      hi[k] = std::max(hi[k], b.hi[k]);
+    }
+  }
+  void grow(const float p[3]) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], p[k]);
+      hi[k] = std::max(hi[k], p[k]);
+    }
+  }
+  // Half surface area (the SAH only needs ratios).
+  double half_area() const {
+    if (!(hi[0] >= lo[0])) return 0.0;
+    double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+  }
+};
+
+struct BNode {
+  Box box;
+  int32_t left = -1, right = -1;  // builder node ids; -1 for a leaf
+  uint32_t begin = 0, count = 0;  // leaf triangle range in `order`
+  uint32_t depth = 0;
+};
+
+struct Builder {
+  const BvhBuildOpts& opts;
+  std::vector<Box> tri_box;
+  std::vector<float> centroid;  // 3 per triangle
+  std::vector<uint32_t> order;
+  std::vector<BNode> nodes;
+  std::atomic<uint32_t> next{0};
+  std::atomic<int> threads_live{0};
+  unsigned hw = 1;
+
+  explicit Builder(const BvhBuildOpts& o) : opts(o) {}
+
+  uint32_t alloc() { return next.fetch_add(1); }
+
+  // Builds the subtree over order[begin, begin+count) into node `id`.
+  void build(uint32_t id, uint32_t begin, uint32_t count, uint32_t depth) {
+    BNode& nd = nodes[id];
+    nd.depth = depth;
+    Box b, cb;
+    b.reset();
+    cb.reset();
+    for (uint32_t i = begin; i < begin + count; ++i) {
+      b.grow(tri_box[order[i]]);
+      cb.grow(&centroid[3 * size_t(order[i])]);
+    }
+    nd.box = b;
+    auto make_leaf = [&]() {
+      nd.left = nd.right = -1;
+      nd.begin = begin;
+      nd.count = count;
+    };
+    if (count <= 1) return make_leaf();
+
+    const int B = opts.bins;
+    double best_cost = INFINITY;
+    int best_axis = -1, best_split = -1;
+    for (int a = 0; a < 3; ++a) {
+      float ext = cb.hi[a] - cb.lo[a];
+      if (!(ext > 0.f)) continue;
+      float scale = float(B) / ext;
+      Box bb[64];
+      uint32_t bn[64];
+      for (int k = 0; k < B; ++k) {
+        bb[k].reset();
+        bn[k] = 0;
+      }
+      for (uint32_t i = begin; i < begin + count; ++i) {
+        uint32_t t = order[i];
+        int k = int((centroid[3 * size_t(t) + a] - cb.lo[a]) * scale);
+        k = k < 0 ? 0 : (k >= B ? B - 1 : k);
+        bb[k].grow(tri_box[t]);
+        ++bn[k];
+      }
+      double right_area[64];
+      uint32_t right_n[64];
+      Box acc;
+      acc.reset();
+      uint32_t n = 0;
+      for (int k = B - 1; k > 0; --k) {
+        acc.grow(bb[k]);
+        n += bn[k];
+        right_area[k] = acc.half_area();
+        right_n[k] = n;
+      }
+      acc.reset();
+      n = 0;
+      for (int k = 0; k < B - 1; ++k) {
+        acc.grow(bb[k]);
+        n += bn[k];
+        if (n == 0 || right_n[k + 1] == 0) continue;
+        double c = acc.half_area() * n + right_area[k + 1] * right_n[k + 1];
+        if (c < best_cost) {
+          best_cost = c;
+          best_axis = a;
+          best_split = k;
+        }
+      }
+    }
+    double parent_area = b.half_area();
+    double leaf_cost = double(count);
+    double split_cost = opts.traversal_cost + (parent_area > 0 ? best_cost / parent_area : INFINITY);
+    uint32_t mid;
+    if (best_axis < 0) {
+      // All centroids coincide: no SAH plane separates them.
+      if (count <= uint32_t(opts.max_leaf)) return make_leaf();
+      mid = begin + count / 2;
+    } else {
+      if (count <= uint32_t(opts.max_leaf) && leaf_cost <= split_cost) return make_leaf();
+      float ext = cb.hi[best_axis] - cb.lo[best_axis];
+      float scale = float(B) / ext;
+      float lo = cb.lo[best_axis];
+      uint32_t* m = std::partition(order.data() + begin, order.data() + begin + count, [&](uint32_t t) {
+        int k = int((centroid[3 * size_t(t) + best_axis] - lo) * scale);
+        k = k < 0 ? 0 : (k >= B ? B - 1 : k);
+        return k <= best_split;
+      });
+      mid = uint32_t(m - order.data());
+      if (mid == begin || mid == begin + count) mid = begin + count / 2;
+    }
+    uint32_t l = alloc(), r = alloc();
+    nd.left = int32_t(l);
+    nd.right = int32_t(r);
+    uint32_t nl = mid - begin, nr = count - nl;
+    // Large subtrees are built on their own thread.
+    if (nl > 65536 && nr > 65536 && threads_live.load() < int(hw)) {
+      threads_live.fetch_add(1);
+      std::thread th([=]() { build(l, begin, nl, depth + 1); });
+      build(r, mid, nr, depth + 1);
+      th.join();
+      threads_live.fetch_sub(1);
+    } else {
+      build(l, begin, nl, depth + 1);
+      build(r, mid, nr, depth + 1);
+    }
+  }
+};
+
+void set_child(BvhNode& dn, int side, const BNode& c, int32_t idx, int32_t count) {
+  if (side == 0) {
+    dn.f[0] = c.box.lo[0];
+    dn.f[1] = c.box.hi[0];
+    dn.f[2] = c.box.lo[1];
+    dn.f[3] = c.box.hi[1];
+    dn.f[8] = c.box.lo[2];
+    dn.f[9] = c.box.hi[2];
+  } else {
+    dn.f[4] = c.box.lo[0];
+    dn.f[5] = c.box.hi[0];
+    dn.f[6] = c.box.lo[1];
+    dn.f[7] = c.box.hi[1];
+    dn.f[10] = c.box.lo[2];
+    dn.f[11] = c.box.hi[2];
+  }
+  dn.child[side] = idx;
+  dn.count[side] = count;
+}
+
+}  // namespace
+
+void build_bvh(const float* verts, uint32_t n, const BvhBuildOpts& opts, Bvh* out) {
+  *out = Bvh();
+  Builder b(opts);
+  b.hw = std::max(1u, std::thread::hardware_concurrency());
+  b.tri_box.resize(n);
+  b.centroid.resize(3 * size_t(n));
+  b.order.resize(n);
+  for (uint32_t t = 0; t < n; ++t) {
+    Box bx;
+    bx.reset();
+    for (int v = 0; v < 3; ++v) bx.grow(&verts[9 * size_t(t) + 3 * size_t(v)]);
+    b.tri_box[t] = bx;
+    for (int k = 0; k < 3; ++k) b.centroid[3 * size_t(t) + k] = 0.5f * (bx.lo[k] + bx.hi[k]);
+    b.order[t] = t;
+  }
+  b.nodes.resize(n > 0 ? 2 * size_t(n) - 1 : 1);
+  uint32_t root = b.alloc();
+  if (n > 0) b.build(root, 0, n, 0);
+  b.nodes.resize(b.next.load());
+
+  // Flatten: device nodes are the builder's inner nodes in depth-first
+  // preorder; a root that is a leaf gets a device node of its own.
+  out->tri_order.resize(n);
+  std::vector<uint32_t> slot_of(b.nodes.size(), 0);
+  std::vector<uint32_t> stack;
+  const BNode& rt = b.nodes[root];
+  BvhNode empty;
+  memset(&empty, 0, sizeof empty);
+  empty.child[0] = empty.child[1] = -1;
+  // Emit triangles in leaf order while assigning device slots.
+  uint32_t tri_slot = 0;
+  std::vector<uint32_t> leaf_slot(b.nodes.size(), 0);
+  {
+    // Leaves in DFS order (left first) get contiguous triangle slots.
+    stack.push_back(root);
+    while (!stack.empty()) {
+      uint32_t id = stack.back();
+      stack.pop_back();
+      const BNode& nd = b.nodes[id];
+      if (nd.left < 0) {
+        leaf_slot[id] = tri_slot;
+        for (uint32_t i = 0; i < nd.count; ++i) out->tri_order[tri_slot++] = b.order[nd.begin + i];
+        out->max_depth = std::max(out->max_depth, nd.depth);
+      } else {
+        stack.push_back(uint32_t(nd.right));
+        stack.push_back(uint32_t(nd.left));
+      }
+    }
+  }
+  if (rt.left < 0) {
+    BvhNode dn = empty;
+    set_child(dn, 0, rt, 0, int32_t(rt.count));
+    if (rt.count == 0) dn.child[0] = -1;
+    for (int k = 0; k < 3; ++k) {  // empty right child: boxes unused
+      (void)k;
+    }
+    out->nodes.push_back(dn);
+  } else {
+    // Preorder over inner nodes.
+    std::vector<uint32_t> inner;
+    stack.push_back(root);
+    while (!stack.empty()) {
+      uint32_t id = stack.back();
+      stack.pop_back();
+      const BNode& nd = b.nodes[id];
+      if (nd.left < 0) continue;
+      slot_of[id] = uint32_t(inner.size());
+      inner.push_back(id);
+      stack.push_back(uint32_t(nd.right));
+      stack.push_back(uint32_t(nd.left));
+    }
+    out->nodes.resize(inner.size());
+    for (size_t i = 0; i < inner.size(); ++i) {
+      const BNode& nd = b.nodes[inner[i]];
+      BvhNode dn = empty;
+      for (int side = 0; side < 2; ++side) {
+        const BNode& c = b.nodes[side == 0 ? nd.left : nd.right];
+        if (c.left < 0)
+          set_child(dn, side, c, int32_t(leaf_slot[side == 0 ? nd.left : nd.right]), int32_t(c.count));
+        else
+          set_child(dn, side, c, int32_t(slot_of[side == 0 ? nd.left : nd.right]), 0);
+      }
+      out->nodes[i] = dn;
+    }
+  }
+  // SAH cost of the finished tree (traversal cost per inner node visit +
+  // 1 per triangle), relative to the root area.
+  double ra = rt.box.half_area();
+  double cost = 0.0;
+  if (ra > 0) {
+    for (const BNode& nd : b.nodes) {
+      double a = nd.box.half_area() / ra;
+      if (nd.left < 0)
+        cost += a * nd.count;
+      else
+        cost += a * opts.traversal_cost;
+    }
+  }
+  out->sah_cost = cost;
+}
+
+}  // namespace aqua

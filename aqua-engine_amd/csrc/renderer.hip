@@ -1,0 +1,1049 @@
+// Renderer: HIP kernels of the wavefront path tracer and their host-side
+// orchestration behind the C ABI of include/aqua/aqua.h.
+//
+// One renderer = one device, one HIP stream; every launch of a frame is
+// queued on that stream and ordered by it, and queue lengths are read by the
+// kernels from device memory, so the host never waits inside a frame.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "aqua/aqua.h"
+#include "device/kernels.h"
+#include "host/bvh.h"
+#include "host/internal.h"
+
+using namespace aqd;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// Wave-aggregated queue append: every lane of the wave must call it; lanes
+// with `want` get consecutive slots starting at *counter.
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
+  unsigned long long mask = __ballot(want);
+  uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),
+                                             __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+  uint32_t total = uint32_t(__popcll(mask));
+  uint32_t base = 0;
+  if (total != 0u && __lane_id() == 0) base = atomicAdd(counter, total);
+  base = __shfl(base, 0);
+  return base + below;
+}
+
+__global__ void k_reset(uint32_t* counters, uint32_t initial_rays) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kCntWords) counters[i] = (i == kCntRays) ? initial_rays : 0u;
+}
+
+__global__ void k_zero_stats(unsigned long long* st) {
+  uint32_t i = threadIdx.x;
+  if (i < kStWords) st[i] = 0ull;
+}
+
+__device__ __forceinline__ v3 camera_dir(const CamParams& c, uint32_t px, uint32_t py, float u0,
+                                         float u1) {
+  float fx = (float(px) + u0) / float(c.width);
+  float fy = (float(py) + u1) / float(c.height);
+  float nx = 2.0f * fx - 1.0f;
+  float ny = 1.0f - 2.0f * fy;
+  float cx = (nx * c.tan_half) * c.aspect;
+  float cy = ny * c.tan_half;
+  v3 d = v3{(c.m[0] * cx + c.m[1] * cy) - c.m[2], (c.m[4] * cx + c.m[5] * cy) - c.m[6],
+            (c.m[8] * cx + c.m[9] * cy) - c.m[10]};
+  return normalize(d);
+}
+
+// Camera rays + fresh path state for samples [sample0, sample0 + S) of
+// every pixel; path id = s_local * npix + pixel.
+__global__ void k_camera(PassParams pp, Queues q) {
+  uint32_t n = pp.npix * pp.spp_in_pass;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t s_local = i / pp.npix;
+    uint32_t pixel = i - s_local * pp.npix;
+    uint32_t px = pixel % pp.cam.width, py = pixel / pp.cam.width;
+    uint32_t sample = pp.sample0 + s_local;
+    Pcg rng = pcg_seed(pp.seed, pixel);
+    pcg_advance(rng, ((uint64_t(pp.frame) << 20) | uint64_t(sample)) * uint64_t(pp.dims));
+    float u[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) u[k] = u01(pcg_next(rng));
+    v3 d = camera_dir(pp.cam, px, py, u[0], u[1]);
+    q.ro[0][i] = make_float4(pp.cam.m[3], pp.cam.m[7], pp.cam.m[11], kFltMax);
+    q.rd[0][i] = make_float4(d.x, d.y, d.z, 0.0f);
+    q.tp[0][i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    q.L[0][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    q.rng[0][i] = make_uint4(uint32_t(rng.state), uint32_t(rng.state >> 32), i, 0u);
+  }
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_intersect(DevBvh bvh, const float4* __restrict__ ro,
+                                                      const float4* __restrict__ rd,
+                                                      const uint32_t* count, uint2* hit,
+                                                      unsigned long long* stats) {
+  uint32_t n = *count;
+  TraceCount tc{0u, 0u};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 o = ro[i], d = rd[i];
+    float t;
+    uint32_t slot = trace_closest<COUNT>(bvh, xyz(o), xyz(d), o.w, &t, &tc);
+    hit[i] = make_uint2(slot, __float_as_uint(t));
+  }
+  if (COUNT) {
+    atomicAdd(&stats[kStNodes], (unsigned long long)tc.nodes);
+    atomicAdd(&stats[kStTris], (unsigned long long)tc.tris);
+  }
+}
+
+__device__ __forceinline__ uint32_t pick_light(const float* cdf, uint32_t n, float u) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (u < cdf[mid])
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+// One path vertex per thread: emission (MIS-weighted), then, below the
+// depth limit, next-event estimation, BSDF sampling and Russian roulette.
+__global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, uint32_t b, Queues q) {
+  const uint32_t c = b & 1u, nx = c ^ 1u;
+  const uint32_t n = q.counters[kCntRays + b];
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    bool valid = i < n;
+    bool alive = false, shadow = false;
+    float4 n_ro, n_rd, n_tp, s_o, s_d, s_c;
+    uint4 n_rng;
+    v3 L = v3{0.0f, 0.0f, 0.0f};
+    uint32_t path_id = 0;
+    if (valid) {
+      float4 ro4 = q.ro[c][i], rd4 = q.rd[c][i], tp4 = q.tp[c][i], L4 = q.L[c][i];
+      uint4 rg = q.rng[c][i];
+      uint2 h = q.hit[i];
+      v3 o = xyz(ro4), d = xyz(rd4), tp = xyz(tp4);
+      float pdf_prev = tp4.w;
+      L = xyz(L4);
+      path_id = rg.z;
+      uint32_t depth = rg.w;
+      if (h.x != kMiss) {
+        float t = __uint_as_float(h.y);
+        const float4* tr = sc.bvh.tris + 3 * h.x;
+        float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
+        v3 v0 = xyz(p0), v1 = xyz(p1), v2 = xyz(p2);
+        Mat m = load_mat(sc.mats, __float_as_uint(p1.w));
+        v3 p = v3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z};
+        v3 wo = neg(d);
+        v3 ng = normalize(cross(sub(v1, v0), sub(v2, v0)));
+        float cg = dot(ng, wo);
+        if ((m.emission.x > 0.0f || m.emission.y > 0.0f || m.emission.z > 0.0f) && cg > 0.0f) {
+          float w = 1.0f;
+          if (depth > 0u) {
+            float pl = ((m.light_pdf * t) * t) / cg;
+            float b2 = pdf_prev * pdf_prev, l2 = pl * pl;
+            w = b2 / (b2 + l2);
+          }
+          L = v3{L.x + (tp.x * m.emission.x) * w, L.y + (tp.y * m.emission.y) * w,
+                 L.z + (tp.z * m.emission.z) * w};
+        }
+        if (depth < pp.max_depth) {
+          uint32_t pixel = path_id % pp.npix;
+          Pcg rng;
+          rng.state = (uint64_t(rg.y) << 32) | uint64_t(rg.x);
+          rng.inc = (uint64_t(pixel) << 1u) | 1u;
+          float u[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) u[k] = u01(pcg_next(rng));
+          v3 ns = cg < 0.0f ? neg(ng) : ng;
+          Bsdf bs = make_bsdf(m);
+          // ---- next-event estimation ----
+          if (sc.n_lights > 0u) {
+            uint32_t li = pick_light(sc.light_cdf, sc.n_lights, u[0]);
+            float psel = sc.light_pmf[li];
+            const float4* lp = sc.lights + 4 * li;
+            bool is_area = li < sc.n_area;
+            v3 wi, Le;
+            float d2, dist, pdf_l = 0.0f;
+            bool ok;
+            if (is_area) {
+              float4 a0 = lp[0], a1 = lp[1], a2 = lp[2], a3 = lp[3];
+              v3 lv0 = xyz(a0), lv1 = xyz(a1), lv2 = xyz(a2);
+              float area = a0.w;
+              Le = xyz(a3);
+              float su = __builtin_sqrtf(u[1]);
+              float b0 = 1.0f - su;
+              float b1 = u[2] * su;
+              float b2 = (1.0f - b0) - b1;
+              v3 qp = v3{(lv0.x * b0 + lv1.x * b1) + lv2.x * b2, (lv0.y * b0 + lv1.y * b1) + lv2.y * b2,
+                         (lv0.z * b0 + lv1.z * b1) + lv2.z * b2};
+              v3 nl = normalize(cross(sub(lv1, lv0), sub(lv2, lv0)));
+              v3 dv = sub(qp, p);
+              d2 = dot(dv, dv);
+              dist = __builtin_sqrtf(d2);
+              wi = scl(dv, 1.0f / dist);
+              float cl = -dot(nl, wi);
+              ok = cl > 0.0f && d2 > 0.0f;
+              if (ok) pdf_l = (psel * d2) / (area * cl);
+            } else {
+              float4 a0 = lp[0], a3 = lp[3];
+              Le = xyz(a3);
+              v3 dv = sub(xyz(a0), p);
+              d2 = dot(dv, dv);
+              dist = __builtin_sqrtf(d2);
+              wi = scl(dv, 1.0f / dist);
+              ok = d2 > 0.0f;
+            }
+            if (ok) {
+              float pb, ci;
+              v3 f = bsdf_eval(bs, ns, wo, wi, &pb, &ci);
+              if (ci > 0.0f) {
+                float sfac;
+                if (is_area) {
+                  float l2 = pdf_l * pdf_l, bb2 = pb * pb;
+                  float w = l2 / (l2 + bb2);
+                  sfac = (ci * w) / pdf_l;
+                } else {
+                  sfac = ci / (d2 * psel);
+                }
+                v3 con = v3{((tp.x * f.x) * Le.x) * sfac, ((tp.y * f.y) * Le.y) * sfac,
+                            ((tp.z * f.z) * Le.z) * sfac};
+                if (con.x > 0.0f || con.y > 0.0f || con.z > 0.0f) {
+                  shadow = true;
+                  v3 so = offset_ray(p, ns);
+                  s_o = make_float4(so.x, so.y, so.z, dist * 0.9999f);
+                  s_d = make_float4(wi.x, wi.y, wi.z, 0.0f);
+                  s_c = make_float4(con.x, con.y, con.z, 0.0f);
+                }
+              }
+            }
+          }
+          // ---- BSDF sampling + Russian roulette ----
+          v3 wi, f;
+          float pdf, ci;
+          if (bsdf_sample(bs, ns, wo, u[3], u[4], u[5], &wi, &f, &pdf, &ci)) {
+            float s = ci / pdf;
+            v3 tpn = v3{tp.x * (f.x * s), tp.y * (f.y * s), tp.z * (f.z * s)};
+            bool cont = tpn.x > 0.0f || tpn.y > 0.0f || tpn.z > 0.0f;
+            if (cont && depth >= pp.rr_depth) {
+              float qq = fmax2(tpn.x, fmax2(tpn.y, tpn.z));
+              if (qq > 0.95f) qq = 0.95f;
+              if (u[6] >= qq)
+                cont = false;
+              else
+                tpn = v3{tpn.x / qq, tpn.y / qq, tpn.z / qq};
+            }
+            if (cont) {
+              alive = true;
+              v3 no = offset_ray(p, ns);
+              n_ro = make_float4(no.x, no.y, no.z, kFltMax);
+              n_rd = make_float4(wi.x, wi.y, wi.z, 0.0f);
+              n_tp = make_float4(tpn.x, tpn.y, tpn.z, pdf);
+              n_rng = make_uint4(uint32_t(rng.state), uint32_t(rng.state >> 32), path_id, depth + 1u);
+            }
+          }
+        }
+      }
+    }
+    uint32_t j = wave_append(alive, &q.counters[kCntRays + b + 1]);
+    if (alive) {
+      q.ro[nx][j] = n_ro;
+      q.rd[nx][j] = n_rd;
+      q.tp[nx][j] = n_tp;
+      q.L[nx][j] = make_float4(L.x, L.y, L.z, 0.0f);
+      q.rng[nx][j] = n_rng;
+    } else if (valid) {
+      q.Lbuf[path_id] = make_float4(L.x, L.y, L.z, 0.0f);
+    }
+    uint32_t k = wave_append(shadow, &q.counters[kCntShadow + b]);
+    if (shadow) {
+      uint32_t dst = alive ? j : (0x80000000u | path_id);
+      s_d.w = __uint_as_float(dst);
+      q.sho[k] = s_o;
+      q.shd[k] = s_d;
+      q.shc[k] = s_c;
+    }
+  }
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_shadow(DevBvh bvh, Queues q, uint32_t b) {
+  const uint32_t nx = (b & 1u) ^ 1u;
+  uint32_t n = q.counters[kCntShadow + b];
+  TraceCount tc{0u, 0u};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 o = q.sho[i], d = q.shd[i];
+    if (!trace_any<COUNT>(bvh, xyz(o), xyz(d), o.w, &tc)) {
+      float4 cc = q.shc[i];
+      uint32_t dst = __float_as_uint(d.w);
+      float4* Lp = (dst & 0x80000000u) ? &q.Lbuf[dst & 0x7FFFFFFFu] : &q.L[nx][dst];
+      float4 l = *Lp;
+      *Lp = make_float4(l.x + cc.x, l.y + cc.y, l.z + cc.z, 0.0f);
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&q.stats[kStShNodes], (unsigned long long)tc.nodes);
+    atomicAdd(&q.stats[kStShTris], (unsigned long long)tc.tris);
+  }
+}
+
+__device__ __forceinline__ bool finite3(float4 v) {
+  return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
+}
+
+// Adds the pass's samples of each pixel to the film in sample order.
+__global__ void k_film32(PassParams pp, const float4* Lbuf, float4* film, unsigned long long* stats) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pp.npix) return;
+  float4 acc = film[p];
+  uint32_t bad = 0;
+  for (uint32_t s = 0; s < pp.spp_in_pass; ++s) {
+    float4 l = Lbuf[size_t(s) * pp.npix + p];
+    if (!finite3(l)) {
+      ++bad;
+      continue;
+    }
+    acc.x = acc.x + l.x;
+    acc.y = acc.y + l.y;
+    acc.z = acc.z + l.z;
+  }
+  film[p] = acc;
+  if (bad) atomicAdd(&stats[kStNonfinite], (unsigned long long)bad);
+}
+
+__global__ void k_film64(PassParams pp, const float4* Lbuf, double* film, unsigned long long* stats) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pp.npix) return;
+  double ax = film[3 * size_t(p)], ay = film[3 * size_t(p) + 1], az = film[3 * size_t(p) + 2];
+  uint32_t bad = 0;
+  for (uint32_t s = 0; s < pp.spp_in_pass; ++s) {
+    float4 l = Lbuf[size_t(s) * pp.npix + p];
+    if (!finite3(l)) {
+      ++bad;
+      continue;
+    }
+    ax += double(l.x);
+    ay += double(l.y);
+    az += double(l.z);
+  }
+  film[3 * size_t(p)] = ax;
+  film[3 * size_t(p) + 1] = ay;
+  film[3 * size_t(p) + 2] = az;
+  if (bad) atomicAdd(&stats[kStNonfinite], (unsigned long long)bad);
+}
+
+__global__ void k_zero_f4(float4* p, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+__global__ void k_zero_f64(double* p, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// Hit slots -> triangle ids (load order) and distances.
+__global__ void k_hits_out(const uint2* hit, const float4* tris, uint32_t n, uint32_t* ids, float* ts) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint2 h = hit[i];
+  ids[i] = h.x == kMiss ? kMiss : __float_as_uint(tris[3 * h.x].w);
+  ts[i] = __uint_as_float(h.y);
+}
+
+__global__ void k_occluded(DevBvh bvh, const float4* ro, const float4* rd, uint32_t n, uint8_t* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  TraceCount tc;
+  float4 o = ro[i], d = rd[i];
+  out[i] = trace_any<false>(bvh, xyz(o), xyz(d), o.w, &tc) ? 1 : 0;
+}
+
+// relMSE / relRMSE partial sums in FP64, one triple per block.
+__global__ void k_err_partial(const float4* film, float inv_spp, const double* ref, double inv_ref_spp,
+                              uint32_t npix, double* partial) {
+  __shared__ double sh[3][kBlock];
+  double a = 0.0, bsum = 0.0, c = 0.0;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    float4 f = film[p];
+    float xs[3] = {f.x * inv_spp, f.y * inv_spp, f.z * inv_spp};
+    for (int k = 0; k < 3; ++k) {
+      double x = double(xs[k]);
+      double y = ref[3 * size_t(p) + k] * inv_ref_spp;
+      double dxy = x - y;
+      a += dxy * dxy / (y * y + 1e-2);
+      bsum += dxy * dxy;
+      c += y * y;
+    }
+  }
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = bsum;
+  sh[2][threadIdx.x] = c;
+  __syncthreads();
+  for (uint32_t s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) partial[3 * blockIdx.x + k] = sh[k][0];
+}
+
+__global__ void k_err_final(const double* partial, uint32_t nblocks, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (uint32_t i = 0; i < nblocks; ++i) {
+    a += partial[3 * i];
+    b += partial[3 * i + 1];
+    c += partial[3 * i + 2];
+  }
+  out[0] = a;
+  out[1] = b;
+  out[2] = c;
+}
+
+// Elementwise FP32 arithmetic of the device for host-side comparison:
+// out[5*i + {0..4}] = a/b, sqrt(a), fma(a, b, c), sin(2 pi u), cos(2 pi u).
+__global__ void k_fp_probe(const float* a, const float* b, const float* cc, const float* u, uint32_t n,
+                           float* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s, c;
+  sincos_2pi(u[i], &s, &c);
+  out[5 * i + 0] = a[i] / b[i];
+  out[5 * i + 1] = __builtin_sqrtf(a[i]);
+  out[5 * i + 2] = __builtin_fmaf(a[i], b[i], cc[i]);
+  out[5 * i + 3] = s;
+  out[5 * i + 4] = c;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return aqua::fail(AQUA_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                               \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int ensure(size_t count) {
+    if (count <= n && p) return AQUA_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return aqua::fail(e == hipErrorOutOfMemory ? AQUA_ERR_OOM : AQUA_ERR_HIP,
+                        "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    n = count;
+    return AQUA_OK;
+  }
+  int upload(const T* src, size_t count, hipStream_t s) {
+    if (int e = ensure(count)) return e;
+    if (count == 0) return AQUA_OK;
+    HIPCHK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    return AQUA_OK;
+  }
+};
+
+struct aqua_renderer {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  aqua_integrator_cfg cfg{};
+  aqua_camera camera{};
+  aqua_renderer_opts opts{};
+  uint32_t n_tris = 0, n_area = 0, n_lights = 0;
+  float delta = 0.0f;
+  std::vector<uint32_t> tri_order;
+  DevBuf<float4> tris, nodes, mats, lights;
+  DevBuf<float> cdf, pmf;
+  // path queues
+  size_t cap = 0;
+  DevBuf<float4> ro[2], rd[2], tp[2], L[2], sho, shd, shc, Lbuf;
+  DevBuf<uint4> rng[2];
+  DevBuf<uint2> hit;
+  DevBuf<uint32_t> counters;
+  DevBuf<unsigned long long> stats;
+  // films
+  uint32_t film_w = 0, film_h = 0, film_spp = 0, oracle_spp = 0;
+  DevBuf<float4> film;
+  DevBuf<double> film64;
+  DevBuf<double> err_partial;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> kev;
+
+  ~aqua_renderer() {
+    if (stream) {
+      (void)hipSetDevice(device);
+      (void)hipStreamSynchronize(stream);
+    }
+    for (hipEvent_t e : kev) (void)hipEventDestroy(e);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  DevScene dev_scene() const {
+    DevScene s;
+    s.bvh.nodes = nodes.p;
+    s.bvh.tris = tris.p;
+    s.bvh.delta = delta;
+    s.mats = mats.p;
+    s.lights = lights.p;
+    s.light_cdf = cdf.p;
+    s.light_pmf = pmf.p;
+    s.n_area = n_area;
+    s.n_lights = n_lights;
+    return s;
+  }
+
+  Queues queues() {
+    Queues q;
+    for (int k = 0; k < 2; ++k) {
+      q.ro[k] = ro[k].p;
+      q.rd[k] = rd[k].p;
+      q.tp[k] = tp[k].p;
+      q.L[k] = L[k].p;
+      q.rng[k] = rng[k].p;
+    }
+    q.hit = hit.p;
+    q.sho = sho.p;
+    q.shd = shd.p;
+    q.shc = shc.p;
+    q.Lbuf = Lbuf.p;
+    q.counters = counters.p;
+    q.stats = stats.p;
+    return q;
+  }
+
+  int ensure_queues(size_t n) {
+    if (n <= cap) return AQUA_OK;
+    for (int k = 0; k < 2; ++k) {
+      if (int e = ro[k].ensure(n)) return e;
+      if (int e = rd[k].ensure(n)) return e;
+      if (int e = tp[k].ensure(n)) return e;
+      if (int e = L[k].ensure(n)) return e;
+      if (int e = rng[k].ensure(n)) return e;
+    }
+    if (int e = hit.ensure(n)) return e;
+    if (int e = sho.ensure(n)) return e;
+    if (int e = shd.ensure(n)) return e;
+    if (int e = shc.ensure(n)) return e;
+    if (int e = Lbuf.ensure(n)) return e;
+    cap = n;
+    return AQUA_OK;
+  }
+};
+
+namespace {
+
+int grid_for(size_t n) {
+  size_t g = (n + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > 16384) g = 16384;
+  return int(g);
+}
+
+CamParams cam_params(const aqua_camera& c) {
+  CamParams p;
+  memcpy(p.m, c.cam_to_world, sizeof p.m);
+  p.tan_half = c.tan_half_fov;
+  p.aspect = c.aspect;
+  p.width = c.width;
+  p.height = c.height;
+  return p;
+}
+
+int check_camera(const aqua_camera* c) {
+  if (c->width == 0 || c->height == 0 || c->width > 16384 || c->height > 16384)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "camera resolution %ux%u out of range", c->width, c->height);
+  for (float v : c->cam_to_world)
+    if (!std::isfinite(v)) return aqua::fail(AQUA_ERR_NONFINITE, "camera matrix is not finite");
+  if (!std::isfinite(c->tan_half_fov) || !(c->tan_half_fov > 0.0f) || !std::isfinite(c->aspect) ||
+      !(c->aspect > 0.0f))
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "camera fov / aspect out of range");
+  return AQUA_OK;
+}
+
+int select_device(int device) {
+  if (device < 0)
+    return aqua::fail(AQUA_ERR_INVALID_ARG,
+                      "device %d: the renderer runs on a GPU only (there is no CPU path)", device);
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return aqua::fail(AQUA_ERR_NO_DEVICE, "no HIP device available");
+  if (device >= count)
+    return aqua::fail(AQUA_ERR_NO_DEVICE, "device %d out of range (%d devices)", device, count);
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return aqua::fail(AQUA_ERR_NO_DEVICE, "device %d is %s; this build targets gfx950", device,
+                      prop.gcnArchName);
+  HIPCHK(hipSetDevice(device));
+  return AQUA_OK;
+}
+
+// One pass: S samples of every pixel through all bounces, then the film.
+int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats* st, bool count) {
+  hipStream_t s = r->stream;
+  uint32_t n = pp.npix * pp.spp_in_pass;
+  DevScene sc = r->dev_scene();
+  Queues q = r->queues();
+  int g = grid_for(n);
+  k_reset<<<1, kCntWords, 0, s>>>(q.counters, n);
+  k_camera<<<g, kBlock, 0, s>>>(pp, q);
+  bool timing = r->opts.timing != 0;
+  for (uint32_t b = 0; b <= pp.max_depth; ++b) {
+    uint32_t c = b & 1u;
+    const uint32_t* cnt = q.counters + kCntRays + b;
+    if (timing) HIPCHK(hipEventRecord(r->kev[2 * b], s));
+    if (count)
+      k_intersect<true><<<g, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, q.hit, q.stats);
+    else
+      k_intersect<false><<<g, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, q.hit, q.stats);
+    if (timing) HIPCHK(hipEventRecord(r->kev[2 * b + 1], s));
+    k_shade<<<g, kBlock, 0, s>>>(sc, pp, b, q);
+    if (b < pp.max_depth) {
+      if (count)
+        k_shadow<true><<<g, kBlock, 0, s>>>(sc.bvh, q, b);
+      else
+        k_shadow<false><<<g, kBlock, 0, s>>>(sc.bvh, q, b);
+    }
+  }
+  int gp = int((pp.npix + kBlock - 1) / kBlock);
+  if (fp64)
+    k_film64<<<gp, kBlock, 0, s>>>(pp, q.Lbuf, r->film64.p, q.stats);
+  else
+    k_film32<<<gp, kBlock, 0, s>>>(pp, q.Lbuf, r->film.p, q.stats);
+  HIPCHK(hipGetLastError());
+  // Per-pass counters (ray counts) are read back once the pass is queued;
+  // the copy waits on the stream, so passes stay ordered.
+  uint32_t hc[kCntWords];
+  HIPCHK(hipMemcpyAsync(hc, q.counters, sizeof hc, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (st) {
+    st->camera_rays += hc[kCntRays];
+    for (uint32_t b = 1; b <= pp.max_depth; ++b) st->extension_rays += hc[kCntRays + b];
+    for (uint32_t b = 0; b < pp.max_depth; ++b) st->shadow_rays += hc[kCntShadow + b];
+    st->n_intersect_launches += pp.max_depth + 1;
+    if (timing) {
+      for (uint32_t b = 0; b <= pp.max_depth; ++b) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, r->kev[2 * b], r->kev[2 * b + 1]));
+        st->ms_intersect += ms;
+      }
+    }
+  }
+  return AQUA_OK;
+}
+
+int prepare_frame(aqua_renderer* r, const aqua_camera* cam_in, uint32_t spp, aqua_camera* cam,
+                  PassParams* pp, uint32_t* S) {
+  *cam = cam_in ? *cam_in : r->camera;
+  if (int e = check_camera(cam)) return e;
+  if (spp == 0 || spp > (1u << 20)) return aqua::fail(AQUA_ERR_INVALID_ARG, "spp %u out of range", spp);
+  uint64_t npix = uint64_t(cam->width) * cam->height;
+  uint32_t s_pass = r->opts.samples_per_pass;
+  if (s_pass == 0) {
+    uint64_t target = 8ull << 20;
+    s_pass = uint32_t(std::max<uint64_t>(1, target / npix));
+  }
+  s_pass = std::min(s_pass, spp);
+  if (npix * s_pass >= 0x7FFFFFFFull)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "too many paths in one pass");
+  if (int e = r->ensure_queues(size_t(npix) * s_pass)) return e;
+  *S = s_pass;
+  memset(pp, 0, sizeof *pp);
+  pp->cam = cam_params(*cam);
+  pp->npix = uint32_t(npix);
+  pp->seed = r->cfg.seed;
+  pp->max_depth = r->cfg.max_depth;
+  pp->rr_depth = r->cfg.rr_depth;
+  pp->dims = 8u * (r->cfg.max_depth + 2u);
+  return AQUA_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int aqua_device_count(int* count) {
+  if (!count) return aqua::fail(AQUA_ERR_INVALID_ARG, "null count");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *count = e == hipSuccess ? c : 0;
+  return AQUA_OK;
+}
+
+int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg, int device,
+                         aqua_renderer** out) {
+  if (!scene || !cfg || !out) return aqua::fail(AQUA_ERR_INVALID_ARG, "aqua_renderer_create: null argument");
+  *out = nullptr;
+  if (cfg->max_depth >= kMaxBounces)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "max_depth %u exceeds %u", cfg->max_depth, kMaxBounces - 1);
+  if (int e = select_device(device)) return e;
+  auto r = std::make_unique<aqua_renderer>();
+  r->device = device;
+  r->cfg = *cfg;
+  r->camera = scene->camera;
+  r->opts.timing = 1;
+  HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&r->ev0));
+  HIPCHK(hipEventCreate(&r->ev1));
+  r->kev.resize(2 * (kMaxBounces + 1));
+  for (hipEvent_t& e : r->kev) HIPCHK(hipEventCreate(&e));
+
+  // Geometry in BVH leaf order.
+  const uint32_t n = uint32_t(scene->mat_ids.size());
+  aqua::Bvh bvh;
+  aqua::BvhBuildOpts bo;
+  aqua::build_bvh(scene->verts.data(), n, bo, &bvh);
+  if (bvh.max_depth + 2 >= uint32_t(kStackSize))
+    return aqua::fail(AQUA_ERR_UNSUPPORTED, "BVH depth %u exceeds the traversal stack", bvh.max_depth);
+  std::vector<float4> tris(3 * size_t(n));
+  for (uint32_t s = 0; s < n; ++s) {
+    uint32_t id = bvh.tri_order[s];
+    const float* v = &scene->verts[9 * size_t(id)];
+    uint32_t m = scene->mat_ids[id];
+    float idf, mf;
+    memcpy(&idf, &id, 4);
+    memcpy(&mf, &m, 4);
+    tris[3 * size_t(s) + 0] = make_float4(v[0], v[1], v[2], idf);
+    tris[3 * size_t(s) + 1] = make_float4(v[3], v[4], v[5], mf);
+    tris[3 * size_t(s) + 2] = make_float4(v[6], v[7], v[8], 0.0f);
+  }
+  r->n_tris = n;
+  r->tri_order = bvh.tri_order;
+  std::vector<float4> nodes(4 * bvh.nodes.size());
+  memcpy(nodes.data(), bvh.nodes.data(), bvh.nodes.size() * sizeof(aqua::BvhNode));
+  // Culling slack: 2^-16 of the largest coordinate magnitude.
+  float mx = 0.0f;
+  for (int k = 0; k < 3; ++k) {
+    mx = std::max(mx, std::fabs(scene->info.bounds_lo[k]));
+    mx = std::max(mx, std::fabs(scene->info.bounds_hi[k]));
+  }
+  r->delta = std::ldexp(mx > 0.0f ? mx : 1.0f, -16);
+
+  aqua::LightTable lt;
+  aqua::build_light_table(scene, &lt);
+  std::vector<float4> mats(3 * scene->materials.size());
+  for (size_t m = 0; m < scene->materials.size(); ++m) {
+    const aqua_material& a = scene->materials[m];
+    mats[3 * m + 0] = make_float4(a.base_color[0], a.base_color[1], a.base_color[2], a.metallic);
+    mats[3 * m + 1] = make_float4(a.emission[0], a.emission[1], a.emission[2], a.roughness);
+    mats[3 * m + 2] = make_float4(a.specular, lt.mat_pdf[m], 0.0f, 0.0f);
+  }
+  r->n_area = uint32_t(lt.tri.size());
+  r->n_lights = uint32_t(lt.pmf.size());
+  if (!(lt.total_power > 0.0)) r->n_lights = 0;
+  std::vector<float4> lights(4 * std::max<size_t>(1, lt.pmf.size()), make_float4(0, 0, 0, 0));
+  for (uint32_t i = 0; i < r->n_area; ++i) {
+    const float* v = &scene->verts[9 * size_t(lt.tri[i])];
+    const aqua_material& a = scene->materials[scene->mat_ids[lt.tri[i]]];
+    lights[4 * i + 0] = make_float4(v[0], v[1], v[2], lt.area[i]);
+    lights[4 * i + 1] = make_float4(v[3], v[4], v[5], 0.0f);
+    lights[4 * i + 2] = make_float4(v[6], v[7], v[8], 0.0f);
+    lights[4 * i + 3] = make_float4(a.emission[0], a.emission[1], a.emission[2], 0.0f);
+  }
+  for (size_t j = 0; j < scene->point_lights.size(); ++j) {
+    const aqua_point_light& p = scene->point_lights[j];
+    size_t i = r->n_area + j;
+    lights[4 * i + 0] = make_float4(p.pos[0], p.pos[1], p.pos[2], 0.0f);
+    lights[4 * i + 3] = make_float4(p.intensity[0], p.intensity[1], p.intensity[2], 0.0f);
+  }
+  std::vector<float> cdf = lt.cdf, pmf = lt.pmf;
+  if (cdf.empty()) {
+    cdf.push_back(1.0f);
+    pmf.push_back(0.0f);
+  }
+  if (n == 0) {
+    // An empty scene still needs a valid root: one node with two empty children.
+    nodes.assign(4, make_float4(0, 0, 0, 0));
+    int32_t e4[4] = {-1, -1, 0, 0};
+    memcpy(&nodes[3], e4, 16);
+    tris.assign(3, make_float4(0, 0, 0, 0));
+  }
+  hipStream_t s = r->stream;
+  if (int e = r->tris.upload(tris.data(), tris.size(), s)) return e;
+  if (int e = r->nodes.upload(nodes.data(), nodes.size(), s)) return e;
+  if (int e = r->mats.upload(mats.data(), mats.size(), s)) return e;
+  if (int e = r->lights.upload(lights.data(), lights.size(), s)) return e;
+  if (int e = r->cdf.upload(cdf.data(), cdf.size(), s)) return e;
+  if (int e = r->pmf.upload(pmf.data(), pmf.size(), s)) return e;
+  if (int e = r->counters.ensure(kCntWords)) return e;
+  if (int e = r->stats.ensure(kStWords)) return e;
+  HIPCHK(hipStreamSynchronize(s));
+  *out = r.release();
+  return AQUA_OK;
+}
+
+int aqua_renderer_set_opts(aqua_renderer* r, const aqua_renderer_opts* o) {
+  if (!r || !o) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  r->opts = *o;
+  return AQUA_OK;
+}
+
+void aqua_renderer_destroy(aqua_renderer* r) { delete r; }
+
+static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t frame, uint32_t spp0,
+                          uint32_t spp, bool fp64, aqua_frame_stats* stats) {
+  HIPCHK(hipSetDevice(r->device));
+  aqua_camera cam;
+  PassParams pp;
+  uint32_t S;
+  if (int e = prepare_frame(r, cam_in, spp, &cam, &pp, &S)) return e;
+  uint64_t npix = pp.npix;
+  bool fresh = fp64 ? spp0 == 0 : true;
+  if (fp64) {
+    if (int e = r->film64.ensure(3 * npix)) return e;
+    if (!fresh && (cam.width != r->film_w || cam.height != r->film_h))
+      return aqua::fail(AQUA_ERR_INVALID_ARG, "oracle resume with a different resolution");
+    if (fresh) {
+      k_zero_f64<<<int((3 * npix + kBlock - 1) / kBlock), kBlock, 0, r->stream>>>(r->film64.p, 3 * npix);
+      r->oracle_spp = 0;
+    }
+  } else {
+    if (int e = r->film.ensure(npix)) return e;
+    k_zero_f4<<<int((npix + kBlock - 1) / kBlock), kBlock, 0, r->stream>>>(r->film.p, npix);
+  }
+  aqua_frame_stats st;
+  memset(&st, 0, sizeof st);
+  bool count = r->opts.count_traversal != 0;
+  k_zero_stats<<<1, 64, 0, r->stream>>>(r->stats.p);
+  HIPCHK(hipEventRecord(r->ev0, r->stream));
+  for (uint32_t s0 = 0; s0 < spp; s0 += S) {
+    pp.sample0 = spp0 + s0;
+    pp.spp_in_pass = std::min(S, spp - s0);
+    pp.frame = frame;
+    if (int e = run_pass(r, pp, fp64, &st, count)) return e;
+    ++st.n_passes;
+  }
+  HIPCHK(hipEventRecord(r->ev1, r->stream));
+  HIPCHK(hipEventSynchronize(r->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+  st.ms_frame = ms;
+  unsigned long long hs[kStWords];
+  HIPCHK(hipMemcpy(hs, r->stats.p, sizeof hs, hipMemcpyDeviceToHost));
+  st.nodes_visited = hs[kStNodes];
+  st.tris_tested = hs[kStTris];
+  st.shadow_nodes_visited = hs[kStShNodes];
+  st.shadow_tris_tested = hs[kStShTris];
+  st.nonfinite_samples = hs[kStNonfinite];
+  r->film_w = cam.width;
+  r->film_h = cam.height;
+  if (fp64)
+    r->oracle_spp += spp;
+  else
+    r->film_spp = spp;
+  if (stats) *stats = st;
+  return AQUA_OK;
+}
+
+int aqua_render_frame(aqua_renderer* r, const aqua_camera* camera, uint32_t frame, uint32_t spp,
+                      aqua_frame_stats* stats) {
+  if (!r) return aqua::fail(AQUA_ERR_INVALID_ARG, "null renderer");
+  if (frame >= (1u << 24)) return aqua::fail(AQUA_ERR_INVALID_ARG, "frame index too large");
+  return render_samples(r, camera, frame, 0, spp, false, stats);
+}
+
+int aqua_render_oracle(aqua_renderer* r, const aqua_camera* camera, uint32_t frame, uint32_t spp_begin,
+                       uint32_t spp_count) {
+  if (!r) return aqua::fail(AQUA_ERR_INVALID_ARG, "null renderer");
+  if (frame >= (1u << 24)) return aqua::fail(AQUA_ERR_INVALID_ARG, "frame index too large");
+  if (spp_begin != 0 && spp_begin != r->oracle_spp)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "oracle resume at %u but the film holds %u samples", spp_begin,
+                      r->oracle_spp);
+  if (uint64_t(spp_begin) + spp_count > (1u << 20))
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "oracle sample range too large");
+  return render_samples(r, camera, frame, spp_begin, spp_count, true, nullptr);
+}
+
+int aqua_film_read(aqua_renderer* r, float* rgb, uint32_t* spp) {
+  if (!r || !rgb) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (r->film_spp == 0) return aqua::fail(AQUA_ERR_INVALID_ARG, "no frame rendered yet");
+  HIPCHK(hipSetDevice(r->device));
+  size_t npix = size_t(r->film_w) * r->film_h;
+  std::vector<float4> h(npix);
+  HIPCHK(hipMemcpy(h.data(), r->film.p, npix * sizeof(float4), hipMemcpyDeviceToHost));
+  float inv = 1.0f / float(r->film_spp);
+  for (size_t p = 0; p < npix; ++p) {
+    rgb[3 * p] = h[p].x * inv;
+    rgb[3 * p + 1] = h[p].y * inv;
+    rgb[3 * p + 2] = h[p].z * inv;
+  }
+  if (spp) *spp = r->film_spp;
+  return AQUA_OK;
+}
+
+int aqua_oracle_read(aqua_renderer* r, double* rgb, uint32_t* spp) {
+  if (!r || !rgb) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (r->oracle_spp == 0) return aqua::fail(AQUA_ERR_INVALID_ARG, "no oracle samples yet");
+  HIPCHK(hipSetDevice(r->device));
+  size_t n = 3 * size_t(r->film_w) * r->film_h;
+  HIPCHK(hipMemcpy(rgb, r->film64.p, n * sizeof(double), hipMemcpyDeviceToHost));
+  double inv = 1.0 / double(r->oracle_spp);
+  for (size_t i = 0; i < n; ++i) rgb[i] *= inv;
+  if (spp) *spp = r->oracle_spp;
+  return AQUA_OK;
+}
+
+int aqua_film_error(aqua_renderer* r, double* relmse, double* relrmse) {
+  if (!r || !relmse || !relrmse) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (r->film_spp == 0 || r->oracle_spp == 0)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "need both a frame and oracle samples");
+  HIPCHK(hipSetDevice(r->device));
+  uint32_t npix = r->film_w * r->film_h;
+  int nb = std::min(1024, int((npix + kBlock - 1) / kBlock));
+  if (int e = r->err_partial.ensure(3 * size_t(nb) + 3)) return e;
+  k_err_partial<<<nb, kBlock, 0, r->stream>>>(r->film.p, 1.0f / float(r->film_spp), r->film64.p,
+                                               1.0 / double(r->oracle_spp), npix, r->err_partial.p);
+  k_err_final<<<1, 1, 0, r->stream>>>(r->err_partial.p, uint32_t(nb), r->err_partial.p + 3 * nb);
+  HIPCHK(hipGetLastError());
+  double h[3];
+  HIPCHK(hipMemcpyAsync(h, r->err_partial.p + 3 * nb, sizeof h, hipMemcpyDeviceToHost, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  *relmse = h[0] / (3.0 * npix);
+  *relrmse = h[2] > 0 ? std::sqrt(h[1] / h[2]) : 0.0;
+  return AQUA_OK;
+}
+
+int aqua_primary_hits(aqua_renderer* r, const aqua_camera* cam_in, uint32_t frame, uint32_t sample,
+                      uint32_t* prim_ids, float* t_hit) {
+  if (!r || !prim_ids) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (frame >= (1u << 24) || sample >= (1u << 20))
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "frame / sample out of range");
+  HIPCHK(hipSetDevice(r->device));
+  aqua_camera cam;
+  PassParams pp;
+  uint32_t S;
+  if (int e = prepare_frame(r, cam_in, 1, &cam, &pp, &S)) return e;
+  pp.spp_in_pass = 1;
+  pp.sample0 = sample;
+  pp.frame = frame;
+  uint32_t n = pp.npix;
+  Queues q = r->queues();
+  DevScene sc = r->dev_scene();
+  DevBuf<uint32_t> ids;
+  DevBuf<float> ts;
+  if (int e = ids.ensure(n)) return e;
+  if (int e = ts.ensure(n)) return e;
+  hipStream_t s = r->stream;
+  k_reset<<<1, kCntWords, 0, s>>>(q.counters, n);
+  k_camera<<<grid_for(n), kBlock, 0, s>>>(pp, q);
+  k_intersect<false><<<grid_for(n), kBlock, 0, s>>>(sc.bvh, q.ro[0], q.rd[0], q.counters, q.hit, q.stats);
+  k_hits_out<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.hit, sc.bvh.tris, n, ids.p, ts.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(prim_ids, ids.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (t_hit) HIPCHK(hipMemcpyAsync(t_hit, ts.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return AQUA_OK;
+}
+
+static int upload_rays(aqua_renderer* r, uint32_t n, const float* rays) {
+  std::vector<float4> o(n), d(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* x = rays + 8 * size_t(i);
+    for (int k = 0; k < 8; ++k)
+      if (k != 7 && !std::isfinite(x[k]) && !(k == 3 && x[k] == INFINITY))
+        return aqua::fail(AQUA_ERR_NONFINITE, "ray %u has a non-finite component", i);
+    if (x[4] == 0.0f && x[5] == 0.0f && x[6] == 0.0f)
+      return aqua::fail(AQUA_ERR_INVALID_ARG, "ray %u has a zero direction", i);
+    o[i] = make_float4(x[0], x[1], x[2], x[3]);
+    d[i] = make_float4(x[4], x[5], x[6], 0.0f);
+  }
+  if (int e = r->ensure_queues(n)) return e;
+  HIPCHK(hipMemcpyAsync(r->ro[0].p, o.data(), n * sizeof(float4), hipMemcpyHostToDevice, r->stream));
+  HIPCHK(hipMemcpyAsync(r->rd[0].p, d.data(), n * sizeof(float4), hipMemcpyHostToDevice, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
+  return AQUA_OK;
+}
+
+int aqua_trace_closest(aqua_renderer* r, uint32_t n, const float* rays, uint32_t* prim_ids, float* t_hit) {
+  if (!r || (n && (!rays || !prim_ids))) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return AQUA_OK;
+  HIPCHK(hipSetDevice(r->device));
+  if (int e = upload_rays(r, n, rays)) return e;
+  Queues q = r->queues();
+  DevScene sc = r->dev_scene();
+  DevBuf<uint32_t> ids;
+  DevBuf<float> ts;
+  if (int e = ids.ensure(n)) return e;
+  if (int e = ts.ensure(n)) return e;
+  hipStream_t s = r->stream;
+  k_reset<<<1, kCntWords, 0, s>>>(q.counters, n);
+  k_intersect<false><<<grid_for(n), kBlock, 0, s>>>(sc.bvh, q.ro[0], q.rd[0], q.counters, q.hit, q.stats);
+  k_hits_out<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.hit, sc.bvh.tris, n, ids.p, ts.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(prim_ids, ids.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (t_hit) HIPCHK(hipMemcpyAsync(t_hit, ts.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return AQUA_OK;
+}
+
+int aqua_trace_occluded(aqua_renderer* r, uint32_t n, const float* rays, uint8_t* occluded) {
+  if (!r || (n && (!rays || !occluded))) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return AQUA_OK;
+  HIPCHK(hipSetDevice(r->device));
+  if (int e = upload_rays(r, n, rays)) return e;
+  DevScene sc = r->dev_scene();
+  DevBuf<uint8_t> out;
+  if (int e = out.ensure(n)) return e;
+  hipStream_t s = r->stream;
+  k_occluded<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(sc.bvh, r->ro[0].p, r->rd[0].p, n, out.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(occluded, out.p, n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return AQUA_OK;
+}
+
+int aqua_fp_probe(uint32_t n, const float* a, const float* b, const float* c, const float* u, float* out) {
+  if (n == 0) return AQUA_OK;
+  if (!a || !b || !c || !u || !out) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (int e = select_device(dev)) return e;
+  DevBuf<float> da, db, dc, du, dout;
+  hipStream_t s = nullptr;
+  if (int e = da.upload(a, n, s)) return e;
+  if (int e = db.upload(b, n, s)) return e;
+  if (int e = dc.upload(c, n, s)) return e;
+  if (int e = du.upload(u, n, s)) return e;
+  if (int e = dout.ensure(5 * size_t(n))) return e;
+  k_fp_probe<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(da.p, db.p, dc.p, du.p, n, dout.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(out, dout.p, 5 * size_t(n) * sizeof(float), hipMemcpyDeviceToHost));
+  return AQUA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+/*
+ * aqua.h -- C ABI of the MI355X (gfx950) path tracer.
+ *
+ * The reference crate (`arukas`, Cargo.toml:2-5) ships no code: src/lib.rs is
+ * empty, so it has no FFI of its own. What it does define is data: the scene
+ * JSON (scenes/cbox.json:2-627, scenes/room.json:2-3899), the integrator JSON
+ * (scenes/integrator.json:2-8) and the BSON `.mesh` geometry files
+ * (scenes/CornellBox-Original_floor_0.mesh). Every entry point below takes or
+ * produces exactly those formats, so a Rust host of the reference binds this
+ * header with `extern "C"` and `#[repr(C)]` mirrors of the structs.
+ *
+ * Conventions
+ *   - Every call returns an `int` status (AQUA_OK == 0). On failure a
+ *     thread-local message is available from aqua_last_error(). No C++
+ *     exception and no abort crosses this boundary.
+ *   - Structs hold fixed-width scalars only (no bool, no pointers to C++
+ *     types); sizes are fixed and listed beside each struct.
+ *   - Handles are opaque, created by *_load / *_create, released by *_destroy.
+ *   - The renderer computes on a GPU only. There is no CPU fallback: a
+ *     negative device index is rejected with AQUA_ERR_INVALID_ARG, and a host
+ *     without a usable gfx950 device gets AQUA_ERR_NO_DEVICE.
+ */
+#ifndef AQUA_AQUA_H
+#define AQUA_AQUA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AQUA_ABI_VERSION 1
+
+enum aqua_status {
+  AQUA_OK = 0,
+  AQUA_ERR_IO = 1,            /* file missing / unreadable / unwritable          */
+  AQUA_ERR_PARSE = 2,         /* malformed JSON or BSON, unknown enum tag         */
+  AQUA_ERR_MISSING_MESH = 3,  /* a shape names a .mesh file that is absent        */
+  AQUA_ERR_NO_DEVICE = 4,     /* no GPU, or the device index is out of range      */
+  AQUA_ERR_OOM = 5,           /* host or device allocation failed                 */
+  AQUA_ERR_HIP = 6,           /* a HIP runtime call failed (message has details)  */
+  AQUA_ERR_INVALID_ARG = 7,   /* null handle, bad size, out-of-range index        */
+  AQUA_ERR_NONFINITE = 8,     /* non-finite input geometry or parameters          */
+  AQUA_ERR_UNSUPPORTED = 9    /* a feature of the format this build does not run  */
+};
+
+typedef struct aqua_scene aqua_scene;       /* immutable after load; shareable */
+typedef struct aqua_renderer aqua_renderer; /* one per device and thread       */
+
+/* ---- scene ------------------------------------------------------------- */
+
+/* 32 bytes. */
+typedef struct aqua_load_opts {
+  uint32_t skip_missing;      /* 1: shapes whose .mesh is absent are skipped and counted
+                                 (room.json:3021 names the withheld mesh recorded in
+                                 .MISSING_LARGE_BLOBS:1); 0: AQUA_ERR_MISSING_MESH        */
+  uint32_t image_as_constant; /* 1: `{"Image": path}` colours load as constant 0.5 grey
+                                 (room.json:6 and 15 others); 0: AQUA_ERR_UNSUPPORTED     */
+  uint32_t reserved[6];
+} aqua_load_opts;
+
+/* 48 bytes. The subset of the 17 Principled parameters (cbox.json:5-63) that
+ * the metallic-roughness BSDF uses; the loader checks that the rest hold the
+ * values the shipped scenes use everywhere (subsurface, sheen, clearcoat,
+ * transmission 0) and rejects others with AQUA_ERR_UNSUPPORTED. */
+typedef struct aqua_material {
+  float base_color[3]; /* linear RGB (Srgb inputs decoded with IEC 61966-2-1)   */
+  float metallic;
+  float emission[3];   /* linear RGB radiance; > 0 makes the mesh an area light  */
+  float roughness;     /* GGX alpha = max(roughness^2, 1e-3)                     */
+  float specular;      /* dielectric F0 = 0.08 * specular                        */
+  float pad[3];
+} aqua_material;
+
+/* 64 bytes. Pinhole camera. Camera space looks down -Z, +Y up (the shipped
+ * cameras sit at +Z looking at the scene: cbox.json:518-541). */
+typedef struct aqua_camera {
+  float cam_to_world[12]; /* row-major 3x4; columns: right, up, back (+Z), origin */
+  float tan_half_fov;     /* tan(vertical field of view / 2)                      */
+  float aspect;           /* width / height                                       */
+  uint32_t width, height;
+} aqua_camera;
+
+/* 24 bytes. `Point` light (cbox.json:544-561): radiant intensity, W/sr. */
+typedef struct aqua_point_light {
+  float pos[3];
+  float intensity[3];
+} aqua_point_light;
+
+/* 64 bytes. */
+typedef struct aqua_scene_info {
+  uint32_t n_triangles;
+  uint32_t n_materials;
+  uint32_t n_shapes;            /* shapes loaded                          */
+  uint32_t n_skipped_shapes;    /* shapes skipped for a missing .mesh     */
+  uint32_t n_emissive_triangles;
+  uint32_t n_point_lights;
+  uint32_t n_image_textures;    /* Image colours replaced by a constant   */
+  uint32_t n_vertices;          /* summed over the loaded meshes          */
+  float bounds_lo[3];
+  float bounds_hi[3];
+  uint32_t reserved[2];
+} aqua_scene_info;
+
+/* Load a scene JSON (schema of scenes/cbox.json) and every .mesh it names
+ * (paths relative to the JSON's directory; '\\' separators accepted). */
+int aqua_scene_load(const char* scene_json_path, const aqua_load_opts* opts, aqua_scene** out);
+
+/* Build a scene straight from flat arrays: `verts` holds 9 floats per
+ * triangle (v0, v1, v2), `mat_ids` one material index per triangle. */
+int aqua_scene_from_arrays(uint32_t n_triangles, const float* verts, const uint32_t* mat_ids,
+                           uint32_t n_materials, const aqua_material* materials,
+                           const aqua_camera* camera, uint32_t n_point_lights,
+                           const aqua_point_light* point_lights, aqua_scene** out);
+
+int aqua_scene_get_info(const aqua_scene* scene, aqua_scene_info* out);
+/* Flattened geometry in load order: verts[9*n], mat_ids[n] (either may be NULL). */
+int aqua_scene_get_triangles(const aqua_scene* scene, float* verts, uint32_t* mat_ids);
+int aqua_scene_get_materials(const aqua_scene* scene, aqua_material* out);
+int aqua_scene_get_camera(const aqua_scene* scene, aqua_camera* out);
+int aqua_scene_get_point_lights(const aqua_scene* scene, aqua_point_light* out);
+void aqua_scene_destroy(aqua_scene* scene);
+
+/* ---- procedural scenes (written in the reference schema: JSON + .mesh) -- */
+
+enum aqua_procgen_kind {
+  AQUA_PROCGEN_BOX_ROOM = 1,    /* 5 walls (10 tris), boxes 0.6/1.2 at +-17 deg (24 tris),
+                                   ceiling quad light L=(17,12,4) (CornellBox-Original.mtl:88),
+                                   512x512, fov 40                                             */
+  AQUA_PROCGEN_INTERIOR = 2     /* room + n_meshes seeded subdivided boxes / icospheres,
+                                   GGX metallic-roughness, 3 quad lights                       */
+};
+
+/* 32 bytes. */
+typedef struct aqua_procgen_params {
+  uint32_t kind;         /* aqua_procgen_kind                              */
+  uint32_t seed;         /* PCG32 seed of every random choice              */
+  uint32_t n_meshes;     /* INTERIOR: object meshes (e.g. 150, 400)        */
+  uint32_t target_tris;  /* INTERIOR: approximate triangle budget          */
+  uint32_t width, height;/* camera resolution written into the JSON        */
+  uint32_t reserved[2];
+} aqua_procgen_params;
+
+/* Writes <out_dir>/scene.json plus one .mesh per shape; the JSON path is
+ * copied into scene_path (capacity `cap`). Same params => same bytes. */
+int aqua_procgen_write(const aqua_procgen_params* params, const char* out_dir, char* scene_path,
+                       size_t cap);
+
+/* ---- integrator (scenes/integrator.json:2-8) --------------------------- */
+
+enum aqua_integrator_type { AQUA_INTEGRATOR_PT = 0, AQUA_INTEGRATOR_NRC = 1 };
+
+/* 64 bytes. */
+typedef struct aqua_integrator_cfg {
+  uint32_t type;            /* "type": "pt" | "nrc"            (integrator.json:2) */
+  uint32_t spp;             /* "spp"                           (integrator.json:3) */
+  uint32_t max_depth;       /* "max_depth": bounces            (integrator.json:5) */
+  uint32_t batch_size;      /* "batch_size"                    (integrator.json:4) */
+  uint32_t training_iters;  /* "training_iters"                (integrator.json:6) */
+  float learning_rate;      /* "learning_rate"                 (integrator.json:7) */
+  uint32_t visualize_cache; /* "visualize_cache"               (integrator.json:8) */
+  uint32_t seed;            /* "seed" (extension, default 0): PCG32 initstate     */
+  uint32_t rr_depth;        /* "rr_depth" (extension, default 3)                  */
+  uint32_t reserved[7];
+} aqua_integrator_cfg;
+
+void aqua_integrator_default(aqua_integrator_cfg* out);
+int aqua_integrator_load(const char* integrator_json_path, aqua_integrator_cfg* out);
+
+/* ---- renderer ----------------------------------------------------------- */
+
+/* 128 bytes. Counters and HIP-event timings of one aqua_render_frame call. */
+typedef struct aqua_frame_stats {
+  uint64_t camera_rays;
+  uint64_t extension_rays;     /* closest-hit rays after the camera rays   */
+  uint64_t shadow_rays;
+  uint64_t nonfinite_samples;  /* samples whose radiance was zeroed        */
+  double ms_frame;             /* events around the whole frame            */
+  double ms_intersect;         /* summed closest-hit kernel time           */
+  double ms_shade;
+  double ms_shadow;
+  uint32_t n_intersect_launches;
+  uint32_t n_passes;
+  uint64_t nodes_visited;      /* only when counting is on, else 0         */
+  uint64_t tris_tested;        /* closest-hit rays only                    */
+  uint64_t shadow_nodes_visited;
+  uint64_t shadow_tris_tested;
+  uint64_t reserved[3];
+} aqua_frame_stats;
+
+/* 32 bytes. */
+typedef struct aqua_renderer_opts {
+  uint32_t samples_per_pass;   /* 0 = choose (about 8 M paths in flight)      */
+  uint32_t count_traversal;    /* 1 = count nodes / triangles (slower)        */
+  uint32_t timing;             /* 1 = per-kernel HIP-event timing             */
+  uint32_t reserved[5];
+} aqua_renderer_opts;
+
+int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg, int device,
+                         aqua_renderer** out);
+int aqua_renderer_set_opts(aqua_renderer* r, const aqua_renderer_opts* opts);
+
+/* Render `spp` samples per pixel of frame `frame` into the renderer's FP32
+ * film (cleared first). `camera` NULL = the scene's camera. Sample s of
+ * pixel p draws from PCG32 stream p, advanced to ((frame << 20) | s) * D,
+ * D = 8 * (max_depth + 2), so results do not depend on the pass split. */
+int aqua_render_frame(aqua_renderer* r, const aqua_camera* camera, uint32_t frame, uint32_t spp,
+                      aqua_frame_stats* stats);
+/* Mean radiance of the last frame, RGB float per pixel, row 0 = top. */
+int aqua_film_read(aqua_renderer* r, float* rgb, uint32_t* spp);
+
+/* FP64-accumulated render: adds samples [spp_begin, spp_begin + spp_count)
+ * of `frame` into an FP64 film, so a long oracle render can be resumed;
+ * spp_begin == 0 clears it first. */
+int aqua_render_oracle(aqua_renderer* r, const aqua_camera* camera, uint32_t frame,
+                       uint32_t spp_begin, uint32_t spp_count);
+/* Mean of the FP64 film (double RGB per pixel) and the samples it holds. */
+int aqua_oracle_read(aqua_renderer* r, double* rgb, uint32_t* spp);
+/* relMSE = mean_{p,c} (x - y)^2 / (y^2 + 1e-2) and relRMSE =
+ * sqrt(sum (x - y)^2 / sum y^2) of the FP32 film (x) against the FP64 film
+ * (y), reduced in FP64 on the GPU. */
+int aqua_film_error(aqua_renderer* r, double* relmse, double* relrmse);
+
+/* First-hit triangle id (load order; 0xFFFFFFFF = miss) and distance for
+ * every pixel, sample `sample` of frame `frame`. */
+int aqua_primary_hits(aqua_renderer* r, const aqua_camera* camera, uint32_t frame,
+                      uint32_t sample, uint32_t* prim_ids, float* t_hit);
+
+/* Closest hit of arbitrary rays given on the host, 8 floats each:
+ * ox oy oz tmax dx dy dz 0 (directions need not be unit length). */
+int aqua_trace_closest(aqua_renderer* r, uint32_t n, const float* rays, uint32_t* prim_ids,
+                       float* t_hit);
+/* Any hit in (0, tmax): occluded[i] = 1 / 0. */
+int aqua_trace_occluded(aqua_renderer* r, uint32_t n, const float* rays, uint8_t* occluded);
+
+void aqua_renderer_destroy(aqua_renderer* r);
+
+/* ---- misc ---------------------------------------------------------------- */
+
+const char* aqua_last_error(void);
+int aqua_abi_version(void);
+/* Number of HIP devices visible (0 on a host without a GPU). */
+int aqua_device_count(int* count);
+
+/* Diagnostics: the BVH the renderer builds for `scene`, in its device
+ * layout (64-byte nodes: child boxes, child index, child triangle count;
+ * count > 0 = leaf over triangle slots [index, index + count), count == 0 =
+ * inner node `index`, index < 0 = empty). Call with nodes == NULL to get
+ * *n_nodes, then again with room for that many; order[slot] = triangle id. */
+int aqua_scene_bvh(const aqua_scene* scene, uint32_t* n_nodes, float* nodes, uint32_t* order,
+                   uint32_t* max_depth);
+
+/* Diagnostics: the device's FP32 arithmetic on n inputs, on the current
+ * device: out[5i..5i+4] = a/b, sqrt(a), fma(a, b, c), sin(2 pi u), cos(2 pi u),
+ * where sin/cos are the kernels' own sincos_2pi. */
+int aqua_fp_probe(uint32_t n, const float* a, const float* b, const float* c, const float* u,
+                  float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AQUA_AQUA_H */

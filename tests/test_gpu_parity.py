@@ -1,0 +1,145 @@
+"""GPU kernels vs the CPU restatement (oracle/) on identical inputs and seeds.
+
+Bar: primary-hit triangle ids bit-exact; FP32 film bit-exact (relRMSE
+tolerance 1e-4 written below is the contract, the kernels meet it with 0).
+"""
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, aqua, oracle, oracle_scene_of, small_camera
+
+pytestmark = pytest.mark.gpu
+
+RELRMSE_TOL = 1e-4
+
+
+def relrmse(x, y):
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    den = np.sum(y * y)
+    return float(np.sqrt(np.sum((x - y) ** 2) / den)) if den > 0 else float(np.sqrt(np.sum((x - y) ** 2)))
+
+
+def round_f32(r: Fraction) -> np.float32:
+    """Nearest-even FP32 value of an exact rational."""
+    f = np.float32(float(r))
+    best = f
+    for cand in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        if not np.isfinite(cand):
+            continue
+        d, bd = abs(Fraction(float(cand)) - r), abs(Fraction(float(best)) - r)
+        if d < bd or (d == bd and (int(cand.view(np.uint32)) & 1) == 0):
+            best = cand
+    return best
+
+
+def cfg(spp=4, max_depth=5, seed=0):
+    c = aqua.integrator_default()
+    c.spp = spp
+    c.max_depth = max_depth
+    c.seed = seed
+    return c
+
+
+def test_device_fp_arithmetic_is_ieee(gpu):
+    rng = np.random.default_rng(1)
+    n = 4096
+    a = (rng.random(n) * 10 ** rng.uniform(-6, 6, n)).astype(np.float32)
+    b = (rng.random(n) * 10 ** rng.uniform(-6, 6, n) + 1e-30).astype(np.float32)
+    c = (rng.standard_normal(n) * 10 ** rng.uniform(-6, 6, n)).astype(np.float32)
+    u = rng.random(n).astype(np.float32)
+    u[:4] = [0.0, 0.25, 0.5, np.float32(0.99999994)]
+    out = aqua.fp_probe(a, b, c, u)
+    np.testing.assert_array_equal(out[:, 0], a / b)          # correctly rounded divide
+    np.testing.assert_array_equal(out[:, 1], np.sqrt(a))     # correctly rounded sqrt
+    for i in range(0, n, 16):                                # single-rounding fma
+        exact = Fraction(float(a[i])) * Fraction(float(b[i])) + Fraction(float(c[i]))
+        assert out[i, 2] == round_f32(exact), i
+    s, co = oracle.sincos_2pi(u)
+    np.testing.assert_array_equal(out[:, 3], s)
+    np.testing.assert_array_equal(out[:, 4], co)
+
+
+def _hits_match(scene, orc_scene, cam=None, samples=(0, 1)):
+    r = aqua.Renderer(scene, cfg())
+    c = cam or scene.camera()
+    cam16 = aqua.camera_to_array(c)
+    for s in samples:
+        ids_g, t_g = r.primary_hits(sample=s, camera=cam)
+        ids_c, t_c = orc_scene.primary_hits(cam16, sample=s)
+        frac = float(np.mean(ids_g == ids_c))
+        assert frac == 1.0, f"sample {s}: {frac:.6f} of primary ids match"
+        hit = ids_c != aqua.MISS
+        np.testing.assert_array_equal(t_g[hit], t_c[hit])
+
+
+def test_primary_hits_box_room(gpu, box_room, box_room_oracle):
+    _hits_match(box_room, box_room_oracle)
+
+
+def test_primary_hits_shipped_cbox(gpu):
+    sc = aqua.Scene.load(os.path.join(GOLDEN, "cbox", "cbox.json"))
+    _hits_match(sc, oracle_scene_of(sc))
+
+
+def test_trace_closest_random_rays_vs_brute_force(gpu, box_room, box_room_oracle):
+    rng = np.random.default_rng(7)
+    n = 20000
+    o = rng.uniform([-0.99, 0.01, -0.99], [0.99, 1.99, 0.99], (n, 3))
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = 3.0e38
+    rays[:, 4:7] = d
+    r = aqua.Renderer(box_room, cfg())
+    ids_g, t_g = r.trace_closest(rays)
+    ids_b, t_b = box_room_oracle.trace_closest(rays, brute=True)
+    ids_c, t_c = box_room_oracle.trace_closest(rays)
+    np.testing.assert_array_equal(ids_c, ids_b)
+    np.testing.assert_array_equal(ids_g, ids_b)
+    np.testing.assert_array_equal(t_g, t_b)
+    # any-hit with tmax just short of the closest hit: never occluded
+    short = rays.copy()
+    short[:, 3] = np.where(ids_b != aqua.MISS, t_b * np.float32(0.999), 3.0e38)
+    occ_g = r.trace_occluded(short)
+    occ_c = box_room_oracle.trace_occluded(short, brute=True)
+    np.testing.assert_array_equal(occ_g, occ_c)
+
+
+def _render_match(scene, orc_scene, cam, spp, max_depth, seed=0, frame=0):
+    r = aqua.Renderer(scene, cfg(spp, max_depth, seed))
+    st = r.render(spp, frame=frame, camera=cam)
+    img_g = r.film(cam)
+    img_c, ost = orc_scene.render(aqua.camera_to_array(cam), spp, max_depth, seed=seed, frame=frame)
+    err = relrmse(img_g, img_c)
+    assert err <= RELRMSE_TOL, f"relRMSE {err:.3e}"
+    assert st.camera_rays == ost.camera_rays
+    assert st.extension_rays == ost.extension_rays
+    assert st.shadow_rays == ost.shadow_rays
+    return img_g, img_c, st
+
+
+def test_render_box_room_parity(gpu, box_room, box_room_oracle):
+    cam = small_camera(box_room.camera(), 96, 96)
+    g, c, st = _render_match(box_room, box_room_oracle, cam, spp=16, max_depth=5)
+    np.testing.assert_array_equal(g, c)
+    assert g.mean() > 0.05
+
+
+def test_render_shipped_cbox_point_light_parity(gpu):
+    sc = aqua.Scene.load(os.path.join(GOLDEN, "cbox", "cbox.json"))
+    cam = small_camera(sc.camera(), 64, 64)
+    g, c, _ = _render_match(sc, oracle_scene_of(sc), cam, spp=8, max_depth=5)
+    np.testing.assert_array_equal(g, c)
+
+
+def test_render_interior_ggx_parity(gpu, tmp_path):
+    path = aqua.procgen(aqua.PROCGEN_INTERIOR, str(tmp_path), seed=3, n_meshes=24, target_tris=60000,
+                        width=80, height=48)
+    sc = aqua.Scene.load(path)
+    g, c, _ = _render_match(sc, oracle_scene_of(sc), sc.camera(), spp=4, max_depth=8)
+    np.testing.assert_array_equal(g, c)
