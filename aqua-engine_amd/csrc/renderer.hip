@@ -622,19 +622,22 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   for (uint32_t b = 0; b <= pp.max_depth; ++b) {
     uint32_t c = b & 1u;
     const uint32_t* cnt = q.counters + kCntRays + b;
-    if (timing) HIPCHK(hipEventRecord(r->kev[2 * b], s));
+    hipEvent_t* ev = &r->kev[4 * b];
+    if (timing) HIPCHK(hipEventRecord(ev[0], s));
     if (count)
       k_intersect<true><<<g, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, q.hit, q.stats);
     else
       k_intersect<false><<<g, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, q.hit, q.stats);
-    if (timing) HIPCHK(hipEventRecord(r->kev[2 * b + 1], s));
+    if (timing) HIPCHK(hipEventRecord(ev[1], s));
     k_shade<<<g, kBlock, 0, s>>>(sc, pp, b, q);
+    if (timing) HIPCHK(hipEventRecord(ev[2], s));
     if (b < pp.max_depth) {
       if (count)
         k_shadow<true><<<g, kBlock, 0, s>>>(sc.bvh, q, b);
       else
         k_shadow<false><<<g, kBlock, 0, s>>>(sc.bvh, q, b);
     }
+    if (timing) HIPCHK(hipEventRecord(ev[3], s));
   }
   int gp = int((pp.npix + kBlock - 1) / kBlock);
   if (fp64)
@@ -654,9 +657,14 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     st->n_intersect_launches += pp.max_depth + 1;
     if (timing) {
       for (uint32_t b = 0; b <= pp.max_depth; ++b) {
+        const hipEvent_t* ev = &r->kev[4 * b];
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, r->kev[2 * b], r->kev[2 * b + 1]));
+        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
         st->ms_intersect += ms;
+        HIPCHK(hipEventElapsedTime(&ms, ev[1], ev[2]));
+        st->ms_shade += ms;
+        HIPCHK(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        st->ms_shadow += ms;
       }
     }
   }
@@ -720,7 +728,7 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&r->ev0));
   HIPCHK(hipEventCreate(&r->ev1));
-  r->kev.resize(2 * (kMaxBounces + 1));
+  r->kev.resize(4 * (kMaxBounces + 1));
   for (hipEvent_t& e : r->kev) HIPCHK(hipEventCreate(&e));
 
   // Geometry in BVH leaf order.

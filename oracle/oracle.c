@@ -546,6 +546,14 @@ static inline int box_test(const Ray* r, const float* lo, const float* hi, float
   return tn <= tf * FAR_SCALE + delta;
 }
 
+/* Entry distance only, for ordering children (never for culling). */
+static inline float box_entry(const Ray* r, const float* lo, const float* hi) {
+  float t0x = fmaf(lo[0], r->inv.x, -r->oi.x), t1x = fmaf(hi[0], r->inv.x, -r->oi.x);
+  float t0y = fmaf(lo[1], r->inv.y, -r->oi.y), t1y = fmaf(hi[1], r->inv.y, -r->oi.y);
+  float t0z = fmaf(lo[2], r->inv.z, -r->oi.z), t1z = fmaf(hi[2], r->inv.z, -r->oi.z);
+  return mx(mx(mn(t0x, t1x), mn(t0y, t1y)), mx(mn(t0z, t1z), 0.0f));
+}
+
 typedef struct {
   uint64_t nodes, tris;
 } TCount;
@@ -567,6 +575,8 @@ static uint32_t closest(const orc_scene* s, V o, V d, float tmax, int brute, flo
     *t_out = tbest;
     return best;
   }
+  /* Depth-first, nearer child first; every box is (re)tested with the
+   * current best distance when it is popped. */
   uint32_t stack[128];
   int sp = 0;
   if (s->n_tris) stack[sp++] = 0;
@@ -586,8 +596,17 @@ static uint32_t closest(const orc_scene* s, V o, V d, float tmax, int brute, flo
         }
       }
     } else {
-      stack[sp++] = (uint32_t)nd->index + 1;
-      stack[sp++] = (uint32_t)nd->index;
+      const ONode* l = &s->nodes[nd->index];
+      const ONode* rr = &s->nodes[nd->index + 1];
+      float tl = box_entry(&r, l->lo, l->hi);
+      float tr = box_entry(&r, rr->lo, rr->hi);
+      if (tl <= tr) {
+        stack[sp++] = (uint32_t)nd->index + 1;
+        stack[sp++] = (uint32_t)nd->index;
+      } else {
+        stack[sp++] = (uint32_t)nd->index;
+        stack[sp++] = (uint32_t)nd->index + 1;
+      }
     }
   }
   *t_out = tbest;
@@ -854,6 +873,15 @@ void orc_bsdf_sample(const float* mat, const float* n, const float* wo, float ul
   out10[9] = 0.0f;
 }
 
+void orc_ggx(float roughness, float cos_h, float cos_v, float* out2) {
+  OMat m;
+  memset(&m, 0, sizeof m);
+  m.roughness = roughness;
+  Bsdf b = bsdf_make(&m);
+  out2[0] = ggx_d(&b, cos_h);
+  out2[1] = smith_lambda(&b, cos_v);
+}
+
 uint32_t orc_light_table(const orc_scene* s, float* pmf, float* cdf, float* area, float* mat_pdf) {
   uint32_t n = s->n_area + s->n_point;
   for (uint32_t i = 0; i < n; ++i) {
@@ -897,6 +925,18 @@ static V camera_dir(const Cam* c, uint32_t px, uint32_t py, float u0, float u1) 
   V d = v((c->m[0] * cx + c->m[1] * cy) - c->m[2], (c->m[4] * cx + c->m[5] * cy) - c->m[6],
           (c->m[8] * cx + c->m[9] * cy) - c->m[10]);
   return vnorm(d);
+}
+
+void orc_camera_ray(const float* camera, uint32_t px, uint32_t py, float u0, float u1, float* out6) {
+  Cam cam;
+  cam_from16(camera, &cam);
+  V d = camera_dir(&cam, px, py, u0, u1);
+  out6[0] = cam.m[3];
+  out6[1] = cam.m[7];
+  out6[2] = cam.m[11];
+  out6[3] = d.x;
+  out6[4] = d.y;
+  out6[5] = d.z;
 }
 
 static uint32_t pick_light(const float* cdf, uint32_t n, float u) {

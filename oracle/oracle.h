@@ -80,6 +80,11 @@ void orc_bsdf_eval(const float* mat, const float* n, const float* wo, const floa
 /* out: wi xyz, f rgb, pdf, cos_i, ok. */
 void orc_bsdf_sample(const float* mat, const float* n, const float* wo, float ul, float u1, float u2,
                      float* out10);
+/* GGX normal distribution D(cos_h) and Smith Lambda(cos_v) for alpha =
+ * max(roughness^2, 1e-3). */
+void orc_ggx(float roughness, float cos_h, float cos_v, float* out2);
+/* Camera ray (origin xyz, unit direction xyz) of pixel (px, py) with jitter. */
+void orc_camera_ray(const float* camera, uint32_t px, uint32_t py, float u0, float u1, float* out6);
 /* Light table as the integrator uses it: pmf / cdf per light, area per
  * emissive triangle, light pdf per material. Returns the light count. */
 uint32_t orc_light_table(const orc_scene* s, float* pmf, float* cdf, float* area, float* mat_pdf);

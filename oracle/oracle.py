@@ -46,6 +46,8 @@ _SIGS = {
     "orc_bsdf_eval": (None, [_P, _P, _P, _P, _P]),
     "orc_bsdf_sample": (None, [_P, _P, _P, C.c_float, C.c_float, C.c_float, _P]),
     "orc_light_table": (C.c_uint32, [_P, _P, _P, _P, _P]),
+    "orc_ggx": (None, [C.c_float, C.c_float, C.c_float, _P]),
+    "orc_camera_ray": (None, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, _P]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -173,6 +175,18 @@ def sincos_2pi(u):
 def bsdf_eval(mat12, n, wo, wi):
     out = np.zeros(5, np.float32)
     lib().orc_bsdf_eval(_p(f32(mat12)), _p(f32(n)), _p(f32(wo)), _p(f32(wi)), _p(out))
+    return out
+
+
+def ggx(roughness, cos_h, cos_v):
+    out = np.zeros(2, np.float32)
+    lib().orc_ggx(roughness, cos_h, cos_v, _p(out))
+    return float(out[0]), float(out[1])
+
+
+def camera_ray(camera16, px, py, u0, u1):
+    out = np.zeros(6, np.float32)
+    lib().orc_camera_ray(_p(f32(camera16)), px, py, u0, u1, _p(out))
     return out
 
 
