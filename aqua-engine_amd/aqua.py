@@ -249,11 +249,11 @@ class Scene:
         return a
 
     def bvh(self):
-        """(nodes float32[n,16] in the device layout, order uint32[ntri], max_depth)."""
+        """(nodes float32[n,32] in the device's 4-wide layout, order uint32[ntri], max_depth)."""
         n = C.c_uint32(0)
         d = C.c_uint32(0)
         _check(lib().aqua_scene_bvh(self._h, C.byref(n), None, None, C.byref(d)))
-        nodes = np.zeros((n.value, 16), np.float32)
+        nodes = np.zeros((n.value, 32), np.float32)
         order = np.zeros(self.info().n_triangles, np.uint32)
         _check(lib().aqua_scene_bvh(self._h, C.byref(n), _ptr(nodes), _ptr(order), C.byref(d)))
         return nodes, order, d.value

@@ -19,7 +19,9 @@ constexpr uint32_t kMaxBounces = 64;
 // counters[] layout
 constexpr uint32_t kCntRays = 0;        // [0, 65): rays in the queue of bounce b
 constexpr uint32_t kCntShadow = 80;     // [80, 144): shadow rays emitted at bounce b
-constexpr uint32_t kCntWords = 160;
+constexpr uint32_t kCntFetch = 160;     // [160, 225): closest-hit work cursor of bounce b
+constexpr uint32_t kCntFetchSh = 240;   // [240, 304): any-hit work cursor of bounce b
+constexpr uint32_t kCntWords = 320;
 // stats64[] layout
 enum : uint32_t {
   kStNodes = 0,

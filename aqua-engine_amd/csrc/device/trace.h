@@ -1,4 +1,4 @@
-// Ray traversal of the BVH2 and the watertight ray/triangle test.
+// Ray traversal of the 4-wide BVH and the watertight ray/triangle test.
 //
 // Closest hit is the least (t, triangle id) pair over every triangle the
 // watertight test accepts with 0 < t < tmax. Node culling is conservative
@@ -81,7 +81,7 @@ AQ_DEV bool tri_hit(const RayPre& r, v3 v0, v3 v1, v3 v2, float* t_out) {
   return true;
 }
 
-// Slab test of one child box; returns the entry distance or +inf if culled.
+// Slab test of one box; returns the entry distance, or +inf when culled.
 AQ_DEV float box_enter(const RayPre& r, float lx, float hx, float ly, float hy, float lz, float hz,
                        float tbest, float delta) {
   float t0x = __builtin_fmaf(lx, r.inv.x, -r.oi.x), t1x = __builtin_fmaf(hx, r.inv.x, -r.oi.x);
@@ -93,7 +93,7 @@ AQ_DEV float box_enter(const RayPre& r, float lx, float hx, float ly, float hy, 
 }
 
 struct DevBvh {
-  const float4* nodes;  // 4 per node
+  const float4* nodes;  // 8 per 4-wide node
   const float4* tris;   // 3 per triangle slot: (v0, id), (v1, material), (v2, 0)
   float delta;
 };
@@ -102,63 +102,132 @@ struct TraceCount {
   uint32_t nodes, tris;
 };
 
-constexpr int kStackSize = 64;
+// ---- traversal stack: a 16-entry ring in LDS (one column per thread) whose
+// oldest entries spill to a private array when it is full. With at most three
+// pushes per visited node the host bounds the total depth (3 x tree depth)
+// by kLdsStack + kSpillStack.
+constexpr int kLdsStack = 16;
+constexpr int kSpillStack = 64;
+
+// Only scalars live in the struct, so they stay in registers; the spill
+// array is a separate private array owned by the kernel.
+struct TStack {
+  uint32_t* lds;    // this thread's column
+  uint32_t* spill;  // kSpillStack entries of private memory
+  int stride;       // threads per block
+  int top, cnt, gsp;
+};
+
+AQ_DEV void st_init(TStack& s, uint32_t* lds_base, uint32_t* spill, int tid, int stride) {
+  s.lds = lds_base + tid;
+  s.spill = spill;
+  s.stride = stride;
+  s.top = kLdsStack - 1;
+  s.cnt = 0;
+  s.gsp = 0;
+}
+AQ_DEV void st_push(TStack& s, uint32_t v) {
+  int slot = s.top + 1 == kLdsStack ? 0 : s.top + 1;
+  if (s.cnt == kLdsStack)
+    s.spill[s.gsp++] = s.lds[slot * s.stride];
+  else
+    ++s.cnt;
+  s.lds[slot * s.stride] = v;
+  s.top = slot;
+}
+AQ_DEV bool st_empty(const TStack& s) { return s.cnt == 0 && s.gsp == 0; }
+AQ_DEV uint32_t st_pop(TStack& s) {
+  if (s.cnt > 0) {
+    uint32_t v = s.lds[s.top * s.stride];
+    s.top = s.top == 0 ? kLdsStack - 1 : s.top - 1;
+    --s.cnt;
+    return v;
+  }
+  return s.spill[--s.gsp];
+}
+
+AQ_DEV int sel4(int4 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+
+// Four slab tests of one node. Empty children get +inf.
+AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float delta, float tn[4], int4* child,
+                       int4* count) {
+  float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+  *child = *reinterpret_cast<const int4*>(np + 6);
+  *count = *reinterpret_cast<const int4*>(np + 7);
+  tn[0] = child->x >= 0 ? box_enter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tbest, delta) : __builtin_inff();
+  tn[1] = child->y >= 0 ? box_enter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tbest, delta) : __builtin_inff();
+  tn[2] = child->z >= 0 ? box_enter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tbest, delta) : __builtin_inff();
+  tn[3] = child->w >= 0 ? box_enter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tbest, delta) : __builtin_inff();
+}
+
+AQ_DEV void cswap(uint32_t& a, uint32_t& b) {
+  uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo;
+  b = hi;
+}
 
 // Closest hit. Returns the triangle slot (0xFFFFFFFF on a miss) and t.
 template <bool COUNT>
-AQ_DEV uint32_t trace_closest(const DevBvh& bvh, v3 o, v3 d, float tmax, float* t_out,
+AQ_DEV uint32_t trace_closest(const DevBvh& bvh, TStack& st, v3 o, v3 d, float tmax, float* t_out,
                               TraceCount* cnt) {
   RayPre r = ray_pre(o, d);
   float tbest = tmax;
   uint32_t best_id = 0xFFFFFFFFu, best_slot = 0xFFFFFFFFu;
-  int stack[kStackSize];
-  int sp = 0;
-  int node = 0;
+  uint32_t node = 0;
   while (true) {
-    const float4* np = bvh.nodes + 4 * node;
-    float4 a = np[0], b = np[1], c = np[2];
-    int4 e = *reinterpret_cast<const int4*>(np + 3);
+    float tn[4];
+    int4 child, count;
+    node_test4(r, bvh.nodes + 8 * node, tbest, bvh.delta, tn, &child, &count);
     if (COUNT) cnt->nodes++;
-    float tl = e.x >= 0 ? box_enter(r, a.x, a.y, a.z, a.w, c.x, c.y, tbest, bvh.delta) : __builtin_inff();
-    float tr = e.y >= 0 ? box_enter(r, b.x, b.y, b.z, b.w, c.z, c.w, tbest, bvh.delta) : __builtin_inff();
-    bool hl = tl != __builtin_inff(), hr = tr != __builtin_inff();
-    // Leaf children are tested right away.
-    for (int side = 0; side < 2; ++side) {
-      bool h = side == 0 ? hl : hr;
-      int cntc = side == 0 ? e.z : e.w;
-      if (!h || cntc == 0) continue;
-      int first = side == 0 ? e.x : e.y;
-      for (int k = 0; k < cntc; ++k) {
-        const float4* tp = bvh.tris + 3 * (first + k);
-        float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-        if (COUNT) cnt->tris++;
-        float t;
-        if (tri_hit(r, xyz(p0), xyz(p1), xyz(p2), &t)) {
-          uint32_t id = __float_as_uint(p0.w);
-          if (t > 0.0f && t < tmax && (t < tbest || (t == tbest && id < best_id))) {
-            tbest = t;
-            best_id = id;
-            best_slot = uint32_t(first + k);
+    // Order hit children near to far: key = entry distance bits with the
+    // child index in the low two bits (entry distances are >= 0).
+    uint32_t key[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      key[k] = tn[k] != __builtin_inff() ? ((__float_as_uint(tn[k]) & ~3u) | uint32_t(k)) : 0xFFFFFFFFu;
+    cswap(key[0], key[1]);
+    cswap(key[2], key[3]);
+    cswap(key[0], key[2]);
+    cswap(key[1], key[3]);
+    cswap(key[1], key[2]);
+    // Leaves are tested in near-to-far order; the nearest inner child is
+    // visited next and the other inner children are pushed far-first.
+    // (Indices stay compile-time so nothing lands in scratch memory.)
+    int cj[4];
+    bool inner[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bool valid = key[j] != 0xFFFFFFFFu;
+      int k = int(key[j] & 3u);
+      int c = sel4(child, k), n = sel4(count, k);
+      cj[j] = c;
+      inner[j] = valid && n == 0;
+      if (valid && n > 0) {
+        for (int s = c; s < c + n; ++s) {
+          const float4* tp = bvh.tris + 3 * s;
+          float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+          if (COUNT) cnt->tris++;
+          float t;
+          if (tri_hit(r, xyz(p0), xyz(p1), xyz(p2), &t)) {
+            uint32_t id = __float_as_uint(p0.w);
+            if (t > 0.0f && t < tmax && (t < tbest || (t == tbest && id < best_id))) {
+              tbest = t;
+              best_id = id;
+              best_slot = uint32_t(s);
+            }
           }
         }
       }
-      if (side == 0)
-        hl = false;
-      else
-        hr = false;
     }
-    if (hl && hr) {
-      bool left_first = tl <= tr;
-      int near = left_first ? e.x : e.y, far = left_first ? e.y : e.x;
-      stack[sp++] = far;
-      node = near;
-    } else if (hl) {
-      node = e.x;
-    } else if (hr) {
-      node = e.y;
+    int first = inner[0] ? 0 : (inner[1] ? 1 : (inner[2] ? 2 : (inner[3] ? 3 : -1)));
+#pragma unroll
+    for (int j = 3; j >= 1; --j)
+      if (inner[j] && j != first) st_push(st, uint32_t(cj[j]));
+    if (first >= 0) {
+      node = uint32_t(first == 0 ? cj[0] : (first == 1 ? cj[1] : (first == 2 ? cj[2] : cj[3])));
     } else {
-      if (sp == 0) break;
-      node = stack[--sp];
+      if (st_empty(st)) break;
+      node = st_pop(st);
     }
   }
   *t_out = tbest;
@@ -167,47 +236,43 @@ AQ_DEV uint32_t trace_closest(const DevBvh& bvh, v3 o, v3 d, float tmax, float* 
 
 // Any hit in (0, tmax).
 template <bool COUNT>
-AQ_DEV bool trace_any(const DevBvh& bvh, v3 o, v3 d, float tmax, TraceCount* cnt) {
+AQ_DEV bool trace_any(const DevBvh& bvh, TStack& st, v3 o, v3 d, float tmax, TraceCount* cnt) {
   RayPre r = ray_pre(o, d);
-  int stack[kStackSize];
-  int sp = 0;
-  int node = 0;
+  uint32_t node = 0;
   while (true) {
-    const float4* np = bvh.nodes + 4 * node;
-    float4 a = np[0], b = np[1], c = np[2];
-    int4 e = *reinterpret_cast<const int4*>(np + 3);
+    float tn[4];
+    int4 child, count;
+    node_test4(r, bvh.nodes + 8 * node, tmax, bvh.delta, tn, &child, &count);
     if (COUNT) cnt->nodes++;
-    float tl = e.x >= 0 ? box_enter(r, a.x, a.y, a.z, a.w, c.x, c.y, tmax, bvh.delta) : __builtin_inff();
-    float tr = e.y >= 0 ? box_enter(r, b.x, b.y, b.z, b.w, c.z, c.w, tmax, bvh.delta) : __builtin_inff();
-    bool hl = tl != __builtin_inff(), hr = tr != __builtin_inff();
-    for (int side = 0; side < 2; ++side) {
-      bool h = side == 0 ? hl : hr;
-      int cntc = side == 0 ? e.z : e.w;
-      if (!h || cntc == 0) continue;
-      int first = side == 0 ? e.x : e.y;
-      for (int k = 0; k < cntc; ++k) {
-        const float4* tp = bvh.tris + 3 * (first + k);
-        float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-        if (COUNT) cnt->tris++;
-        float t;
-        if (tri_hit(r, xyz(p0), xyz(p1), xyz(p2), &t) && t > 0.0f && t < tmax) return true;
+    int next = -1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (tn[k] == __builtin_inff()) continue;
+      int c = sel4(child, k), n = sel4(count, k);
+      if (n > 0) {
+        for (int s = c; s < c + n; ++s) {
+          const float4* tp = bvh.tris + 3 * s;
+          float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+          if (COUNT) cnt->tris++;
+          float t;
+          if (tri_hit(r, xyz(p0), xyz(p1), xyz(p2), &t) && t > 0.0f && t < tmax) {
+            // drain the stack so the caller's next ray starts empty
+            st.cnt = 0;
+            st.gsp = 0;
+            return true;
+          }
+        }
+      } else if (next < 0) {
+        next = c;
+      } else {
+        st_push(st, uint32_t(c));
       }
-      if (side == 0)
-        hl = false;
-      else
-        hr = false;
     }
-    if (hl && hr) {
-      bool left_first = tl <= tr;
-      stack[sp++] = left_first ? e.y : e.x;
-      node = left_first ? e.x : e.y;
-    } else if (hl) {
-      node = e.x;
-    } else if (hr) {
-      node = e.y;
+    if (next >= 0) {
+      node = uint32_t(next);
     } else {
-      if (sp == 0) break;
-      node = stack[--sp];
+      if (st_empty(st)) break;
+      node = st_pop(st);
     }
   }
   return false;

@@ -85,13 +85,15 @@ int aqua_scene_bvh(const aqua_scene* s, uint32_t* n_nodes, float* nodes, uint32_
     aqua::Bvh b;
     aqua::BvhBuildOpts o;
     aqua::build_bvh(s->verts.data(), uint32_t(s->mat_ids.size()), o, &b);
+    aqua::Bvh4 b4;
+    aqua::collapse_bvh4(b, &b4);
     if (nodes) {
-      if (*n_nodes < b.nodes.size()) return aqua::fail(AQUA_ERR_INVALID_ARG, "node buffer too small");
-      memcpy(nodes, b.nodes.data(), b.nodes.size() * sizeof(aqua::BvhNode));
+      if (*n_nodes < b4.nodes.size()) return aqua::fail(AQUA_ERR_INVALID_ARG, "node buffer too small");
+      memcpy(nodes, b4.nodes.data(), b4.nodes.size() * sizeof(aqua::Bvh4Node));
       if (order) memcpy(order, b.tri_order.data(), b.tri_order.size() * sizeof(uint32_t));
     }
-    *n_nodes = uint32_t(b.nodes.size());
-    if (max_depth) *max_depth = b.max_depth;
+    *n_nodes = uint32_t(b4.nodes.size());
+    if (max_depth) *max_depth = b4.max_inner_depth;
     return AQUA_OK;
   })
 }

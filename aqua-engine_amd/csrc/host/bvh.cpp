@@ -284,4 +284,92 @@ void build_bvh(const float* verts, uint32_t n, const BvhBuildOpts& opts, Bvh* ou
   out->sah_cost = cost;
 }
 
+namespace {
+
+struct Ref {
+  float lo[3], hi[3];
+  int32_t child, count;
+  double area() const {
+    double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+  }
+};
+
+void refs_of(const BvhNode& n, std::vector<Ref>* out) {
+  for (int side = 0; side < 2; ++side) {
+    if (n.child[side] < 0) continue;
+    Ref r;
+    r.lo[0] = n.f[side * 4 + 0];
+    r.hi[0] = n.f[side * 4 + 1];
+    r.lo[1] = n.f[side * 4 + 2];
+    r.hi[1] = n.f[side * 4 + 3];
+    r.lo[2] = n.f[8 + side * 2];
+    r.hi[2] = n.f[9 + side * 2];
+    r.child = n.child[side];
+    r.count = n.count[side];
+    out->push_back(r);
+  }
+}
+
+uint32_t collapse_rec(const Bvh& bin, std::vector<Ref> refs, uint32_t depth, Bvh4* out) {
+  while (refs.size() < 4) {
+    int best = -1;
+    double best_area = -1.0;
+    for (size_t i = 0; i < refs.size(); ++i)
+      if (refs[i].count == 0 && refs[i].child >= 0 && refs[i].area() > best_area) {
+        best_area = refs[i].area();
+        best = int(i);
+      }
+    if (best < 0) break;
+    std::vector<Ref> sub;
+    refs_of(bin.nodes[size_t(refs[size_t(best)].child)], &sub);
+    refs.erase(refs.begin() + best);
+    if (refs.size() + sub.size() > 4) {
+      // cannot happen (4 - 1 + 2 > 4 only when size was 4)
+      refs.insert(refs.begin() + best, sub.begin(), sub.end());
+      break;
+    }
+    refs.insert(refs.begin() + best, sub.begin(), sub.end());
+  }
+  uint32_t id = uint32_t(out->nodes.size());
+  out->nodes.emplace_back();
+  out->max_inner_depth = std::max(out->max_inner_depth, depth);
+  Bvh4Node node;
+  memset(&node, 0, sizeof node);
+  for (int k = 0; k < 4; ++k) {
+    node.child[k] = -1;
+    node.count[k] = 0;
+  }
+  for (size_t k = 0; k < refs.size(); ++k) {
+    const Ref& r = refs[k];
+    node.lo_x[k] = r.lo[0];
+    node.hi_x[k] = r.hi[0];
+    node.lo_y[k] = r.lo[1];
+    node.hi_y[k] = r.hi[1];
+    node.lo_z[k] = r.lo[2];
+    node.hi_z[k] = r.hi[2];
+    if (r.count > 0) {
+      node.child[k] = r.child;
+      node.count[k] = r.count;
+    } else {
+      std::vector<Ref> sub;
+      refs_of(bin.nodes[size_t(r.child)], &sub);
+      node.child[k] = int32_t(collapse_rec(bin, sub, depth + 1, out));
+      node.count[k] = 0;
+    }
+  }
+  out->nodes[id] = node;
+  return id;
+}
+
+}  // namespace
+
+void collapse_bvh4(const Bvh& bin, Bvh4* out) {
+  *out = Bvh4();
+  out->nodes.reserve(bin.nodes.size() / 2 + 1);
+  std::vector<Ref> refs;
+  if (!bin.nodes.empty()) refs_of(bin.nodes[0], &refs);
+  collapse_rec(bin, refs, 0, out);
+}
+
 }  // namespace aqua

@@ -39,4 +39,25 @@ struct Bvh {
 // verts: 9 floats per triangle.
 void build_bvh(const float* verts, uint32_t n_tris, const BvhBuildOpts& opts, Bvh* out);
 
+// 4-wide node, 128 bytes: the boxes of up to four children as six float4
+// rows (lo.x[4], hi.x[4], lo.y[4], hi.y[4], lo.z[4], hi.z[4]) so one node
+// fetch feeds four slab tests, then child[4] and count[4] with the same
+// meaning as in BvhNode (count > 0 leaf, count == 0 inner, child < 0 empty).
+struct Bvh4Node {
+  float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+  int32_t child[4];
+  int32_t count[4];
+};
+static_assert(sizeof(Bvh4Node) == 128, "node layout");
+
+struct Bvh4 {
+  std::vector<Bvh4Node> nodes;  // node 0 is the root, preorder
+  uint32_t max_inner_depth = 0; // deepest inner node (root = 0)
+};
+
+// Collapses a binary tree into 4-wide nodes: each node repeatedly opens its
+// inner child of largest surface area until it has four children (the
+// usual greedy SAH-preserving collapse). Triangle slots are unchanged.
+void collapse_bvh4(const Bvh& bin, Bvh4* out);
+
 }  // namespace aqua

@@ -86,18 +86,35 @@ __global__ void k_camera(PassParams pp, Queues q) {
   }
 }
 
+// Persistent waves: each wave takes the next 64 rays from a work cursor
+// until the queue is exhausted, so long rays do not hold up a static share.
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t* cursor) {
+  uint32_t base = 0;
+  if (__lane_id() == 0) base = atomicAdd(cursor, 64u);
+  return __shfl(base, 0);
+}
+
 template <bool COUNT>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevBvh bvh, const float4* __restrict__ ro,
                                                       const float4* __restrict__ rd,
-                                                      const uint32_t* count, uint2* hit,
+                                                      const uint32_t* count, uint32_t* cursor, uint2* hit,
                                                       unsigned long long* stats) {
-  uint32_t n = *count;
+  __shared__ uint32_t lds_stack[kLdsStack * kBlock];
+  uint32_t spill[kSpillStack];
+  TStack st;
+  st_init(st, lds_stack, spill, threadIdx.x, kBlock);
+  const uint32_t n = *count;
   TraceCount tc{0u, 0u};
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float4 o = ro[i], d = rd[i];
-    float t;
-    uint32_t slot = trace_closest<COUNT>(bvh, xyz(o), xyz(d), o.w, &t, &tc);
-    hit[i] = make_uint2(slot, __float_as_uint(t));
+  while (true) {
+    uint32_t base = wave_fetch(cursor);
+    if (base >= n) break;
+    uint32_t i = base + __lane_id();
+    if (i < n) {
+      float4 o = ro[i], d = rd[i];
+      float t;
+      uint32_t slot = trace_closest<COUNT>(bvh, st, xyz(o), xyz(d), o.w, &t, &tc);
+      hit[i] = make_uint2(slot, __float_as_uint(t));
+    }
   }
   if (COUNT) {
     atomicAdd(&stats[kStNodes], (unsigned long long)tc.nodes);
@@ -280,12 +297,21 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
 
 template <bool COUNT>
 __global__ void __launch_bounds__(kBlock) k_shadow(DevBvh bvh, Queues q, uint32_t b) {
+  __shared__ uint32_t lds_stack[kLdsStack * kBlock];
+  uint32_t spill[kSpillStack];
+  TStack st;
+  st_init(st, lds_stack, spill, threadIdx.x, kBlock);
   const uint32_t nx = (b & 1u) ^ 1u;
-  uint32_t n = q.counters[kCntShadow + b];
+  const uint32_t n = q.counters[kCntShadow + b];
+  uint32_t* cursor = q.counters + kCntFetchSh + b;
   TraceCount tc{0u, 0u};
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  while (true) {
+    uint32_t base = wave_fetch(cursor);
+    if (base >= n) break;
+    uint32_t i = base + __lane_id();
+    if (i >= n) continue;
     float4 o = q.sho[i], d = q.shd[i];
-    if (!trace_any<COUNT>(bvh, xyz(o), xyz(d), o.w, &tc)) {
+    if (!trace_any<COUNT>(bvh, st, xyz(o), xyz(d), o.w, &tc)) {
       float4 cc = q.shc[i];
       uint32_t dst = __float_as_uint(d.w);
       float4* Lp = (dst & 0x80000000u) ? &q.Lbuf[dst & 0x7FFFFFFFu] : &q.L[nx][dst];
@@ -362,12 +388,17 @@ __global__ void k_hits_out(const uint2* hit, const float4* tris, uint32_t n, uin
   ts[i] = __uint_as_float(h.y);
 }
 
-__global__ void k_occluded(DevBvh bvh, const float4* ro, const float4* rd, uint32_t n, uint8_t* out) {
+__global__ void __launch_bounds__(kBlock) k_occluded(DevBvh bvh, const float4* ro, const float4* rd,
+                                                     uint32_t n, uint8_t* out) {
+  __shared__ uint32_t lds_stack[kLdsStack * kBlock];
+  uint32_t spill[kSpillStack];
+  TStack st;
+  st_init(st, lds_stack, spill, threadIdx.x, kBlock);
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   TraceCount tc;
   float4 o = ro[i], d = rd[i];
-  out[i] = trace_any<false>(bvh, xyz(o), xyz(d), o.w, &tc) ? 1 : 0;
+  out[i] = trace_any<false>(bvh, st, xyz(o), xyz(d), o.w, &tc) ? 1 : 0;
 }
 
 // relMSE / relRMSE partial sums in FP64, one triple per block.
@@ -480,6 +511,7 @@ struct aqua_renderer {
   aqua_renderer_opts opts{};
   uint32_t n_tris = 0, n_area = 0, n_lights = 0;
   float delta = 0.0f;
+  int persistent_blocks = 2048;  // resident blocks of the traversal kernels
   std::vector<uint32_t> tri_order;
   DevBuf<float4> tris, nodes, mats, lights;
   DevBuf<float> cdf, pmf;
@@ -591,7 +623,7 @@ int check_camera(const aqua_camera* c) {
   return AQUA_OK;
 }
 
-int select_device(int device) {
+int select_device(int device, int* cu_count = nullptr) {
   if (device < 0)
     return aqua::fail(AQUA_ERR_INVALID_ARG,
                       "device %d: the renderer runs on a GPU only (there is no CPU path)", device);
@@ -602,6 +634,7 @@ int select_device(int device) {
     return aqua::fail(AQUA_ERR_NO_DEVICE, "device %d out of range (%d devices)", device, count);
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (cu_count) *cu_count = prop.multiProcessorCount;
   if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return aqua::fail(AQUA_ERR_NO_DEVICE, "device %d is %s; this build targets gfx950", device,
                       prop.gcnArchName);
@@ -616,34 +649,36 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   DevScene sc = r->dev_scene();
   Queues q = r->queues();
   int g = grid_for(n);
+  int gp = std::min(g, r->persistent_blocks);
   k_reset<<<1, kCntWords, 0, s>>>(q.counters, n);
   k_camera<<<g, kBlock, 0, s>>>(pp, q);
   bool timing = r->opts.timing != 0;
   for (uint32_t b = 0; b <= pp.max_depth; ++b) {
     uint32_t c = b & 1u;
     const uint32_t* cnt = q.counters + kCntRays + b;
+    uint32_t* cur = q.counters + kCntFetch + b;
     hipEvent_t* ev = &r->kev[4 * b];
     if (timing) HIPCHK(hipEventRecord(ev[0], s));
     if (count)
-      k_intersect<true><<<g, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, q.hit, q.stats);
+      k_intersect<true><<<gp, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, cur, q.hit, q.stats);
     else
-      k_intersect<false><<<g, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, q.hit, q.stats);
+      k_intersect<false><<<gp, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, cur, q.hit, q.stats);
     if (timing) HIPCHK(hipEventRecord(ev[1], s));
     k_shade<<<g, kBlock, 0, s>>>(sc, pp, b, q);
     if (timing) HIPCHK(hipEventRecord(ev[2], s));
     if (b < pp.max_depth) {
       if (count)
-        k_shadow<true><<<g, kBlock, 0, s>>>(sc.bvh, q, b);
+        k_shadow<true><<<gp, kBlock, 0, s>>>(sc.bvh, q, b);
       else
-        k_shadow<false><<<g, kBlock, 0, s>>>(sc.bvh, q, b);
+        k_shadow<false><<<gp, kBlock, 0, s>>>(sc.bvh, q, b);
     }
     if (timing) HIPCHK(hipEventRecord(ev[3], s));
   }
-  int gp = int((pp.npix + kBlock - 1) / kBlock);
+  int gf = int((pp.npix + kBlock - 1) / kBlock);
   if (fp64)
-    k_film64<<<gp, kBlock, 0, s>>>(pp, q.Lbuf, r->film64.p, q.stats);
+    k_film64<<<gf, kBlock, 0, s>>>(pp, q.Lbuf, r->film64.p, q.stats);
   else
-    k_film32<<<gp, kBlock, 0, s>>>(pp, q.Lbuf, r->film.p, q.stats);
+    k_film32<<<gf, kBlock, 0, s>>>(pp, q.Lbuf, r->film.p, q.stats);
   HIPCHK(hipGetLastError());
   // Per-pass counters (ray counts) are read back once the pass is queued;
   // the copy waits on the stream, so passes stay ordered.
@@ -719,9 +754,13 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   *out = nullptr;
   if (cfg->max_depth >= kMaxBounces)
     return aqua::fail(AQUA_ERR_INVALID_ARG, "max_depth %u exceeds %u", cfg->max_depth, kMaxBounces - 1);
-  if (int e = select_device(device)) return e;
+  int cus = 0;
+  if (int e = select_device(device, &cus)) return e;
   auto r = std::make_unique<aqua_renderer>();
   r->device = device;
+  // 8 blocks of 256 per CU keep every SIMD busy at the traversal kernels'
+  // register and LDS footprint.
+  r->persistent_blocks = std::max(1, cus) * 8;
   r->cfg = *cfg;
   r->camera = scene->camera;
   r->opts.timing = 1;
@@ -736,8 +775,12 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   aqua::Bvh bvh;
   aqua::BvhBuildOpts bo;
   aqua::build_bvh(scene->verts.data(), n, bo, &bvh);
-  if (bvh.max_depth + 2 >= uint32_t(kStackSize))
-    return aqua::fail(AQUA_ERR_UNSUPPORTED, "BVH depth %u exceeds the traversal stack", bvh.max_depth);
+  aqua::Bvh4 bvh4;
+  aqua::collapse_bvh4(bvh, &bvh4);
+  // At most three entries are pushed per visited node.
+  if (3 * (bvh4.max_inner_depth + 1) > uint32_t(kLdsStack + kSpillStack))
+    return aqua::fail(AQUA_ERR_UNSUPPORTED, "BVH depth %u exceeds the traversal stack",
+                      bvh4.max_inner_depth);
   std::vector<float4> tris(3 * size_t(n));
   for (uint32_t s = 0; s < n; ++s) {
     uint32_t id = bvh.tri_order[s];
@@ -752,8 +795,8 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   }
   r->n_tris = n;
   r->tri_order = bvh.tri_order;
-  std::vector<float4> nodes(4 * bvh.nodes.size());
-  memcpy(nodes.data(), bvh.nodes.data(), bvh.nodes.size() * sizeof(aqua::BvhNode));
+  std::vector<float4> nodes(8 * bvh4.nodes.size());
+  memcpy(nodes.data(), bvh4.nodes.data(), bvh4.nodes.size() * sizeof(aqua::Bvh4Node));
   // Culling slack: 2^-16 of the largest coordinate magnitude.
   float mx = 0.0f;
   for (int k = 0; k < 3; ++k) {
@@ -795,10 +838,10 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
     pmf.push_back(0.0f);
   }
   if (n == 0) {
-    // An empty scene still needs a valid root: one node with two empty children.
-    nodes.assign(4, make_float4(0, 0, 0, 0));
-    int32_t e4[4] = {-1, -1, 0, 0};
-    memcpy(&nodes[3], e4, 16);
+    // An empty scene still needs a valid root: one node, all children empty.
+    nodes.assign(8, make_float4(0, 0, 0, 0));
+    int32_t e4[4] = {-1, -1, -1, -1};
+    memcpy(&nodes[6], e4, 16);
     tris.assign(3, make_float4(0, 0, 0, 0));
   }
   hipStream_t s = r->stream;
@@ -969,7 +1012,8 @@ int aqua_primary_hits(aqua_renderer* r, const aqua_camera* cam_in, uint32_t fram
   hipStream_t s = r->stream;
   k_reset<<<1, kCntWords, 0, s>>>(q.counters, n);
   k_camera<<<grid_for(n), kBlock, 0, s>>>(pp, q);
-  k_intersect<false><<<grid_for(n), kBlock, 0, s>>>(sc.bvh, q.ro[0], q.rd[0], q.counters, q.hit, q.stats);
+  k_intersect<false><<<std::min(grid_for(n), r->persistent_blocks), kBlock, 0, s>>>(
+      sc.bvh, q.ro[0], q.rd[0], q.counters, q.counters + kCntFetch, q.hit, q.stats);
   k_hits_out<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.hit, sc.bvh.tris, n, ids.p, ts.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(prim_ids, ids.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1010,7 +1054,8 @@ int aqua_trace_closest(aqua_renderer* r, uint32_t n, const float* rays, uint32_t
   if (int e = ts.ensure(n)) return e;
   hipStream_t s = r->stream;
   k_reset<<<1, kCntWords, 0, s>>>(q.counters, n);
-  k_intersect<false><<<grid_for(n), kBlock, 0, s>>>(sc.bvh, q.ro[0], q.rd[0], q.counters, q.hit, q.stats);
+  k_intersect<false><<<std::min(grid_for(n), r->persistent_blocks), kBlock, 0, s>>>(
+      sc.bvh, q.ro[0], q.rd[0], q.counters, q.counters + kCntFetch, q.hit, q.stats);
   k_hits_out<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.hit, sc.bvh.tris, n, ids.p, ts.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(prim_ids, ids.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));

@@ -244,10 +244,12 @@ int aqua_abi_version(void);
 int aqua_device_count(int* count);
 
 /* Diagnostics: the BVH the renderer builds for `scene`, in its device
- * layout (64-byte nodes: child boxes, child index, child triangle count;
- * count > 0 = leaf over triangle slots [index, index + count), count == 0 =
- * inner node `index`, index < 0 = empty). Call with nodes == NULL to get
- * *n_nodes, then again with room for that many; order[slot] = triangle id. */
+ * layout: 128-byte 4-wide nodes of 32 floats -- lo.x[4], hi.x[4], lo.y[4],
+ * hi.y[4], lo.z[4], hi.z[4], then int32 index[4] and count[4]; count > 0 =
+ * leaf over triangle slots [index, index + count), count == 0 = inner node
+ * `index`, index < 0 = empty. Call with nodes == NULL to get *n_nodes, then
+ * again with room for that many; order[slot] = triangle id; max_depth =
+ * depth of the deepest inner node. */
 int aqua_scene_bvh(const aqua_scene* scene, uint32_t* n_nodes, float* nodes, uint32_t* order,
                    uint32_t* max_depth);
 

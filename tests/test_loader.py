@@ -275,29 +275,44 @@ def test_interior_generator(tmp_path):
     assert np.all((mats[:, :3] >= 0.1) & (mats[:, :3] <= 0.9))
 
 
-def test_scene_bvh_invariants(box_room):
-    nodes, order, depth = box_room.bvh()
-    ntri = box_room.info().n_triangles
+def _check_bvh4(nodes, order, verts):
+    """Every triangle in exactly one leaf; every leaf triangle and every
+    child node inside its parent's box; nodes in preorder."""
+    ntri = len(verts)
     assert sorted(order.tolist()) == list(range(ntri))
-    idx = nodes[:, 12:14].view(np.int32)
-    cnt = nodes[:, 14:16].view(np.int32)
+    idx = nodes[:, 24:28].view(np.int32)
+    cnt = nodes[:, 28:32].view(np.int32)
     covered = np.zeros(ntri, int)
-    verts, _ = box_room.triangles()
     for n in range(len(nodes)):
-        for side in range(2):
-            if idx[n, side] < 0:
+        for k in range(4):
+            if idx[n, k] < 0:
                 continue
-            lo = nodes[n, [0 + 4 * side, 2 + 4 * side, 8 + 2 * side]]
-            hi = nodes[n, [1 + 4 * side, 3 + 4 * side, 9 + 2 * side]]
-            if cnt[n, side] > 0:
-                for s in range(idx[n, side], idx[n, side] + cnt[n, side]):
+            lo = nodes[n, [0 + k, 8 + k, 16 + k]]
+            hi = nodes[n, [4 + k, 12 + k, 20 + k]]
+            if cnt[n, k] > 0:
+                for s in range(idx[n, k], idx[n, k] + cnt[n, k]):
                     covered[s] += 1
                     p = verts[order[s]].reshape(3, 3)
                     assert np.all(p >= lo) and np.all(p <= hi)
             else:
-                child = idx[n, side]
+                child = idx[n, k]
                 assert child > n   # preorder
-                clo = np.minimum(nodes[child, [0, 2, 8]], nodes[child, [4, 6, 10]])
-                chi = np.maximum(nodes[child, [1, 3, 9]], nodes[child, [5, 7, 11]])
+                used = idx[child] >= 0
+                clo = np.stack([nodes[child, 0:4], nodes[child, 8:12], nodes[child, 16:20]], 1)[used]
+                chi = np.stack([nodes[child, 4:8], nodes[child, 12:16], nodes[child, 20:24]], 1)[used]
                 assert np.all(clo >= lo) and np.all(chi <= hi)
     assert np.all(covered == 1)
+
+
+def test_scene_bvh_invariants(box_room, tmp_path):
+    verts, _ = box_room.triangles()
+    nodes, order, depth = box_room.bvh()
+    _check_bvh4(nodes, order, verts)
+    p = aqua.procgen(aqua.PROCGEN_INTERIOR, str(tmp_path), seed=2, n_meshes=6, target_tris=4000,
+                     width=8, height=8)
+    sc = aqua.Scene.load(p)
+    verts, _ = sc.triangles()
+    nodes, order, depth = sc.bvh()
+    _check_bvh4(nodes, order, verts)
+    # 4-wide: far fewer nodes than triangles, and the tree stays shallow
+    assert len(nodes) < len(verts) / 2 and depth < 20
