@@ -100,6 +100,23 @@ class RendererOpts(C.Structure):
                 ("timing", C.c_uint32), ("reserved", C.c_uint32 * 5)]
 
 
+NRC_F16, NRC_BF16, NRC_FP8 = 0, 1, 2
+
+
+class NrcCfg(C.Structure):
+    _fields_ = [("n_levels", C.c_uint32), ("log2_table", C.c_uint32), ("base_res", C.c_uint32),
+                ("level_scale", C.c_float), ("learning_rate", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("train_precision", C.c_uint32),
+                ("seed", C.c_uint32), ("max_batch", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+
+
+class NrcStats(C.Structure):
+    _fields_ = [("ms_total", C.c_double), ("ms_fwd_bwd", C.c_double), ("ms_wgrad", C.c_double),
+                ("ms_adam", C.c_double), ("loss", C.c_float), ("steps", C.c_uint32),
+                ("rows", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+
+
+assert C.sizeof(NrcCfg) == 64 and C.sizeof(NrcStats) == 64
 assert C.sizeof(LoadOpts) == 32 and C.sizeof(Material) == 48 and C.sizeof(Camera) == 64
 assert C.sizeof(SceneInfo) == 64 and C.sizeof(ProcgenParams) == 32
 assert C.sizeof(IntegratorCfg) == 64 and C.sizeof(FrameStats) == 128 and C.sizeof(RendererOpts) == 32
@@ -136,6 +153,16 @@ _SIGS = {
     "aqua_trace_occluded": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "aqua_renderer_destroy": (None, [_P]),
     "aqua_fp_probe": (C.c_int, [C.c_uint32, _P, _P, _P, _P, _P]),
+    "aqua_nrc_default_cfg": (None, [C.POINTER(NrcCfg)]),
+    "aqua_nrc_create": (C.c_int, [C.c_int, C.POINTER(NrcCfg), C.POINTER(_P)]),
+    "aqua_nrc_destroy": (None, [_P]),
+    "aqua_nrc_param_counts": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "aqua_nrc_get_params": (C.c_int, [_P, _P, _P]),
+    "aqua_nrc_set_params": (C.c_int, [_P, _P, _P]),
+    "aqua_nrc_infer": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P]),
+    "aqua_nrc_train": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(NrcStats)]),
+    "aqua_nrc_train_device": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(NrcStats)]),
+    "aqua_nrc_gradients": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, C.POINTER(C.c_float)]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -355,6 +382,73 @@ class Renderer:
         occ = np.zeros(len(r), np.uint8)
         _check(lib().aqua_trace_occluded(self._h, len(r), _ptr(r), _ptr(occ)))
         return occ
+
+
+def nrc_default_cfg() -> NrcCfg:
+    c = NrcCfg()
+    lib().aqua_nrc_default_cfg(C.byref(c))
+    return c
+
+
+class Nrc:
+    """aqua_nrc: the neural radiance cache on one device."""
+
+    def __init__(self, cfg: Optional[NrcCfg] = None, device: int = 0):
+        self.cfg = cfg or nrc_default_cfg()
+        self._h = C.c_void_p()
+        _check(lib().aqua_nrc_create(device, C.byref(self.cfg), C.byref(self._h)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().aqua_nrc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def counts(self):
+        t, m = C.c_uint64(), C.c_uint32()
+        _check(lib().aqua_nrc_param_counts(self._h, C.byref(t), C.byref(m)))
+        return t.value, m.value
+
+    def params(self):
+        nt, nm = self.counts()
+        table = np.zeros(nt, np.float32)
+        mlp = np.zeros(nm, np.float32)
+        _check(lib().aqua_nrc_get_params(self._h, _ptr(table), _ptr(mlp)))
+        return table, mlp
+
+    def set_params(self, table, mlp):
+        table = np.ascontiguousarray(table, np.float32)
+        mlp = np.ascontiguousarray(mlp, np.float32)
+        _check(lib().aqua_nrc_set_params(self._h, _ptr(table), _ptr(mlp)))
+
+    def infer(self, records, precision=NRC_F16) -> np.ndarray:
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, 16)
+        out = np.zeros((len(rec), 3), np.float32)
+        _check(lib().aqua_nrc_infer(self._h, len(rec), _ptr(rec), precision, _ptr(out)))
+        return out
+
+    def gradients(self, records, targets):
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, 16)
+        tgt = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        nt, nm = self.counts()
+        gt = np.zeros(nt, np.float32)
+        gm = np.zeros(nm, np.float32)
+        loss = C.c_float()
+        _check(lib().aqua_nrc_gradients(self._h, len(rec), _ptr(rec), _ptr(tgt), _ptr(gt), _ptr(gm),
+                                        C.byref(loss)))
+        return loss.value, gt, gm
+
+    def train(self, records, targets, steps=1) -> NrcStats:
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, 16)
+        tgt = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        st = NrcStats()
+        _check(lib().aqua_nrc_train(self._h, len(rec), _ptr(rec), _ptr(tgt), steps, C.byref(st)))
+        return st
 
 
 def fp_probe(a, b, c, u) -> np.ndarray:

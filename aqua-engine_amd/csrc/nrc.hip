@@ -1,0 +1,1117 @@
+// Neural radiance cache on gfx950: hash-grid + SH encoding and a 64-wide MLP
+// on MFMA, fused forward / backward / hash-grid scatter, MFMA weight
+// gradients, Adam.
+//
+// Methods: multiresolution hash encoding (Mueller et al. 2022, "Instant
+// Neural Graphics Primitives"); relative L2 loss and online training
+// (Mueller et al. 2021, "Real-time Neural Radiance Caching for Path
+// Tracing"); Adam (Kingma & Ba 2015). The reference only names the
+// integrator (scenes/integrator.json:2 `"type": "nrc"`, :7 learning_rate).
+//
+// MFMA mapping (v_mfma_f32_16x16x32_{f16,bf16,fp8}, wave64). A wave owns 16
+// batch rows. Every layer is computed transposed, out^T = W^T x in^T, so the
+// batch row stays in lane%16 and a lane group g = lane/16 holds, after each
+// layer, the 16 outputs {16*mt + 4*g + j}. The K (input) dimension of every
+// layer is ordered by
+//     kappa(g, ks, i) = 16 * (2*ks + i/4) + 4*g + i%4
+// which is exactly the set a lane group holds, so the accumulator registers
+// of one layer are the B operand of the next with no data movement; the
+// permutation lives in the packed weights. The backward pass multiplies by
+// W (same packing of W^T) and lands in the same layout, so ReLU masks and
+// the hash-feature gradients are lane-local as well.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "aqua/aqua.h"
+#include "device/common.h"
+#include "host/internal.h"
+
+namespace {
+
+constexpr int kLevels = 16;
+constexpr int kHidden = 4;
+constexpr int kMlp = 4 * 64 * 64 + 64 * 16;    // 17408 floats
+constexpr int kFwdPack = kHidden * 8 * 64 + 2 * 64;  // A operands, forward
+constexpr int kBwdPack = kHidden * 8 * 64 + 4 * 64;  // A operands, backward
+constexpr int kBlock = 256;
+constexpr int kChunkRows = 4096;               // rows per weight-gradient partial
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+struct Grid {
+  float res[kLevels];
+  uint32_t dense_mask;
+  uint32_t tmask;
+  uint32_t log2T;
+};
+
+struct Scales {  // FP8 inference: per-layer weight and activation scales
+  float w[5];
+  float a[5];
+};
+
+__device__ __forceinline__ int kappa(int g, int ks, int i) { return 16 * (2 * ks + (i >> 2)) + 4 * g + (i & 3); }
+
+__device__ __forceinline__ uint32_t grid_index(const Grid& G, int lv, int cx, int cy, int cz) {
+  if ((G.dense_mask >> lv) & 1u) {
+    uint32_t n = uint32_t(G.res[lv]) + 1u;
+    return uint32_t(cx) + n * (uint32_t(cy) + n * uint32_t(cz));
+  }
+  return ((uint32_t(cx) * 1u) ^ (uint32_t(cy) * 2654435761u) ^ (uint32_t(cz) * 805459861u)) & G.tmask;
+}
+
+struct Cell {
+  int c[3];
+  float f[3];
+};
+
+__device__ __forceinline__ Cell cell_of(const Grid& G, int lv, float px, float py, float pz) {
+  float N = G.res[lv];
+  float s[3] = {px * N, py * N, pz * N};
+  Cell c;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float fl = floorf(s[d]);
+    if (fl > N - 1.0f) fl = N - 1.0f;
+    c.c[d] = int(fl);
+    c.f[d] = s[d] - fl;
+  }
+  return c;
+}
+
+__device__ __forceinline__ float corner_w(const Cell& c, int k) {
+  float wx = (k & 1) ? c.f[0] : 1.0f - c.f[0];
+  float wy = (k & 2) ? c.f[1] : 1.0f - c.f[1];
+  float wz = (k & 4) ? c.f[2] : 1.0f - c.f[2];
+  return (wx * wy) * wz;
+}
+
+// Two FP16 features of one table entry.
+__device__ __forceinline__ void load_entry(const uint32_t* table16, size_t at, float* a, float* b) {
+  uint32_t u = table16[at];
+  _Float16 lo = __builtin_bit_cast(_Float16, uint16_t(u & 0xFFFFu));
+  _Float16 hi = __builtin_bit_cast(_Float16, uint16_t(u >> 16));
+  *a = float(lo);
+  *b = float(hi);
+}
+
+// SH basis l = 0..3 (16 coefficients) of a unit direction.
+__device__ __forceinline__ void sh16(float x, float y, float z, float* o) {
+  float xx = x * x, yy = y * y, zz = z * z;
+  o[0] = 0.28209479177387814f;
+  o[1] = -0.48860251190291987f * y;
+  o[2] = 0.48860251190291987f * z;
+  o[3] = -0.48860251190291987f * x;
+  o[4] = 1.0925484305920792f * x * y;
+  o[5] = -1.0925484305920792f * y * z;
+  o[6] = 0.94617469575755997f * zz - 0.31539156525251999f;
+  o[7] = -1.0925484305920792f * x * z;
+  o[8] = 0.54627421529603959f * (xx - yy);
+  o[9] = 0.59004358992664352f * y * (-3.0f * xx + yy);
+  o[10] = 2.8906114426405538f * x * y * z;
+  o[11] = 0.45704579946446572f * y * (1.0f - 5.0f * zz);
+  o[12] = 0.3731763325901154f * z * (5.0f * zz - 3.0f);
+  o[13] = 0.45704579946446572f * x * (1.0f - 5.0f * zz);
+  o[14] = 1.4453057213202769f * z * (xx - yy);
+  o[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
+}
+
+// ---- MFMA operand helpers -------------------------------------------------
+
+template <int P>
+struct Op;
+
+template <>
+struct Op<AQUA_NRC_F16> {
+  typedef h8 T;
+  typedef uint4 Packed;
+  static __device__ __forceinline__ T make(const float* v, float) {
+    T r;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r[i] = (_Float16)v[i];
+    return r;
+  }
+  static __device__ __forceinline__ T unpack(uint4 u) { return __builtin_bit_cast(T, u); }
+  static __device__ __forceinline__ f4v mma(T a, T b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ uint16_t bits(float v) {
+    return __builtin_bit_cast(uint16_t, (_Float16)v);
+  }
+};
+
+template <>
+struct Op<AQUA_NRC_BF16> {
+  typedef b8 T;
+  typedef uint4 Packed;
+  static __device__ __forceinline__ T make(const float* v, float) {
+    T r;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r[i] = (__bf16)v[i];
+    return r;
+  }
+  static __device__ __forceinline__ T unpack(uint4 u) { return __builtin_bit_cast(T, u); }
+  static __device__ __forceinline__ f4v mma(T a, T b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ uint16_t bits(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
+};
+
+__device__ __forceinline__ float sat448(float v) { return v > 448.0f ? 448.0f : (v < -448.0f ? -448.0f : v); }
+
+template <>
+struct Op<AQUA_NRC_FP8> {
+  typedef long T;
+  typedef uint2 Packed;
+  // OCP e4m3 (gfx950), values pre-scaled and saturated to +-448.
+  static __device__ __forceinline__ T make(const float* v, float scale) {
+    int lo = 0, hi = 0;
+    lo = __builtin_amdgcn_cvt_pk_fp8_f32(sat448(v[0] * scale), sat448(v[1] * scale), lo, false);
+    lo = __builtin_amdgcn_cvt_pk_fp8_f32(sat448(v[2] * scale), sat448(v[3] * scale), lo, true);
+    hi = __builtin_amdgcn_cvt_pk_fp8_f32(sat448(v[4] * scale), sat448(v[5] * scale), hi, false);
+    hi = __builtin_amdgcn_cvt_pk_fp8_f32(sat448(v[6] * scale), sat448(v[7] * scale), hi, true);
+    return long((uint64_t(uint32_t(hi)) << 32) | uint32_t(lo));
+  }
+  static __device__ __forceinline__ T unpack(uint2 u) { return long((uint64_t(u.y) << 32) | u.x); }
+  static __device__ __forceinline__ f4v mma(T a, T b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, c, 0, 0, 0);
+  }
+};
+
+// ---- the fused kernel -------------------------------------------------------
+
+struct FwdArgs {
+  Grid G;
+  const float4* rec;        // 4 per row
+  const float4* tgt;        // 1 per row (training)
+  uint32_t n, npad;
+  const uint32_t* table16;  // [level][T] two FP16 features
+  const void* pack_fwd;     // A operands
+  const void* pack_bwd;
+  Scales sc;                // FP8 only
+  float* out;               // 3 per row (inference)
+  // training
+  float* table_grad;        // [level][T][2] FP32
+  uint16_t* HT;             // 5 x [64][npad] layer inputs (transposed)
+  uint16_t* DT;             // 4 x [64][npad] + [16][npad] pre-activation grads
+  float* loss_acc;
+  uint32_t* act_amax;       // 5 per-layer input amax (float bits)
+};
+
+__device__ __forceinline__ float wave_max(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+template <int P, bool TRAIN>
+__global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
+  typedef Op<P> O;
+  typedef typename O::T T;
+  typedef typename O::Packed Pk;
+  const Pk* pf = reinterpret_cast<const Pk*>(a.pack_fwd);
+  const int lane = int(__lane_id());
+  const int g = lane >> 4, r16 = lane & 15;
+  const uint32_t T_ = 1u << a.G.log2T;
+  const uint32_t tiles = a.npad / 16u;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  float loss_lane = 0.0f;
+  float amax_lane[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  for (uint32_t tile = wave; tile < tiles; tile += nwaves) {
+    const uint32_t row = tile * 16u + uint32_t(r16);
+    const bool valid = row < a.n;
+    float4 r0 = make_float4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
+    if (valid) {
+      r0 = a.rec[4 * size_t(row)];
+      r1 = a.rec[4 * size_t(row) + 1];
+      r2 = a.rec[4 * size_t(row) + 2];
+      r3 = a.rec[4 * size_t(row) + 3];
+    }
+    float px = fminf(fmaxf(r0.x, 0.0f), 1.0f), py = fminf(fmaxf(r0.y, 0.0f), 1.0f),
+          pz = fminf(fmaxf(r0.z, 0.0f), 1.0f);
+    // ---- encoding: this lane group's 16 features, f[ks*8 + i] = feature kappa(g, ks, i)
+    float f[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
+      Cell c = cell_of(a.G, lv, px, py, pz);
+      float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
+        float w = corner_w(c, k);
+        float e0, e1;
+        load_entry(a.table16, size_t(lv) * T_ + idx, &e0, &e1);
+        s0 += w * e0;
+        s1 += w * e1;
+      }
+      f[2 * q] = valid ? s0 : 0.0f;
+      f[2 * q + 1] = valid ? s1 : 0.0f;
+    }
+    {
+      float sh[16];
+      sh16(r1.x, r1.y, r1.z, sh);
+      float ex[4];
+      if (g == 0) {
+        ex[0] = r2.x; ex[1] = r2.y; ex[2] = r2.z; ex[3] = r0.w;
+      } else if (g == 1) {
+        ex[0] = r3.x; ex[1] = r3.y; ex[2] = r3.z; ex[3] = 1.0f;
+      } else {
+        ex[0] = ex[1] = ex[2] = ex[3] = 0.0f;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float s = g == 0 ? sh[i] : (g == 1 ? sh[4 + i] : (g == 2 ? sh[8 + i] : sh[12 + i]));
+        f[8 + i] = valid ? s : 0.0f;
+        f[12 + i] = valid ? ex[i] : 0.0f;
+      }
+    }
+    // h[L][mt][j]: input of layer L in the accumulator layout (feature 16*mt + 4*g + j)
+    float h[5][4][4];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      int ks = i >> 3, ii = i & 7;
+      int mt = 2 * ks + (ii >> 2), j = ii & 3;
+      h[0][mt][j] = f[i];
+    }
+    // ---- forward
+#pragma unroll
+    for (int L = 0; L <= kHidden; ++L) {
+      float v0[8], v1[8];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        v0[i] = h[L][0][i];
+        v0[4 + i] = h[L][1][i];
+        v1[i] = h[L][2][i];
+        v1[4 + i] = h[L][3][i];
+      }
+      float sa = 1.0f;
+      if (P == AQUA_NRC_FP8) sa = a.sc.a[L];
+      T b0 = O::make(v0, sa), b1 = O::make(v1, sa);
+      if constexpr (TRAIN) {
+        float m = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) m = fmaxf(m, fmaxf(fabsf(v0[i]), fabsf(v1[i])));
+        amax_lane[L] = fmaxf(amax_lane[L], m);
+        uint16_t* HTL = a.HT + size_t(L) * 64 * a.npad;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) HTL[size_t(16 * mt + 4 * g + j) * a.npad + row] = O::bits(h[L][mt][j]);
+      }
+      if (L < kHidden) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+          f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+          acc = O::mma(O::unpack(pf[((L * 4 + mt) * 2 + 0) * 64 + lane]), b0, acc);
+          acc = O::mma(O::unpack(pf[((L * 4 + mt) * 2 + 1) * 64 + lane]), b1, acc);
+          float deq = 1.0f;
+          if (P == AQUA_NRC_FP8) deq = 1.0f / (a.sc.a[L] * a.sc.w[L]);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) h[L + 1][mt][j] = fmaxf(acc[j] * deq, 0.0f);
+        }
+      } else {
+        f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc = O::mma(O::unpack(pf[kHidden * 8 * 64 + 0 * 64 + lane]), b0, acc);
+        acc = O::mma(O::unpack(pf[kHidden * 8 * 64 + 1 * 64 + lane]), b1, acc);
+        float deq = 1.0f;
+        if (P == AQUA_NRC_FP8) deq = 1.0f / (a.sc.a[L] * a.sc.w[L]);
+        float y[4] = {acc[0] * deq, acc[1] * deq, acc[2] * deq, acc[3] * deq};
+        if constexpr (!TRAIN) {
+          if (g == 0 && valid) {
+            a.out[3 * size_t(row) + 0] = y[0];
+            a.out[3 * size_t(row) + 1] = y[1];
+            a.out[3 * size_t(row) + 2] = y[2];
+          }
+        } else {
+          // ---- loss gradient (unscaled: the 1 / (3 n) factor is applied in Adam)
+          float dy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+          if (g == 0 && valid) {
+            float4 t = a.tgt[row];
+            float tv[3] = {t.x, t.y, t.z};
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              float den = y[j] * y[j] + 0.01f;
+              float d = y[j] - tv[j];
+              loss_lane += d * d / den;
+              dy[j] = 2.0f * d / den;
+            }
+          }
+          uint16_t* DT4 = a.DT + size_t(kHidden) * 64 * a.npad;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) DT4[size_t(4 * g + j) * a.npad + row] = O::bits(dy[j]);
+          // ---- backward
+          const Pk* pb = reinterpret_cast<const Pk*>(a.pack_bwd);
+          float dz[4][4];
+          {
+            float v[8] = {dy[0], dy[1], dy[2], dy[3], 0.0f, 0.0f, 0.0f, 0.0f};
+            T bd = O::make(v, 1.0f);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+              f4v acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
+              acc2 = O::mma(O::unpack(pb[kHidden * 8 * 64 + mt * 64 + lane]), bd, acc2);
+#pragma unroll
+              for (int j = 0; j < 4; ++j) dz[mt][j] = h[kHidden][mt][j] > 0.0f ? acc2[j] : 0.0f;
+            }
+          }
+#pragma unroll
+          for (int LL = kHidden - 1; LL >= 0; --LL) {
+            uint16_t* DTL = a.DT + size_t(LL) * 64 * a.npad;
+            float v0b[8], v1b[8];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) DTL[size_t(16 * mt + 4 * g + j) * a.npad + row] = O::bits(dz[mt][j]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              v0b[i] = dz[0][i];
+              v0b[4 + i] = dz[1][i];
+              v1b[i] = dz[2][i];
+              v1b[4 + i] = dz[3][i];
+            }
+            T bd0 = O::make(v0b, 1.0f), bd1 = O::make(v1b, 1.0f);
+            float dh[4][4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+              f4v acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
+              acc2 = O::mma(O::unpack(pb[((LL * 4 + mt) * 2 + 0) * 64 + lane]), bd0, acc2);
+              acc2 = O::mma(O::unpack(pb[((LL * 4 + mt) * 2 + 1) * 64 + lane]), bd1, acc2);
+#pragma unroll
+              for (int j = 0; j < 4; ++j) dh[mt][j] = acc2[j];
+            }
+            if (LL > 0) {
+#pragma unroll
+              for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) dz[mt][j] = h[LL][mt][j] > 0.0f ? dh[mt][j] : 0.0f;
+            } else if (valid) {
+              // ---- hash-grid gradient scatter: features of level q live in
+              // dh[q < 2 ? 0 : 1][2*(q%2) + {0,1}]
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
+                float d0 = dh[q >> 1][2 * (q & 1)], d1 = dh[q >> 1][2 * (q & 1) + 1];
+                if (d0 == 0.0f && d1 == 0.0f) continue;
+                Cell c = cell_of(a.G, lv, px, py, pz);
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                  uint32_t idx =
+                      grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
+                  float w = corner_w(c, k);
+                  float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
+                  atomicAdd(gp, w * d0);
+                  atomicAdd(gp + 1, w * d1);
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  if constexpr (TRAIN) {
+    float ls = wave_sum(loss_lane);
+    if (lane == 0 && ls != 0.0f) atomicAdd(a.loss_acc, ls);
+#pragma unroll
+    for (int L = 0; L < 5; ++L) {
+      float m = wave_max(amax_lane[L]);
+      if (lane == 0) atomicMax(&a.act_amax[L], __float_as_uint(m));
+    }
+  }
+}
+
+// ---- weight gradients: dW_L = H_L^T dZ_L, one partial per row chunk ----------
+
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_nrc_wgrad(const uint16_t* HT, const uint16_t* DT, uint32_t npad,
+                                                      float* partial) {
+  typedef Op<P> O;
+  const int L = blockIdx.y;
+  const uint32_t chunk = blockIdx.x;
+  const int lane = int(__lane_id());
+  const int mt = threadIdx.x >> 6;   // wave -> 16-row block of the input features
+  const int nout = L < kHidden ? 64 : 16;
+  const int nts = nout / 16;
+  const uint16_t* H = HT + size_t(L) * 64 * npad;
+  const uint16_t* D = DT + size_t(L) * 64 * npad;
+  uint32_t b_begin = chunk * kChunkRows, b_end = min(npad, b_begin + uint32_t(kChunkRows));
+  f4v acc[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) acc[nt] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+  for (uint32_t b0 = b_begin; b0 < b_end; b0 += 32) {
+    uint32_t kk = b0 + 8u * uint32_t(lane >> 4);
+    uint4 ua = *reinterpret_cast<const uint4*>(H + size_t(16 * mt + (lane & 15)) * npad + kk);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      if (nt >= nts) break;
+      uint4 ub = *reinterpret_cast<const uint4*>(D + size_t(16 * nt + (lane & 15)) * npad + kk);
+      acc[nt] = O::mma(O::unpack(ua), O::unpack(ub), acc[nt]);
+    }
+  }
+  size_t base = size_t(chunk) * kMlp + size_t(L) * 4096;
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    if (nt >= nts) break;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int in = 16 * mt + 4 * (lane >> 4) + j, out = 16 * nt + (lane & 15);
+      partial[base + size_t(in) * nout + out] = acc[nt][j];
+    }
+  }
+}
+
+// ---- Adam ---------------------------------------------------------------------
+
+struct AdamArgs {
+  float lr, beta1, beta2, eps;
+  float c1, c2;      // 1 / (1 - beta1^t), 1 / (1 - beta2^t)
+  float gscale;      // 1 / (3 n)
+};
+
+__device__ __forceinline__ float adam_one(float p, float gr, float* m, float* v, const AdamArgs& A) {
+  float mm = A.beta1 * *m + (1.0f - A.beta1) * gr;
+  float vv = A.beta2 * *v + (1.0f - A.beta2) * gr * gr;
+  *m = mm;
+  *v = vv;
+  float mh = mm * A.c1, vh = vv * A.c2;
+  return p - A.lr * mh / (sqrtf(vh) + A.eps);
+}
+
+__global__ void k_nrc_adam_mlp(const float* partial, uint32_t nchunks, float* master, float* m, float* v,
+                               AdamArgs A) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= uint32_t(kMlp)) return;
+  float g = 0.0f;
+  for (uint32_t c = 0; c < nchunks; ++c) g += partial[size_t(c) * kMlp + p];
+  master[p] = adam_one(master[p], g * A.gscale, &m[p], &v[p], A);
+}
+
+__global__ void k_nrc_reduce_partials(const float* partial, uint32_t nchunks, float* out, float gscale) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= uint32_t(kMlp)) return;
+  float g = 0.0f;
+  for (uint32_t c = 0; c < nchunks; ++c) g += partial[size_t(c) * kMlp + p];
+  out[p] = g * gscale;
+}
+
+__global__ void k_nrc_adam_table(float* master, float* grad, float* m, float* v, uint32_t* table16, size_t n_entries,
+                                 AdamArgs A) {
+  size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_entries) return;
+  float2 p = reinterpret_cast<float2*>(master)[e];
+  float2 gr = reinterpret_cast<float2*>(grad)[e];
+  float2 mm = reinterpret_cast<float2*>(m)[e];
+  float2 vv = reinterpret_cast<float2*>(v)[e];
+  p.x = adam_one(p.x, gr.x * A.gscale, &mm.x, &vv.x, A);
+  p.y = adam_one(p.y, gr.y * A.gscale, &mm.y, &vv.y, A);
+  reinterpret_cast<float2*>(master)[e] = p;
+  reinterpret_cast<float2*>(m)[e] = mm;
+  reinterpret_cast<float2*>(v)[e] = vv;
+  reinterpret_cast<float2*>(grad)[e] = make_float2(0.0f, 0.0f);
+  uint16_t lo = __builtin_bit_cast(uint16_t, (_Float16)p.x), hi = __builtin_bit_cast(uint16_t, (_Float16)p.y);
+  table16[e] = uint32_t(lo) | (uint32_t(hi) << 16);
+}
+
+// ---- packing ------------------------------------------------------------------
+
+// One block per layer. Forward A operand of (mt, ks, lane, i):
+//   W[kappa(lane/16, ks, i)][16 mt + lane%16]; backward: W^T likewise, i.e.
+//   W[16 mt + lane%16][kappa(lane/16, ks, i)]; the output layer's backward
+//   K index is the output 4*(lane/16) + i (i < 4, else 0).
+template <typename E>
+__device__ __forceinline__ E cvt_e(float v);
+template <>
+__device__ __forceinline__ _Float16 cvt_e<_Float16>(float v) { return (_Float16)v; }
+template <>
+__device__ __forceinline__ __bf16 cvt_e<__bf16>(float v) { return (__bf16)v; }
+
+template <typename E>
+__global__ void k_nrc_repack(const float* master, E* fwd, E* bwd, uint8_t* fwd8, float* wscale,
+                             const uint32_t* act_amax, float* ascale) {
+  const int L = blockIdx.x;
+  const float* W = master + size_t(L) * 4096;
+  const int nout = L < kHidden ? 64 : 16;
+  __shared__ float red[kBlock];
+  float m = 0.0f;
+  for (int i = threadIdx.x; i < 64 * nout; i += blockDim.x) m = fmaxf(m, fabsf(W[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  float sw = red[0] > 0.0f ? 448.0f / red[0] : 1.0f;
+  if (threadIdx.x == 0 && fwd8) {
+    wscale[L] = sw;
+    float am = __uint_as_float(act_amax[L]);
+    ascale[L] = am > 0.0f ? 448.0f / am : 1.0f;
+  }
+  // forward
+  const int n_mt = L < kHidden ? 4 : 1;
+  const int fbase = L * 8 * 64 * 8;  // elements (8 per lane per (mt, ks))
+  for (int e = threadIdx.x; e < n_mt * 2 * 64 * 8; e += blockDim.x) {
+    int i = e & 7, lane = (e >> 3) & 63, ks = (e >> 9) & 1, mt = e >> 10;
+    int kin = kappa(lane >> 4, ks, i), out = 16 * mt + (lane & 15);
+    float w = W[size_t(kin) * nout + out];
+    fwd[fbase + e] = cvt_e<E>(w);
+    if (fwd8) {
+      float q = sat448(w * sw);
+      int pk = __builtin_amdgcn_cvt_pk_fp8_f32(q, 0.0f, 0, false);
+      fwd8[fbase + e] = uint8_t(pk & 0xFF);
+    }
+  }
+  if (!bwd) return;
+  // backward
+  if (L < kHidden) {
+    for (int e = threadIdx.x; e < 4 * 2 * 64 * 8; e += blockDim.x) {
+      int i = e & 7, lane = (e >> 3) & 63, ks = (e >> 9) & 1, mt = e >> 10;
+      int in = 16 * mt + (lane & 15), kout = kappa(lane >> 4, ks, i);
+      bwd[L * 8 * 64 * 8 + e] = cvt_e<E>(W[size_t(in) * 64 + kout]);
+    }
+  } else {
+    for (int e = threadIdx.x; e < 4 * 64 * 8; e += blockDim.x) {
+      int i = e & 7, lane = (e >> 3) & 63, mt = e >> 9;
+      int in = 16 * mt + (lane & 15), out = 4 * (lane >> 4) + i;
+      bwd[kHidden * 8 * 64 * 8 + e] = cvt_e<E>(i < 4 ? W[size_t(in) * 16 + out] : 0.0f);
+    }
+  }
+}
+
+__global__ void k_nrc_init_table(float* master, uint32_t* table16, size_t n_entries, uint32_t seed) {
+  size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_entries) return;
+  aqd::Pcg r = aqd::pcg_seed(seed, e);
+  float a = (float(aqd::u01(aqd::pcg_next(r))) * 2.0f - 1.0f) * 1e-4f;
+  float b = (float(aqd::u01(aqd::pcg_next(r))) * 2.0f - 1.0f) * 1e-4f;
+  master[2 * e] = a;
+  master[2 * e + 1] = b;
+  uint16_t lo = __builtin_bit_cast(uint16_t, (_Float16)a), hi = __builtin_bit_cast(uint16_t, (_Float16)b);
+  table16[e] = uint32_t(lo) | (uint32_t(hi) << 16);
+}
+
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return aqua::fail(AQUA_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                               \
+  } while (0)
+
+namespace {
+
+template <class T>
+struct Buf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~Buf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t count) {
+    if (count <= n && p) return AQUA_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(1, count) * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return aqua::fail(e == hipErrorOutOfMemory ? AQUA_ERR_OOM : AQUA_ERR_HIP, "hipMalloc(%zu): %s",
+                        count * sizeof(T), hipGetErrorString(e));
+    }
+    n = count;
+    return AQUA_OK;
+  }
+};
+
+int blocks_for(size_t n, int per = kBlock) { return int((n + per - 1) / per); }
+
+}  // namespace
+
+struct aqua_nrc {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  aqua_nrc_cfg cfg{};
+  Grid grid{};
+  size_t entries = 0;  // levels * T
+  uint32_t step = 0;
+  int cus = 256;
+  Buf<float> tmaster, tgrad, tm, tv;
+  Buf<uint32_t> t16;
+  Buf<float> mmaster, mm, mv, partial;
+  Buf<uint16_t> pack_fwd, pack_bwd;  // FP16 or BF16 bits, per training precision
+  Buf<uint16_t> pack_fwd_other;      // the other 16-bit precision, for inference in it
+  Buf<uint8_t> pack_fp8;
+  Buf<float> wscale, ascale, loss;
+  Buf<uint32_t> act_amax;
+  Buf<uint16_t> HT, DT;
+  uint32_t cap_rows = 0;
+  Buf<float4> rec_buf, tgt_buf;
+  Buf<float> out_buf;
+  hipEvent_t ev[8] = {};
+
+  ~aqua_nrc() {
+    if (stream) {
+      (void)hipSetDevice(device);
+      (void)hipStreamSynchronize(stream);
+    }
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int ensure_rows(uint32_t n) {
+    uint32_t npad = (n + 31u) & ~31u;
+    if (npad <= cap_rows) return AQUA_OK;
+    if (int e = HT.alloc(size_t(5) * 64 * npad)) return e;
+    if (int e = DT.alloc(size_t(5) * 64 * npad)) return e;
+    uint32_t nchunks = (npad + kChunkRows - 1) / kChunkRows;
+    if (int e = partial.alloc(size_t(nchunks) * kMlp)) return e;
+    cap_rows = npad;
+    return AQUA_OK;
+  }
+
+  // Rebuilds the MFMA operand packs from the FP32 master weights: forward +
+  // backward + FP8 in the training precision, forward only in the other one.
+  int repack() {
+    if (cfg.train_precision == AQUA_NRC_BF16) {
+      k_nrc_repack<__bf16><<<5, kBlock, 0, stream>>>(mmaster.p, reinterpret_cast<__bf16*>(pack_fwd.p),
+                                                      reinterpret_cast<__bf16*>(pack_bwd.p), pack_fp8.p,
+                                                      wscale.p, act_amax.p, ascale.p);
+      k_nrc_repack<_Float16><<<5, kBlock, 0, stream>>>(mmaster.p, reinterpret_cast<_Float16*>(pack_fwd_other.p),
+                                                        nullptr, nullptr, wscale.p, act_amax.p, ascale.p);
+    } else {
+      k_nrc_repack<_Float16><<<5, kBlock, 0, stream>>>(mmaster.p, reinterpret_cast<_Float16*>(pack_fwd.p),
+                                                        reinterpret_cast<_Float16*>(pack_bwd.p), pack_fp8.p,
+                                                        wscale.p, act_amax.p, ascale.p);
+      k_nrc_repack<__bf16><<<5, kBlock, 0, stream>>>(mmaster.p, reinterpret_cast<__bf16*>(pack_fwd_other.p),
+                                                      nullptr, nullptr, wscale.p, act_amax.p, ascale.p);
+    }
+    if (hipGetLastError() != hipSuccess) return aqua::fail(AQUA_ERR_HIP, "repack launch failed");
+    return AQUA_OK;
+  }
+
+  FwdArgs args(uint32_t n, const float4* rec, const float4* tgt) {
+    FwdArgs a;
+    memset(&a, 0, sizeof a);
+    a.G = grid;
+    a.rec = rec;
+    a.tgt = tgt;
+    a.n = n;
+    a.npad = (n + 31u) & ~31u;
+    a.table16 = t16.p;
+    a.table_grad = tgrad.p;
+    a.HT = HT.p;
+    a.DT = DT.p;
+    a.loss_acc = loss.p;
+    a.act_amax = act_amax.p;
+    return a;
+  }
+
+  int fwd_grid(uint32_t npad) const {
+    int tiles = int(npad / 16);
+    int waves = std::min(tiles, cus * 32);
+    return std::max(1, (waves + 3) / 4);
+  }
+};
+
+namespace aqua {
+
+// For the renderer: device-side query of n records already on the NRC's
+// device; writes 3 floats per record to d_out. Stream-ordered on `s`.
+int nrc_infer_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, float* d_out, uint32_t precision,
+                     hipStream_t s) {
+  if (n == 0) return AQUA_OK;
+  FwdArgs a = nrc->args(n, d_rec, nullptr);
+  a.out = d_out;
+  uint32_t npad = a.npad;
+  if (precision == AQUA_NRC_FP8) {
+    a.pack_fwd = nrc->pack_fp8.p;
+    // scales are read from device memory by the host once per call
+    float ws[5], as[5];
+    HIPCHK(hipMemcpyAsync(ws, nrc->wscale.p, sizeof ws, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(as, nrc->ascale.p, sizeof as, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int L = 0; L < 5; ++L) {
+      a.sc.w[L] = ws[L];
+      a.sc.a[L] = as[L];
+    }
+    k_nrc_fwd<AQUA_NRC_FP8, false><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+  } else {
+    bool native = precision == nrc->cfg.train_precision;
+    a.pack_fwd = native ? nrc->pack_fwd.p : nrc->pack_fwd_other.p;
+    if (precision == AQUA_NRC_BF16)
+      k_nrc_fwd<AQUA_NRC_BF16, false><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+    else
+      k_nrc_fwd<AQUA_NRC_F16, false><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+  }
+  HIPCHK(hipGetLastError());
+  return AQUA_OK;
+}
+
+int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
+                     aqua_nrc_stats* stats) {
+  if (n == 0 || steps == 0) return AQUA_OK;
+  if (int e = nrc->ensure_rows(n)) return e;
+  hipStream_t s = nrc->stream;
+  FwdArgs a = nrc->args(n, d_rec, d_tgt);
+  a.pack_fwd = nrc->pack_fwd.p;
+  a.pack_bwd = nrc->pack_bwd.p;
+  uint32_t npad = a.npad;
+  uint32_t nchunks = (npad + kChunkRows - 1) / kChunkRows;
+  double t_fb = 0, t_wg = 0, t_ad = 0;
+  float last_loss = 0.0f;
+  HIPCHK(hipEventRecord(nrc->ev[0], s));
+  for (uint32_t it = 0; it < steps; ++it) {
+    ++nrc->step;
+    const aqua_nrc_cfg& c = nrc->cfg;
+    AdamArgs A;
+    A.lr = c.learning_rate;
+    A.beta1 = c.beta1;
+    A.beta2 = c.beta2;
+    A.eps = c.eps;
+    A.c1 = float(1.0 / (1.0 - std::pow(double(c.beta1), double(nrc->step))));
+    A.c2 = float(1.0 / (1.0 - std::pow(double(c.beta2), double(nrc->step))));
+    A.gscale = float(1.0 / (3.0 * double(n)));
+    k_fill_f32<<<1, 64, 0, s>>>(nrc->loss.p, 1, 0.0f);
+    k_fill_u32<<<1, 64, 0, s>>>(nrc->act_amax.p, 5, 0u);
+    HIPCHK(hipEventRecord(nrc->ev[1], s));
+    if (c.train_precision == AQUA_NRC_BF16)
+      k_nrc_fwd<AQUA_NRC_BF16, true><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+    else
+      k_nrc_fwd<AQUA_NRC_F16, true><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+    HIPCHK(hipEventRecord(nrc->ev[2], s));
+    dim3 gw(nchunks, 5);
+    if (c.train_precision == AQUA_NRC_BF16)
+      k_nrc_wgrad<AQUA_NRC_BF16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, nrc->partial.p);
+    else
+      k_nrc_wgrad<AQUA_NRC_F16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, nrc->partial.p);
+    HIPCHK(hipEventRecord(nrc->ev[3], s));
+    k_nrc_adam_mlp<<<blocks_for(kMlp), kBlock, 0, s>>>(nrc->partial.p, nchunks, nrc->mmaster.p, nrc->mm.p,
+                                                       nrc->mv.p, A);
+    k_nrc_adam_table<<<blocks_for(nrc->entries), kBlock, 0, s>>>(nrc->tmaster.p, nrc->tgrad.p, nrc->tm.p,
+                                                                 nrc->tv.p, nrc->t16.p, nrc->entries, A);
+    if (int e = nrc->repack()) return e;
+    HIPCHK(hipEventRecord(nrc->ev[4], s));
+    HIPCHK(hipGetLastError());
+    if (stats) {
+      HIPCHK(hipEventSynchronize(nrc->ev[4]));
+      float ms;
+      HIPCHK(hipEventElapsedTime(&ms, nrc->ev[1], nrc->ev[2]));
+      t_fb += ms;
+      HIPCHK(hipEventElapsedTime(&ms, nrc->ev[2], nrc->ev[3]));
+      t_wg += ms;
+      HIPCHK(hipEventElapsedTime(&ms, nrc->ev[3], nrc->ev[4]));
+      t_ad += ms;
+    }
+  }
+  HIPCHK(hipEventRecord(nrc->ev[5], s));
+  HIPCHK(hipMemcpyAsync(&last_loss, nrc->loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    float ms;
+    HIPCHK(hipEventElapsedTime(&ms, nrc->ev[0], nrc->ev[5]));
+    stats->ms_total = ms;
+    stats->ms_fwd_bwd = t_fb;
+    stats->ms_wgrad = t_wg;
+    stats->ms_adam = t_ad;
+    stats->loss = last_loss / float(3.0 * double(n));
+    stats->steps = steps;
+    stats->rows = n;
+  }
+  return AQUA_OK;
+}
+
+}  // namespace aqua
+
+namespace {
+
+void init_mlp_host(uint32_t seed, std::vector<float>* w) {
+  // Xavier / Glorot uniform, PCG32 stream `seed` (initseq 0x6e7263 = "nrc").
+  w->assign(kMlp, 0.0f);
+  uint64_t state = 0, inc = (0x6e7263ull << 1u) | 1u;
+  auto next = [&]() {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u), rot = uint32_t(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+  };
+  next();
+  state += seed;
+  next();
+  for (int L = 0; L <= kHidden; ++L) {
+    int nout = L < kHidden ? 64 : 16, nout_used = L < kHidden ? 64 : 3;
+    double s = std::sqrt(6.0 / (64.0 + double(nout_used)));
+    for (int i = 0; i < 64; ++i)
+      for (int o = 0; o < nout; ++o) {
+        double u = double(next() >> 8) * (1.0 / 16777216.0);
+        (*w)[size_t(L) * 4096 + size_t(i) * nout + o] = o < nout_used ? float((2.0 * u - 1.0) * s) : 0.0f;
+      }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void aqua_nrc_default_cfg(aqua_nrc_cfg* c) {
+  if (!c) return;
+  memset(c, 0, sizeof *c);
+  c->n_levels = 16;
+  c->log2_table = 19;
+  c->base_res = 16;
+  c->level_scale = 1.5f;
+  c->learning_rate = 1e-2f;
+  c->beta1 = 0.9f;
+  c->beta2 = 0.99f;
+  c->eps = 1e-15f;
+  c->train_precision = AQUA_NRC_F16;
+  c->seed = 0;
+  c->max_batch = 1u << 18;
+}
+
+int aqua_nrc_create(int device, const aqua_nrc_cfg* cfg, aqua_nrc** out) {
+  if (!cfg || !out) return aqua::fail(AQUA_ERR_INVALID_ARG, "aqua_nrc_create: null argument");
+  *out = nullptr;
+  if (cfg->n_levels != uint32_t(kLevels))
+    return aqua::fail(AQUA_ERR_UNSUPPORTED, "aqua_nrc_create: n_levels must be %d", kLevels);
+  if (cfg->log2_table < 10 || cfg->log2_table > 22)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "aqua_nrc_create: log2_table out of [10, 22]");
+  if (cfg->train_precision > AQUA_NRC_BF16)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "aqua_nrc_create: training precision must be FP16 or BF16");
+  if (!(cfg->learning_rate > 0.0f) || !(cfg->level_scale >= 1.0f) || cfg->base_res < 2)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "aqua_nrc_create: bad hyper-parameters");
+  if (device < 0) return aqua::fail(AQUA_ERR_INVALID_ARG, "device %d: the cache runs on a GPU only", device);
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return aqua::fail(AQUA_ERR_NO_DEVICE, "no HIP device available");
+  if (device >= count) return aqua::fail(AQUA_ERR_NO_DEVICE, "device %d out of range", device);
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return aqua::fail(AQUA_ERR_NO_DEVICE, "device %d is %s; this build targets gfx950", device, prop.gcnArchName);
+  HIPCHK(hipSetDevice(device));
+  auto r = std::make_unique<aqua_nrc>();
+  r->device = device;
+  r->cfg = *cfg;
+  r->cus = prop.multiProcessorCount;
+  HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+  for (hipEvent_t& e : r->ev) HIPCHK(hipEventCreate(&e));
+  const uint32_t T = 1u << cfg->log2_table;
+  r->grid.log2T = cfg->log2_table;
+  r->grid.tmask = T - 1u;
+  r->grid.dense_mask = 0;
+  for (int l = 0; l < kLevels; ++l) {
+    double res = std::floor(double(cfg->base_res) * std::pow(double(cfg->level_scale), double(l)));
+    r->grid.res[l] = float(res);
+    double n1 = res + 1.0;
+    if (n1 * n1 * n1 <= double(T)) r->grid.dense_mask |= 1u << l;
+  }
+  r->entries = size_t(kLevels) * T;
+  hipStream_t s = r->stream;
+  if (int e = r->tmaster.alloc(2 * r->entries)) return e;
+  if (int e = r->tgrad.alloc(2 * r->entries)) return e;
+  if (int e = r->tm.alloc(2 * r->entries)) return e;
+  if (int e = r->tv.alloc(2 * r->entries)) return e;
+  if (int e = r->t16.alloc(r->entries)) return e;
+  if (int e = r->mmaster.alloc(kMlp)) return e;
+  if (int e = r->mm.alloc(kMlp)) return e;
+  if (int e = r->mv.alloc(kMlp)) return e;
+  if (int e = r->pack_fwd.alloc(size_t(kFwdPack) * 8)) return e;
+  if (int e = r->pack_fwd_other.alloc(size_t(kFwdPack) * 8)) return e;
+  if (int e = r->pack_bwd.alloc(size_t(kBwdPack) * 8)) return e;
+  if (int e = r->pack_fp8.alloc(size_t(kFwdPack) * 8)) return e;
+  if (int e = r->wscale.alloc(8)) return e;
+  if (int e = r->ascale.alloc(8)) return e;
+  if (int e = r->loss.alloc(1)) return e;
+  if (int e = r->act_amax.alloc(8)) return e;
+  if (int e = r->ensure_rows(std::max(32u, cfg->max_batch))) return e;
+  k_nrc_init_table<<<blocks_for(r->entries), kBlock, 0, s>>>(r->tmaster.p, r->t16.p, r->entries, cfg->seed);
+  k_fill_f32<<<blocks_for(2 * r->entries), kBlock, 0, s>>>(r->tgrad.p, 2 * r->entries, 0.0f);
+  k_fill_f32<<<blocks_for(2 * r->entries), kBlock, 0, s>>>(r->tm.p, 2 * r->entries, 0.0f);
+  k_fill_f32<<<blocks_for(2 * r->entries), kBlock, 0, s>>>(r->tv.p, 2 * r->entries, 0.0f);
+  k_fill_f32<<<blocks_for(kMlp), kBlock, 0, s>>>(r->mm.p, kMlp, 0.0f);
+  k_fill_f32<<<blocks_for(kMlp), kBlock, 0, s>>>(r->mv.p, kMlp, 0.0f);
+  k_fill_u32<<<1, 64, 0, s>>>(r->act_amax.p, 8, 0u);
+  std::vector<float> w;
+  init_mlp_host(cfg->seed, &w);
+  HIPCHK(hipMemcpyAsync(r->mmaster.p, w.data(), kMlp * sizeof(float), hipMemcpyHostToDevice, s));
+  if (int e = r->repack()) return e;
+  HIPCHK(hipStreamSynchronize(s));
+  *out = r.release();
+  return AQUA_OK;
+}
+
+void aqua_nrc_destroy(aqua_nrc* nrc) { delete nrc; }
+
+int aqua_nrc_param_counts(const aqua_nrc* nrc, uint64_t* n_table, uint32_t* n_mlp) {
+  if (!nrc) return aqua::fail(AQUA_ERR_INVALID_ARG, "null nrc");
+  if (n_table) *n_table = 2 * uint64_t(nrc->entries);
+  if (n_mlp) *n_mlp = kMlp;
+  return AQUA_OK;
+}
+
+int aqua_nrc_get_params(aqua_nrc* nrc, float* table, float* mlp) {
+  if (!nrc) return aqua::fail(AQUA_ERR_INVALID_ARG, "null nrc");
+  HIPCHK(hipSetDevice(nrc->device));
+  if (table)
+    HIPCHK(hipMemcpyAsync(table, nrc->tmaster.p, 2 * nrc->entries * sizeof(float), hipMemcpyDeviceToHost,
+                          nrc->stream));
+  if (mlp) HIPCHK(hipMemcpyAsync(mlp, nrc->mmaster.p, kMlp * sizeof(float), hipMemcpyDeviceToHost, nrc->stream));
+  HIPCHK(hipStreamSynchronize(nrc->stream));
+  return AQUA_OK;
+}
+
+int aqua_nrc_set_params(aqua_nrc* nrc, const float* table, const float* mlp) {
+  if (!nrc || !table || !mlp) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipSetDevice(nrc->device));
+  hipStream_t s = nrc->stream;
+  for (size_t i = 0; i < 2 * nrc->entries; i += 4096)
+    for (size_t k = i; k < std::min(2 * nrc->entries, i + 4096); ++k)
+      if (!std::isfinite(table[k])) return aqua::fail(AQUA_ERR_NONFINITE, "non-finite table parameter");
+  for (int k = 0; k < kMlp; ++k)
+    if (!std::isfinite(mlp[k])) return aqua::fail(AQUA_ERR_NONFINITE, "non-finite MLP parameter");
+  HIPCHK(hipMemcpyAsync(nrc->tmaster.p, table, 2 * nrc->entries * sizeof(float), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(nrc->mmaster.p, mlp, kMlp * sizeof(float), hipMemcpyHostToDevice, s));
+  // FP16 table copy from the master
+  {
+    std::vector<uint32_t> t16(nrc->entries);
+    for (size_t e = 0; e < nrc->entries; ++e) {
+      _Float16 a = (_Float16)table[2 * e], b = (_Float16)table[2 * e + 1];
+      uint16_t ua, ub;
+      memcpy(&ua, &a, 2);
+      memcpy(&ub, &b, 2);
+      t16[e] = uint32_t(ua) | (uint32_t(ub) << 16);
+    }
+    HIPCHK(hipMemcpyAsync(nrc->t16.p, t16.data(), nrc->entries * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  k_fill_f32<<<blocks_for(2 * nrc->entries), kBlock, 0, s>>>(nrc->tgrad.p, 2 * nrc->entries, 0.0f);
+  k_fill_f32<<<blocks_for(2 * nrc->entries), kBlock, 0, s>>>(nrc->tm.p, 2 * nrc->entries, 0.0f);
+  k_fill_f32<<<blocks_for(2 * nrc->entries), kBlock, 0, s>>>(nrc->tv.p, 2 * nrc->entries, 0.0f);
+  k_fill_f32<<<blocks_for(kMlp), kBlock, 0, s>>>(nrc->mm.p, kMlp, 0.0f);
+  k_fill_f32<<<blocks_for(kMlp), kBlock, 0, s>>>(nrc->mv.p, kMlp, 0.0f);
+  nrc->step = 0;
+  if (int e = nrc->repack()) return e;
+  HIPCHK(hipStreamSynchronize(s));
+  return AQUA_OK;
+}
+
+static int upload_records(aqua_nrc* nrc, uint32_t n, const float* records, const float* targets) {
+  for (size_t i = 0; i < size_t(n) * 16; ++i)
+    if (!std::isfinite(records[i])) return aqua::fail(AQUA_ERR_NONFINITE, "non-finite record value");
+  if (int e = nrc->rec_buf.alloc(size_t(n) * 4)) return e;
+  HIPCHK(hipMemcpyAsync(nrc->rec_buf.p, records, size_t(n) * 64, hipMemcpyHostToDevice, nrc->stream));
+  if (targets) {
+    std::vector<float4> t(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      t[i] = make_float4(targets[3 * size_t(i)], targets[3 * size_t(i) + 1], targets[3 * size_t(i) + 2], 0.0f);
+      if (!std::isfinite(t[i].x) || !std::isfinite(t[i].y) || !std::isfinite(t[i].z))
+        return aqua::fail(AQUA_ERR_NONFINITE, "non-finite target");
+    }
+    if (int e = nrc->tgt_buf.alloc(n)) return e;
+    HIPCHK(hipMemcpyAsync(nrc->tgt_buf.p, t.data(), size_t(n) * 16, hipMemcpyHostToDevice, nrc->stream));
+    HIPCHK(hipStreamSynchronize(nrc->stream));
+  }
+  return AQUA_OK;
+}
+
+int aqua_nrc_infer(aqua_nrc* nrc, uint32_t n, const float* records, uint32_t precision, float* rgb) {
+  if (!nrc || (n && (!records || !rgb))) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (precision > AQUA_NRC_FP8) return aqua::fail(AQUA_ERR_INVALID_ARG, "unknown precision %u", precision);
+  if (n == 0) return AQUA_OK;
+  HIPCHK(hipSetDevice(nrc->device));
+  if (int e = upload_records(nrc, n, records, nullptr)) return e;
+  if (int e = nrc->out_buf.alloc(size_t(n) * 3)) return e;
+  if (int e = aqua::nrc_infer_device(nrc, n, nrc->rec_buf.p, nrc->out_buf.p, precision, nrc->stream)) return e;
+  HIPCHK(hipMemcpyAsync(rgb, nrc->out_buf.p, size_t(n) * 12, hipMemcpyDeviceToHost, nrc->stream));
+  HIPCHK(hipStreamSynchronize(nrc->stream));
+  return AQUA_OK;
+}
+
+int aqua_nrc_train(aqua_nrc* nrc, uint32_t n, const float* records, const float* targets, uint32_t steps,
+                   aqua_nrc_stats* stats) {
+  if (!nrc || (n && (!records || !targets))) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return AQUA_OK;
+  HIPCHK(hipSetDevice(nrc->device));
+  if (int e = upload_records(nrc, n, records, targets)) return e;
+  aqua_nrc_stats st;
+  if (int e = aqua::nrc_train_device(nrc, n, nrc->rec_buf.p, nrc->tgt_buf.p, steps, &st)) return e;
+  if (stats) *stats = st;
+  return AQUA_OK;
+}
+
+int aqua_nrc_gradients(aqua_nrc* nrc, uint32_t n, const float* records, const float* targets, float* g_table,
+                       float* g_mlp, float* loss) {
+  if (!nrc || n == 0 || !records || !targets) return aqua::fail(AQUA_ERR_INVALID_ARG, "bad argument");
+  HIPCHK(hipSetDevice(nrc->device));
+  if (int e = upload_records(nrc, n, records, targets)) return e;
+  if (int e = nrc->ensure_rows(n)) return e;
+  hipStream_t s = nrc->stream;
+  FwdArgs a = nrc->args(n, nrc->rec_buf.p, nrc->tgt_buf.p);
+  a.pack_fwd = nrc->pack_fwd.p;
+  a.pack_bwd = nrc->pack_bwd.p;
+  uint32_t npad = a.npad, nchunks = (npad + kChunkRows - 1) / kChunkRows;
+  float gscale = float(1.0 / (3.0 * double(n)));
+  k_fill_f32<<<1, 64, 0, s>>>(nrc->loss.p, 1, 0.0f);
+  k_fill_u32<<<1, 64, 0, s>>>(nrc->act_amax.p, 5, 0u);
+  if (nrc->cfg.train_precision == AQUA_NRC_BF16)
+    k_nrc_fwd<AQUA_NRC_BF16, true><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+  else
+    k_nrc_fwd<AQUA_NRC_F16, true><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+  dim3 gw(nchunks, 5);
+  if (nrc->cfg.train_precision == AQUA_NRC_BF16)
+    k_nrc_wgrad<AQUA_NRC_BF16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, nrc->partial.p);
+  else
+    k_nrc_wgrad<AQUA_NRC_F16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, nrc->partial.p);
+  Buf<float> gm;
+  if (int e = gm.alloc(kMlp)) return e;
+  k_nrc_reduce_partials<<<blocks_for(kMlp), kBlock, 0, s>>>(nrc->partial.p, nchunks, gm.p, gscale);
+  HIPCHK(hipGetLastError());
+  float lsum = 0.0f;
+  HIPCHK(hipMemcpyAsync(&lsum, nrc->loss.p, sizeof lsum, hipMemcpyDeviceToHost, s));
+  if (g_mlp) HIPCHK(hipMemcpyAsync(g_mlp, gm.p, kMlp * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (g_table)
+    HIPCHK(hipMemcpyAsync(g_table, nrc->tgrad.p, 2 * nrc->entries * sizeof(float), hipMemcpyDeviceToHost, s));
+  k_fill_f32<<<blocks_for(2 * nrc->entries), kBlock, 0, s>>>(nrc->tgrad.p, 2 * nrc->entries, 0.0f);
+  HIPCHK(hipStreamSynchronize(s));
+  if (g_table)
+    for (size_t i = 0; i < 2 * nrc->entries; ++i) g_table[i] *= gscale;
+  if (loss) *loss = lsum * gscale;
+  return AQUA_OK;
+}
+
+int aqua_nrc_train_device(aqua_nrc* nrc, uint32_t n, const void* d_records, const void* d_targets, uint32_t steps,
+                          aqua_nrc_stats* stats) {
+  if (!nrc || (n && (!d_records || !d_targets))) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipSetDevice(nrc->device));
+  return aqua::nrc_train_device(nrc, n, static_cast<const float4*>(d_records), static_cast<const float4*>(d_targets),
+                                steps, stats);
+}
+
+}  // extern "C"

@@ -236,6 +236,70 @@ int aqua_trace_occluded(aqua_renderer* r, uint32_t n, const float* rays, uint8_t
 
 void aqua_renderer_destroy(aqua_renderer* r);
 
+/* ---- neural radiance cache ("type": "nrc", integrator.json:2) ------------
+ * Multiresolution hash encoding (16 levels x 2 FP16 features, 2^log2_table
+ * entries per level) of the position, degree-4 SH of the direction, plus
+ * normal / roughness / albedo, into a 64-wide MLP of 4 hidden ReLU layers
+ * and 3 linear outputs, run on MFMA (FP16 or BF16, FP8 e4m3 for inference)
+ * with FP32 accumulation; trained with the relative L2 loss and Adam.
+ *
+ * A record is 16 floats: position xyz normalised to [0,1]^3, roughness,
+ * direction xyz, 0, normal xyz, 0, albedo rgb, 0. */
+
+typedef struct aqua_nrc aqua_nrc;
+
+enum aqua_nrc_precision { AQUA_NRC_F16 = 0, AQUA_NRC_BF16 = 1, AQUA_NRC_FP8 = 2 };
+
+/* 64 bytes. */
+typedef struct aqua_nrc_cfg {
+  uint32_t n_levels;        /* must be 16                                    */
+  uint32_t log2_table;      /* entries per level = 2^log2_table (10..22)    */
+  uint32_t base_res;        /* coarsest grid resolution (16)                */
+  float level_scale;        /* resolution growth per level (1.5)            */
+  float learning_rate;      /* Adam step size (integrator.json:7)           */
+  float beta1, beta2, eps;  /* Adam (0.9, 0.99, 1e-15)                      */
+  uint32_t train_precision; /* AQUA_NRC_F16 or AQUA_NRC_BF16                */
+  uint32_t seed;            /* PCG32 seed of the initial parameters         */
+  uint32_t max_batch;       /* largest training batch (rows)                */
+  uint32_t reserved[5];
+} aqua_nrc_cfg;
+
+/* 64 bytes. HIP-event timings of the last training call. */
+typedef struct aqua_nrc_stats {
+  double ms_total;          /* all steps                                    */
+  double ms_fwd_bwd;        /* fused encode + forward + backward + scatter  */
+  double ms_wgrad;          /* weight-gradient reduction (MFMA)             */
+  double ms_adam;           /* Adam over the table and the MLP              */
+  float loss;               /* relative L2 loss of the last step            */
+  uint32_t steps;
+  uint32_t rows;
+  uint32_t reserved[5];
+} aqua_nrc_stats;
+
+void aqua_nrc_default_cfg(aqua_nrc_cfg* out);
+int aqua_nrc_create(int device, const aqua_nrc_cfg* cfg, aqua_nrc** out);
+void aqua_nrc_destroy(aqua_nrc* nrc);
+/* Parameter counts: hash table floats (levels x entries x 2) and MLP floats
+ * (4 x 64 x 64 + 64 x 16, row-major W[in][out] per layer). */
+int aqua_nrc_param_counts(const aqua_nrc* nrc, uint64_t* n_table, uint32_t* n_mlp);
+int aqua_nrc_get_params(aqua_nrc* nrc, float* table, float* mlp);
+/* Replaces the parameters and clears the Adam state. */
+int aqua_nrc_set_params(aqua_nrc* nrc, const float* table, const float* mlp);
+/* Cache query: rgb[3i..3i+2] for each record (host arrays). */
+int aqua_nrc_infer(aqua_nrc* nrc, uint32_t n, const float* records, uint32_t precision, float* rgb);
+/* `steps` Adam steps on one batch (host arrays; targets 3 floats per row). */
+int aqua_nrc_train(aqua_nrc* nrc, uint32_t n, const float* records, const float* targets, uint32_t steps,
+                   aqua_nrc_stats* stats);
+/* Diagnostics: gradients of the relative L2 loss (mean over rows and the 3
+ * channels) with respect to every parameter, from the same kernels a
+ * training step runs, without updating anything. Also returns the loss. */
+int aqua_nrc_gradients(aqua_nrc* nrc, uint32_t n, const float* records, const float* targets,
+                       float* g_table, float* g_mlp, float* loss);
+/* Same on device-resident buffers: records 4 float4 per row, targets one
+ * float4 per row (rgb, -), both on the NRC's device. */
+int aqua_nrc_train_device(aqua_nrc* nrc, uint32_t n, const void* d_records, const void* d_targets,
+                          uint32_t steps, aqua_nrc_stats* stats);
+
 /* ---- misc ---------------------------------------------------------------- */
 
 const char* aqua_last_error(void);

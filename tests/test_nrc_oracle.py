@@ -1,0 +1,149 @@
+"""Pins of the NRC restatement (oracle/nrc_ref.py): SH orthonormality and the
+addition theorem, hash-grid interpolation reproducing linear fields, dense
+indexing bijectivity, finite-difference gradients, Adam vs torch.optim.Adam."""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT
+
+spec = importlib.util.spec_from_file_location("nrc_ref", os.path.join(ROOT, "oracle", "nrc_ref.py"))
+ref = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(ref)
+
+
+def _sphere_grid(n_t=200, n_p=400):
+    # Gauss-Legendre in cos(theta) x uniform phi: exact for these polynomials
+    mu, wmu = np.polynomial.legendre.leggauss(n_t)
+    ph = (np.arange(n_p) + 0.5) / n_p * 2 * np.pi
+    M, P = np.meshgrid(mu, ph, indexing="ij")
+    st = np.sqrt(1 - M * M)
+    d = np.stack([st * np.cos(P), st * np.sin(P), M], -1).reshape(-1, 3)
+    w = (wmu[:, None] * np.full(n_p, 2 * np.pi / n_p)[None, :]).reshape(-1)
+    return d, w
+
+
+def test_sh_orthonormal():
+    d, w = _sphere_grid()
+    Y = ref.sh4(d)
+    G = (Y * w[:, None]).T @ Y
+    np.testing.assert_allclose(G, np.eye(16), atol=1e-10)
+
+
+def test_sh_addition_theorem():
+    # sum_m Y_lm(d)^2 = (2l + 1) / (4 pi) for every direction and band
+    rng = np.random.default_rng(0)
+    d = rng.standard_normal((1000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    Y = ref.sh4(d)
+    for l, (a, b) in enumerate([(0, 1), (1, 4), (4, 9), (9, 16)]):
+        np.testing.assert_allclose((Y[:, a:b] ** 2).sum(1), (2 * l + 1) / (4 * np.pi), rtol=1e-12)
+
+
+def test_dense_level_indexing_is_bijective():
+    for l in range(ref.N_LEVELS):
+        if not ref.level_is_dense(l):
+            continue
+        n = ref.level_res(l) + 1
+        c = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+        idx = ref.corner_index(l, c)
+        assert sorted(idx.tolist()) == list(range(n ** 3))
+
+
+def test_hash_level_indices_in_range():
+    rng = np.random.default_rng(1)
+    for l in range(ref.N_LEVELS):
+        c = rng.integers(0, ref.level_res(l) + 1, (5000, 3))
+        idx = ref.corner_index(l, c, 12)
+        assert idx.min() >= 0 and idx.max() < (1 << 12)
+
+
+def test_trilinear_reproduces_linear_field():
+    # On a dense level, table values that are linear in the vertex position
+    # interpolate to the same linear function of x.
+    l = 1
+    n = ref.level_res(l) + 1
+    N = ref.level_res(l)
+    table = np.zeros((ref.N_LEVELS, 1 << 19, 2))
+    c = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    idx = ref.corner_index(l, c)
+    table[l, idx, 0] = c[:, 0] / N + 2 * c[:, 1] / N
+    table[l, idx, 1] = c[:, 2] / N
+    rng = np.random.default_rng(2)
+    x = rng.random((500, 3)).astype(np.float32)
+    x[:5] = [[0, 0, 0], [1, 1, 1], [0.5, 0.25, 1.0], [1, 0, 0.999], [0.0, 1.0, 0.5]]
+    rec = np.zeros((500, 16), np.float32)
+    rec[:, 0:3] = x
+    rec[:, 4:7] = [0, 0, 1]
+    f = ref.features_torch(rec, torch.from_numpy(table)).numpy()
+    np.testing.assert_allclose(f[:, 2 * l], x[:, 0] + 2 * x[:, 1], atol=1e-6)
+    np.testing.assert_allclose(f[:, 2 * l + 1], x[:, 2], atol=1e-6)
+    np.testing.assert_allclose(f[:, 55], 1.0)
+
+
+def _problem(n=40, log2=12, seed=0):
+    rng = np.random.default_rng(seed)
+    rec = np.zeros((n, 16), np.float32)
+    rec[:, 0:3] = rng.random((n, 3))
+    rec[:, 3] = rng.random(n)
+    d = rng.standard_normal((n, 3))
+    rec[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    nn = rng.standard_normal((n, 3))
+    rec[:, 8:11] = nn / np.linalg.norm(nn, axis=1, keepdims=True)
+    rec[:, 12:15] = rng.random((n, 3))
+    table = rng.uniform(-0.5, 0.5, (ref.N_LEVELS, 1 << log2, 2))
+    mlp = rng.uniform(-0.3, 0.3, 4 * 4096 + 1024)
+    mlp[4 * 4096:].reshape(64, 16)[:, 3:] = 0.0
+    tgt = rng.random((n, 3))
+    return rec, table, mlp, tgt
+
+
+def test_gradients_match_finite_differences():
+    rec, table, mlp, tgt = _problem()
+    loss, gt, gm = ref.loss_and_grads(rec, tgt, table, mlp)
+
+    def L(tb, mp):
+        # the stop-gradient denominator is held at the unperturbed outputs
+        return ref.loss_and_grads(rec, tgt, tb, mp)[0]
+
+    y0 = ref.infer(rec, table, mlp)
+
+    def L_fixed(tb, mp):
+        y = ref.infer(rec, tb, mp)
+        return float((((y - tgt) ** 2) / (y0 ** 2 + 0.01)).mean())
+
+    assert abs(L_fixed(table, mlp) - loss) < 1e-12
+    rng = np.random.default_rng(4)
+    h = 1e-6
+    for k in rng.choice(np.flatnonzero(np.abs(gm) > 1e-6), 12, replace=False):
+        mp, mm = mlp.copy(), mlp.copy()
+        mp[k] += h
+        mm[k] -= h
+        fd = (L_fixed(table, mp) - L_fixed(table, mm)) / (2 * h)
+        assert abs(fd - gm[k]) <= 1e-5 * max(1.0, abs(gm[k])), (k, fd, gm[k])
+    flat = np.flatnonzero(np.abs(gt.reshape(-1)) > 1e-6)
+    for k in rng.choice(flat, 12, replace=False):
+        tp, tm = table.copy().reshape(-1), table.copy().reshape(-1)
+        tp[k] += h
+        tm[k] -= h
+        fd = (L_fixed(tp.reshape(table.shape), mlp) - L_fixed(tm.reshape(table.shape), mlp)) / (2 * h)
+        assert abs(fd - gt.reshape(-1)[k]) <= 1e-5 * max(1.0, abs(fd)), (k, fd, gt.reshape(-1)[k])
+
+
+def test_adam_matches_torch():
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal(1000)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    tp = torch.tensor(p.copy(), requires_grad=True)
+    opt = torch.optim.Adam([tp], lr=1e-2, betas=(0.9, 0.99), eps=1e-15)
+    for step in range(1, 4):
+        g = rng.standard_normal(1000)
+        p, m, v = ref.adam_step(p, g, m, v, step, 1e-2)
+        opt.zero_grad()
+        tp.grad = torch.tensor(g)
+        opt.step()
+        np.testing.assert_allclose(p, tp.detach().numpy(), rtol=1e-12, atol=1e-14)
