@@ -31,7 +31,8 @@ def test_struct_sizes_match_header():
     py = {"aqua_load_opts": aqua.LoadOpts, "aqua_material": aqua.Material, "aqua_camera": aqua.Camera,
           "aqua_scene_info": aqua.SceneInfo, "aqua_procgen_params": aqua.ProcgenParams,
           "aqua_integrator_cfg": aqua.IntegratorCfg, "aqua_frame_stats": aqua.FrameStats,
-          "aqua_renderer_opts": aqua.RendererOpts, "aqua_point_light": aqua.PointLight}
+          "aqua_renderer_opts": aqua.RendererOpts, "aqua_point_light": aqua.PointLight,
+          "aqua_nrc_cfg": aqua.NrcCfg, "aqua_nrc_stats": aqua.NrcStats}
     assert set(sizes) <= set(py)
     for k, s in sizes.items():
         assert C.sizeof(py[k]) == s, k
