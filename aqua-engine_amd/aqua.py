@@ -77,7 +77,9 @@ class IntegratorCfg(C.Structure):
     _fields_ = [("type", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
                 ("batch_size", C.c_uint32), ("training_iters", C.c_uint32),
                 ("learning_rate", C.c_float), ("visualize_cache", C.c_uint32),
-                ("seed", C.c_uint32), ("rr_depth", C.c_uint32), ("reserved", C.c_uint32 * 7)]
+                ("seed", C.c_uint32), ("rr_depth", C.c_uint32), ("nrc_precision", C.c_uint32),
+                ("nrc_train_paths", C.c_uint32), ("nrc_query_vertex", C.c_uint32),
+                ("reserved", C.c_uint32 * 4)]
 
 
 class FrameStats(C.Structure):
@@ -88,7 +90,8 @@ class FrameStats(C.Structure):
                 ("n_intersect_launches", C.c_uint32), ("n_passes", C.c_uint32),
                 ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64),
                 ("shadow_nodes_visited", C.c_uint64), ("shadow_tris_tested", C.c_uint64),
-                ("reserved", C.c_uint64 * 3)]
+                ("nrc_queries", C.c_uint32), ("nrc_train_records", C.c_uint32),
+                ("nrc_loss", C.c_float), ("ms_nrc", C.c_float), ("reserved", C.c_uint64 * 1)]
 
     @property
     def rays(self) -> int:
@@ -163,6 +166,11 @@ _SIGS = {
     "aqua_nrc_train": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(NrcStats)]),
     "aqua_nrc_train_device": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(NrcStats)]),
     "aqua_nrc_gradients": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, C.POINTER(C.c_float)]),
+    "aqua_renderer_nrc": (_P, [_P]),
+    "aqua_renderer_set_nrc_enabled": (C.c_int, [_P, C.c_int]),
+    "aqua_renderer_nrc_batch": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(_P), C.POINTER(_P)]),
+    "aqua_camera_flythrough": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
+                                         C.c_uint32, C.POINTER(Camera)]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -295,6 +303,13 @@ def procgen(kind: int, out_dir: str, seed: int = 0, n_meshes: int = 0, target_tr
     return buf.value.decode()
 
 
+def camera_flythrough(scene: "Scene", frame: int, n_frames: int, seed: int = 0, fov: float = 60.0,
+                      width: int = 1920, height: int = 1080) -> Camera:
+    c = Camera()
+    _check(lib().aqua_camera_flythrough(scene.handle, seed, frame, n_frames, fov, width, height, C.byref(c)))
+    return c
+
+
 def integrator_default() -> IntegratorCfg:
     c = IntegratorCfg()
     lib().aqua_integrator_default(C.byref(c))
@@ -376,6 +391,26 @@ class Renderer:
         t = np.zeros(len(r), np.float32)
         _check(lib().aqua_trace_closest(self._h, len(r), _ptr(r), _ptr(ids), _ptr(t)))
         return ids, t
+
+    def set_nrc_enabled(self, enabled: bool) -> None:
+        _check(lib().aqua_renderer_set_nrc_enabled(self._h, int(enabled)))
+
+    def nrc_handle(self):
+        return lib().aqua_renderer_nrc(self._h)
+
+    def nrc_batch(self):
+        """(n, device pointer of records, device pointer of targets) of the last frame."""
+        n = C.c_uint32()
+        a, b = C.c_void_p(), C.c_void_p()
+        _check(lib().aqua_renderer_nrc_batch(self._h, C.byref(n), C.byref(a), C.byref(b)))
+        return n.value, a.value, b.value
+
+    def nrc_train_batch(self, steps: int = 1) -> "NrcStats":
+        """Adam steps of the cache on the last frame's training batch (device-resident)."""
+        n, a, b = self.nrc_batch()
+        st = NrcStats()
+        _check(lib().aqua_nrc_train_device(self.nrc_handle(), n, a, b, steps, C.byref(st)))
+        return st
 
     def trace_occluded(self, rays: np.ndarray) -> np.ndarray:
         r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)

@@ -21,6 +21,8 @@ constexpr uint32_t kCntRays = 0;        // [0, 65): rays in the queue of bounce 
 constexpr uint32_t kCntShadow = 80;     // [80, 144): shadow rays emitted at bounce b
 constexpr uint32_t kCntFetch = 160;     // [160, 225): closest-hit work cursor of bounce b
 constexpr uint32_t kCntFetchSh = 240;   // [240, 304): any-hit work cursor of bounce b
+constexpr uint32_t kCntQuery = 304;     // cache queries emitted this pass
+constexpr uint32_t kCntTrain = 305;     // cache training samples emitted this pass
 constexpr uint32_t kCntWords = 320;
 // stats64[] layout
 enum : uint32_t {
@@ -57,6 +59,14 @@ struct PassParams {
   uint32_t max_depth;
   uint32_t rr_depth;
   uint32_t dims;  // PCG32 draws reserved per sample
+  // neural radiance cache (nrc == 0: plain path tracing)
+  uint32_t nrc;
+  uint32_t query_depth;   // depth (0 = first hit) at which non-training paths query
+  uint32_t train_stride;  // path p trains when p % stride == offset ...
+  uint32_t train_offset;
+  uint32_t n_train;       // ... and p / stride < n_train
+  float bmin[3];          // position normalisation to [0,1]^3
+  float bscale[3];
 };
 
 struct Queues {
@@ -72,6 +82,11 @@ struct Queues {
   float4* Lbuf;    // radiance of finished paths, by path id
   uint32_t* counters;
   unsigned long long* stats;
+  // cache queries (compacted) and training slots (train index x depth)
+  float4* qrec;    // 4 per query: (pos, roughness), (dir, -), (normal, -), (albedo, -)
+  float4* qaux;    // (throughput, path id bits)
+  float4* trec;    // 4 per training slot, as qrec
+  float4* taux;    // 2 per training slot: (throughput, valid), (radiance after emission, -)
 };
 
 }  // namespace aqd

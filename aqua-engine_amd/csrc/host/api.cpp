@@ -1,5 +1,6 @@
 // C ABI entry points that need no GPU: scenes, procedural scenes, the
 // integrator config and error reporting.
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -100,6 +101,73 @@ int aqua_scene_bvh(const aqua_scene* s, uint32_t* n_nodes, float* nodes, uint32_
 
 int aqua_procgen_write(const aqua_procgen_params* p, const char* out_dir, char* scene_path, size_t cap) {
   AQUA_GUARD(return aqua::procgen_write(p, out_dir, scene_path, cap);)
+}
+
+int aqua_camera_flythrough(const aqua_scene* s, uint32_t seed, uint32_t frame, uint32_t n_frames, float fov_deg,
+                           uint32_t width, uint32_t height, aqua_camera* out) {
+  if (!s || !out || n_frames == 0 || width == 0 || height == 0 || width > 16384 || height > 16384 ||
+      !(fov_deg > 0.0f && fov_deg < 180.0f))
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "aqua_camera_flythrough: bad argument");
+  // 8 control points in the inner part of the bounds (PCG32, stream 0x666c79);
+  // closed uniform Catmull-Rom spline through them; the camera looks at the
+  // centre of the bounds with +y up.
+  const aqua_scene_info& in = s->info;
+  double lo[3], hi[3], c[3];
+  for (int k = 0; k < 3; ++k) {
+    double a = in.bounds_lo[k], b = in.bounds_hi[k];
+    c[k] = 0.5 * (a + b);
+    lo[k] = a + 0.2 * (b - a);
+    hi[k] = b - 0.2 * (b - a);
+  }
+  uint64_t state = 0, inc = (0x666c79ull << 1u) | 1u;
+  auto next = [&]() {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u), rot = uint32_t(old >> 59u);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+  };
+  next();
+  state += seed;
+  next();
+  double P[8][3];
+  for (int i = 0; i < 8; ++i)
+    for (int k = 0; k < 3; ++k) P[i][k] = lo[k] + (hi[k] - lo[k]) * (double(next() >> 8) / 16777216.0);
+  double u = double(frame % n_frames) / double(n_frames) * 8.0;
+  int seg = int(u);
+  double t = u - seg;
+  const double* p0 = P[(seg + 7) % 8];
+  const double* p1 = P[seg % 8];
+  const double* p2 = P[(seg + 1) % 8];
+  const double* p3 = P[(seg + 2) % 8];
+  double pos[3];
+  for (int k = 0; k < 3; ++k) {
+    double t2 = t * t, t3 = t2 * t;
+    pos[k] = 0.5 * ((2 * p1[k]) + (-p0[k] + p2[k]) * t + (2 * p0[k] - 5 * p1[k] + 4 * p2[k] - p3[k]) * t2 +
+                    (-p0[k] + 3 * p1[k] - 3 * p2[k] + p3[k]) * t3);
+  }
+  double f[3] = {c[0] - pos[0], c[1] - pos[1], c[2] - pos[2]};
+  double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+  if (!(fl > 1e-9)) f[0] = 0, f[1] = 0, f[2] = -1, fl = 1;
+  for (double& x : f) x /= fl;
+  double up[3] = {0, 1, 0};
+  // right = f x up, true up = right x f; camera looks down -Z: back = -f
+  double rgt[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
+  double rl = std::sqrt(rgt[0] * rgt[0] + rgt[1] * rgt[1] + rgt[2] * rgt[2]);
+  if (!(rl > 1e-9)) rgt[0] = 1, rgt[1] = 0, rgt[2] = 0, rl = 1;
+  for (double& x : rgt) x /= rl;
+  double u2[3] = {rgt[1] * f[2] - rgt[2] * f[1], rgt[2] * f[0] - rgt[0] * f[2], rgt[0] * f[1] - rgt[1] * f[0]};
+  memset(out, 0, sizeof *out);
+  for (int k = 0; k < 3; ++k) {
+    out->cam_to_world[k * 4 + 0] = float(rgt[k]);
+    out->cam_to_world[k * 4 + 1] = float(u2[k]);
+    out->cam_to_world[k * 4 + 2] = float(-f[k]);
+    out->cam_to_world[k * 4 + 3] = float(pos[k]);
+  }
+  out->tan_half_fov = float(std::tan(0.5 * double(fov_deg) * M_PI / 180.0));
+  out->width = width;
+  out->height = height;
+  out->aspect = float(width) / float(height);
+  return AQUA_OK;
 }
 
 void aqua_integrator_default(aqua_integrator_cfg* out) {

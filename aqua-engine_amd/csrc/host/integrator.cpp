@@ -23,6 +23,9 @@ void integrator_default(aqua_integrator_cfg* c) {
   c->visualize_cache = 0;  // integrator.json:8
   c->seed = 0;
   c->rr_depth = 3;
+  c->nrc_precision = AQUA_NRC_F16;
+  c->nrc_train_paths = 1u << 16;
+  c->nrc_query_vertex = 2;
 }
 
 int integrator_load(const char* path, aqua_integrator_cfg* out) {
@@ -54,7 +57,9 @@ int integrator_load(const char* path, aqua_integrator_cfg* out) {
               {"batch_size", &c.batch_size, 1, 1 << 30},
               {"training_iters", &c.training_iters, 0, 1 << 30},
               {"seed", &c.seed, 0, 4294967295.0},
-              {"rr_depth", &c.rr_depth, 0, 1 << 20}};
+              {"rr_depth", &c.rr_depth, 0, 1 << 20},
+              {"nrc_train_paths", &c.nrc_train_paths, 1, 1 << 22},
+              {"nrc_query_vertex", &c.nrc_query_vertex, 1, 64}};
   for (const U& u : ints) {
     const Json* v = j.get(u.key);
     if (!v) continue;
@@ -66,6 +71,17 @@ int integrator_load(const char* path, aqua_integrator_cfg* out) {
     if (!v->is_num() || !(v->num > 0.0) || !std::isfinite(v->num))
       return fail(AQUA_ERR_PARSE, "%s: 'learning_rate' must be a positive number", path);
     c.learning_rate = float(v->num);
+  }
+  if (const Json* v = j.get("nrc_precision")) {
+    if (v->kind != Json::String) return fail(AQUA_ERR_PARSE, "%s: 'nrc_precision' must be a string", path);
+    if (v->str == "f16")
+      c.nrc_precision = AQUA_NRC_F16;
+    else if (v->str == "bf16")
+      c.nrc_precision = AQUA_NRC_BF16;
+    else if (v->str == "fp8")
+      c.nrc_precision = AQUA_NRC_FP8;
+    else
+      return fail(AQUA_ERR_PARSE, "%s: unknown nrc_precision '%s'", path, v->str.c_str());
   }
   if (const Json* v = j.get("visualize_cache")) {
     if (v->kind != Json::Bool) return fail(AQUA_ERR_PARSE, "%s: 'visualize_cache' must be a bool", path);

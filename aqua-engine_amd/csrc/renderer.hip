@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -142,8 +143,9 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
-    bool alive = false, shadow = false;
+    bool alive = false, shadow = false, query = false;
     float4 n_ro, n_rd, n_tp, s_o, s_d, s_c;
+    float4 qr0, qr1, qr2, qr3, qa;
     uint4 n_rng;
     v3 L = v3{0.0f, 0.0f, 0.0f};
     uint32_t path_id = 0;
@@ -176,7 +178,36 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
           L = v3{L.x + (tp.x * m.emission.x) * w, L.y + (tp.y * m.emission.y) * w,
                  L.z + (tp.z * m.emission.z) * w};
         }
-        if (depth < pp.max_depth) {
+        // ---- neural radiance cache: query (and end the path) or record a
+        // training sample of the reflected radiance at this vertex
+        if (pp.nrc) {
+          bool is_train = (path_id % pp.train_stride) == pp.train_offset && path_id / pp.train_stride < pp.n_train;
+          uint32_t tslot = 0xFFFFFFFFu;
+          if (!is_train && depth == pp.query_depth)
+            query = true;
+          else if (is_train && depth >= pp.query_depth && depth < pp.max_depth)
+            tslot = (path_id / pp.train_stride) * (pp.max_depth + 1u) + depth;
+          if (query || tslot != 0xFFFFFFFFu) {
+            v3 nsq = cg < 0.0f ? neg(ng) : ng;
+            float px = fminf(fmaxf((p.x - pp.bmin[0]) * pp.bscale[0], 0.0f), 1.0f);
+            float py = fminf(fmaxf((p.y - pp.bmin[1]) * pp.bscale[1], 0.0f), 1.0f);
+            float pz = fminf(fmaxf((p.z - pp.bmin[2]) * pp.bscale[2], 0.0f), 1.0f);
+            qr0 = make_float4(px, py, pz, m.roughness);
+            qr1 = make_float4(wo.x, wo.y, wo.z, 0.0f);
+            qr2 = make_float4(nsq.x, nsq.y, nsq.z, 0.0f);
+            qr3 = make_float4(m.base.x, m.base.y, m.base.z, 0.0f);
+            qa = make_float4(tp.x, tp.y, tp.z, __uint_as_float(path_id));
+            if (tslot != 0xFFFFFFFFu) {
+              q.trec[4 * size_t(tslot) + 0] = qr0;
+              q.trec[4 * size_t(tslot) + 1] = qr1;
+              q.trec[4 * size_t(tslot) + 2] = qr2;
+              q.trec[4 * size_t(tslot) + 3] = qr3;
+              q.taux[2 * size_t(tslot) + 0] = make_float4(tp.x, tp.y, tp.z, 1.0f);
+              q.taux[2 * size_t(tslot) + 1] = make_float4(L.x, L.y, L.z, 0.0f);
+            }
+          }
+        }
+        if (depth < pp.max_depth && !query) {
           uint32_t pixel = path_id % pp.npix;
           Pcg rng;
           rng.state = (uint64_t(rg.y) << 32) | uint64_t(rg.x);
@@ -292,7 +323,67 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
       q.shd[k] = s_d;
       q.shc[k] = s_c;
     }
+    if (pp.nrc) {
+      uint32_t qi = wave_append(query, &q.counters[kCntQuery]);
+      if (query) {
+        q.qrec[4 * size_t(qi) + 0] = qr0;
+        q.qrec[4 * size_t(qi) + 1] = qr1;
+        q.qrec[4 * size_t(qi) + 2] = qr2;
+        q.qrec[4 * size_t(qi) + 3] = qr3;
+        q.qaux[qi] = qa;
+      }
+    }
   }
+}
+
+// Adds throughput x max(cache, 0) to the radiance of each querying path.
+__global__ void k_nrc_apply(const float4* qaux, const float* qout, const uint32_t* count, float4* Lbuf) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= *count) return;
+  float4 a = qaux[i];
+  uint32_t path = __float_as_uint(a.w);
+  float r = fmaxf(qout[3 * size_t(i)], 0.0f), g = fmaxf(qout[3 * size_t(i) + 1], 0.0f),
+        bl = fmaxf(qout[3 * size_t(i) + 2], 0.0f);
+  float4 l = Lbuf[path];
+  Lbuf[path] = make_float4(l.x + a.x * r, l.y + a.y * g, l.z + a.z * bl, 0.0f);
+}
+
+// Training targets: the radiance a training path gathered after a recorded
+// vertex, divided by the throughput up to it = the reflected radiance the
+// cache should predict there. Valid samples are compacted into the batch.
+__global__ void k_nrc_targets(const float4* trec, const float4* taux, uint32_t n_slots, uint32_t slots_per_path,
+                              uint32_t stride, uint32_t offset, const float4* Lbuf, float4* out_rec,
+                              float4* out_tgt, uint32_t* counter) {
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n_slots; base += gridDim.x * blockDim.x) {
+    uint32_t s = base + threadIdx.x;
+    bool ok = false;
+    float4 tgt = make_float4(0, 0, 0, 0);
+    if (s < n_slots) {
+      float4 T = taux[2 * size_t(s)];
+      if (T.w == 1.0f) {
+        float4 Lpre = taux[2 * size_t(s) + 1];
+        uint32_t path = (s / slots_per_path) * stride + offset;
+        float4 Lf = Lbuf[path];
+        float tv[3] = {T.x, T.y, T.z}, lf[3] = {Lf.x, Lf.y, Lf.z}, lp[3] = {Lpre.x, Lpre.y, Lpre.z}, o[3];
+        ok = true;
+        for (int c = 0; c < 3; ++c) {
+          o[c] = tv[c] > 1e-7f ? fmaxf(lf[c] - lp[c], 0.0f) / tv[c] : 0.0f;
+          if (!__builtin_isfinite(o[c])) ok = false;
+        }
+        tgt = make_float4(o[0], o[1], o[2], 0.0f);
+      }
+    }
+    uint32_t j = wave_append(ok, counter);
+    if (ok) {
+      for (int k = 0; k < 4; ++k) out_rec[4 * size_t(j) + k] = trec[4 * size_t(s) + k];
+      out_tgt[j] = tgt;
+    }
+  }
+}
+
+__global__ void k_clear_taux(float4* taux, uint32_t n_slots) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n_slots) taux[2 * size_t(s)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 template <bool COUNT>
@@ -529,12 +620,20 @@ struct aqua_renderer {
   DevBuf<double> err_partial;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> kev;
+  // neural radiance cache
+  aqua_nrc* nrc = nullptr;
+  bool nrc_enabled = true;
+  float bmin[3] = {0, 0, 0}, bscale[3] = {1, 1, 1};
+  DevBuf<float4> qrec, qaux, trec, taux, brec, btgt;
+  DevBuf<float> qout;
+  uint32_t batch_n = 0;
 
   ~aqua_renderer() {
     if (stream) {
       (void)hipSetDevice(device);
       (void)hipStreamSynchronize(stream);
     }
+    if (nrc) aqua_nrc_destroy(nrc);
     for (hipEvent_t e : kev) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -571,6 +670,10 @@ struct aqua_renderer {
     q.Lbuf = Lbuf.p;
     q.counters = counters.p;
     q.stats = stats.p;
+    q.qrec = qrec.p;
+    q.qaux = qaux.p;
+    q.trec = trec.p;
+    q.taux = taux.p;
     return q;
   }
 
@@ -588,10 +691,22 @@ struct aqua_renderer {
     if (int e = shd.ensure(n)) return e;
     if (int e = shc.ensure(n)) return e;
     if (int e = Lbuf.ensure(n)) return e;
+    if (nrc) {
+      if (int e = qrec.ensure(4 * n)) return e;
+      if (int e = qaux.ensure(n)) return e;
+      if (int e = qout.ensure(3 * n)) return e;
+    }
     cap = n;
     return AQUA_OK;
   }
 };
+
+namespace aqua {
+int nrc_infer_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, float* d_out, uint32_t precision,
+                     hipStream_t s);
+int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
+                     aqua_nrc_stats* stats);
+}  // namespace aqua
 
 namespace {
 
@@ -650,7 +765,9 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   Queues q = r->queues();
   int g = grid_for(n);
   int gp = std::min(g, r->persistent_blocks);
+  const uint32_t n_slots = pp.nrc ? pp.n_train * (pp.max_depth + 1u) : 0u;
   k_reset<<<1, kCntWords, 0, s>>>(q.counters, n);
+  if (n_slots) k_clear_taux<<<int((n_slots + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.taux, n_slots);
   k_camera<<<g, kBlock, 0, s>>>(pp, q);
   bool timing = r->opts.timing != 0;
   for (uint32_t b = 0; b <= pp.max_depth; ++b) {
@@ -674,11 +791,67 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     }
     if (timing) HIPCHK(hipEventRecord(ev[3], s));
   }
+  HIPCHK(hipGetLastError());
+  float ms_nrc = 0.0f;
+  uint32_t n_query = 0, n_train_rec = 0;
+  float nrc_loss = 0.0f;
+  if (pp.nrc) {
+    // cache queries: infer, then add throughput x prediction to each path
+    uint32_t nq = 0;
+    HIPCHK(hipMemcpyAsync(&nq, q.counters + kCntQuery, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    n_query = nq;
+    uint32_t prec = r->cfg.nrc_precision;
+    hipEvent_t a, b2;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b2));
+    HIPCHK(hipEventRecord(a, s));
+    if (nq) {
+      if (int e = aqua::nrc_infer_device(r->nrc, nq, r->qrec.p, r->qout.p, prec, s)) return e;
+      k_nrc_apply<<<int((nq + kBlock - 1) / kBlock), kBlock, 0, s>>>(r->qaux.p, r->qout.p, q.counters + kCntQuery,
+                                                                     q.Lbuf);
+    }
+    HIPCHK(hipEventRecord(b2, s));
+    HIPCHK(hipEventSynchronize(b2));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b2));
+    ms_nrc += ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b2);
+  }
   int gf = int((pp.npix + kBlock - 1) / kBlock);
   if (fp64)
     k_film64<<<gf, kBlock, 0, s>>>(pp, q.Lbuf, r->film64.p, q.stats);
   else
     k_film32<<<gf, kBlock, 0, s>>>(pp, q.Lbuf, r->film.p, q.stats);
+  if (pp.nrc && n_slots) {
+    // training: targets of this pass's suffix paths, then Adam steps
+    auto t0 = std::chrono::steady_clock::now();
+    k_nrc_targets<<<std::min(grid_for(n_slots), 4096), kBlock, 0, s>>>(
+        q.trec, q.taux, n_slots, pp.max_depth + 1u, pp.train_stride, pp.train_offset, q.Lbuf, r->brec.p,
+        r->btgt.p, q.counters + kCntTrain);
+    HIPCHK(hipGetLastError());
+    uint32_t nt = 0;
+    HIPCHK(hipMemcpyAsync(&nt, q.counters + kCntTrain, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    n_train_rec = nt;
+    r->batch_n = nt;
+    // four Adam steps per pass, each on a quarter of the samples
+    const uint32_t steps = 4;
+    uint32_t per = (nt + steps - 1) / steps;
+    for (uint32_t k = 0; k < steps && per > 0; ++k) {
+      uint32_t lo = k * per, cnt = std::min(per, nt - std::min(nt, lo));
+      if (cnt < 32) continue;
+      aqua_nrc_stats ns;
+      if (int e = aqua::nrc_train_device(r->nrc, cnt, r->brec.p + 4 * size_t(lo), r->btgt.p + lo, 1, &ns)) return e;
+      nrc_loss = ns.loss;
+    }
+    ms_nrc += float(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    if (!std::isfinite(nrc_loss)) {
+      // a diverged cache is not used again: frames fall back to plain tracing
+      r->nrc_enabled = false;
+    }
+  }
   HIPCHK(hipGetLastError());
   // Per-pass counters (ray counts) are read back once the pass is queued;
   // the copy waits on the stream, so passes stay ordered.
@@ -686,6 +859,10 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   HIPCHK(hipMemcpyAsync(hc, q.counters, sizeof hc, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (st) {
+    st->nrc_queries += n_query;
+    st->nrc_train_records += n_train_rec;
+    if (pp.nrc) st->nrc_loss = nrc_loss;
+    st->ms_nrc += ms_nrc;
     st->camera_rays += hc[kCntRays];
     for (uint32_t b = 1; b <= pp.max_depth; ++b) st->extension_rays += hc[kCntRays + b];
     for (uint32_t b = 0; b < pp.max_depth; ++b) st->shadow_rays += hc[kCntShadow + b];
@@ -729,6 +906,30 @@ int prepare_frame(aqua_renderer* r, const aqua_camera* cam_in, uint32_t spp, aqu
   pp->max_depth = r->cfg.max_depth;
   pp->rr_depth = r->cfg.rr_depth;
   pp->dims = 8u * (r->cfg.max_depth + 2u);
+  return AQUA_OK;
+}
+
+// Cache parameters of one pass of `n` paths: every stride-th path (from a
+// per-pass offset) is a training path, the rest query at the query vertex.
+int nrc_pass_params(aqua_renderer* r, uint32_t n, uint32_t frame, uint32_t sample0, PassParams* pp) {
+  uint32_t want = std::max(1u, r->cfg.nrc_train_paths);
+  uint32_t stride = std::max(1u, n / want);
+  uint32_t offset = uint32_t((uint64_t(frame) * 2654435761ull + uint64_t(sample0) * 40503ull) % stride);
+  uint32_t n_train = std::min(want, (n - offset + stride - 1) / stride);
+  pp->nrc = 1;
+  pp->query_depth = r->cfg.nrc_query_vertex >= 1 ? r->cfg.nrc_query_vertex - 1 : 0;
+  pp->train_stride = stride;
+  pp->train_offset = offset;
+  pp->n_train = n_train;
+  for (int k = 0; k < 3; ++k) {
+    pp->bmin[k] = r->bmin[k];
+    pp->bscale[k] = r->bscale[k];
+  }
+  size_t slots = size_t(n_train) * (r->cfg.max_depth + 1u);
+  if (int e = r->trec.ensure(4 * slots)) return e;
+  if (int e = r->taux.ensure(2 * slots)) return e;
+  if (int e = r->brec.ensure(4 * slots)) return e;
+  if (int e = r->btgt.ensure(slots)) return e;
   return AQUA_OK;
 }
 
@@ -853,6 +1054,26 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   if (int e = r->pmf.upload(pmf.data(), pmf.size(), s)) return e;
   if (int e = r->counters.ensure(kCntWords)) return e;
   if (int e = r->stats.ensure(kStWords)) return e;
+  if (cfg->type == AQUA_INTEGRATOR_NRC) {
+    if (cfg->nrc_precision > AQUA_NRC_FP8)
+      return aqua::fail(AQUA_ERR_INVALID_ARG, "unknown nrc_precision %u", cfg->nrc_precision);
+    if (cfg->nrc_query_vertex < 1) return aqua::fail(AQUA_ERR_INVALID_ARG, "nrc_query_vertex must be >= 1");
+    aqua_nrc_cfg nc;
+    aqua_nrc_default_cfg(&nc);
+    nc.learning_rate = cfg->learning_rate;
+    nc.train_precision = cfg->nrc_precision == AQUA_NRC_F16 ? AQUA_NRC_F16 : AQUA_NRC_BF16;
+    nc.seed = cfg->seed;
+    uint32_t tp = std::max(1u, cfg->nrc_train_paths);
+    nc.max_batch = uint32_t(std::min<uint64_t>(uint64_t(tp) * std::max(1u, cfg->max_depth) / 4 + 64, 1u << 26));
+    if (int e = aqua_nrc_create(device, &nc, &r->nrc)) return e;
+    // positions are normalised to the scene bounds (slightly padded)
+    for (int k = 0; k < 3; ++k) {
+      float lo = scene->info.bounds_lo[k], hi = scene->info.bounds_hi[k];
+      float pad = 1e-3f * std::max(1e-6f, hi - lo);
+      r->bmin[k] = lo - pad;
+      r->bscale[k] = 1.0f / std::max(1e-6f, (hi - lo) + 2 * pad);
+    }
+  }
   HIPCHK(hipStreamSynchronize(s));
   *out = r.release();
   return AQUA_OK;
@@ -896,6 +1117,9 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
     pp.sample0 = spp0 + s0;
     pp.spp_in_pass = std::min(S, spp - s0);
     pp.frame = frame;
+    pp.nrc = 0;
+    if (!fp64 && r->nrc && r->nrc_enabled)
+      if (int e = nrc_pass_params(r, pp.npix * pp.spp_in_pass, frame, pp.sample0, &pp)) return e;
     if (int e = run_pass(r, pp, fp64, &st, count)) return e;
     ++st.n_passes;
   }
@@ -1077,6 +1301,24 @@ int aqua_trace_occluded(aqua_renderer* r, uint32_t n, const float* rays, uint8_t
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(occluded, out.p, n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  return AQUA_OK;
+}
+
+aqua_nrc* aqua_renderer_nrc(aqua_renderer* r) { return r ? r->nrc : nullptr; }
+
+int aqua_renderer_set_nrc_enabled(aqua_renderer* r, int enabled) {
+  if (!r) return aqua::fail(AQUA_ERR_INVALID_ARG, "null renderer");
+  if (!r->nrc && enabled) return aqua::fail(AQUA_ERR_INVALID_ARG, "the renderer has no cache (type is not nrc)");
+  r->nrc_enabled = enabled != 0;
+  return AQUA_OK;
+}
+
+int aqua_renderer_nrc_batch(aqua_renderer* r, uint32_t* n, const void** d_records, const void** d_targets) {
+  if (!r || !n || !d_records || !d_targets) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (!r->nrc) return aqua::fail(AQUA_ERR_INVALID_ARG, "the renderer has no cache (type is not nrc)");
+  *n = r->batch_n;
+  *d_records = r->brec.p;
+  *d_targets = r->btgt.p;
   return AQUA_OK;
 }
 

@@ -162,7 +162,14 @@ typedef struct aqua_integrator_cfg {
   uint32_t visualize_cache; /* "visualize_cache"               (integrator.json:8) */
   uint32_t seed;            /* "seed" (extension, default 0): PCG32 initstate     */
   uint32_t rr_depth;        /* "rr_depth" (extension, default 3)                  */
-  uint32_t reserved[7];
+  /* "nrc" only (extensions): */
+  uint32_t nrc_precision;   /* "nrc_precision": "f16" | "bf16" | "fp8" (queries; training
+                               runs in BF16 for fp8)                                 */
+  uint32_t nrc_train_paths; /* "nrc_train_paths": suffix paths traced for training
+                               per frame (default 65536)                             */
+  uint32_t nrc_query_vertex;/* "nrc_query_vertex": path vertex (1 = first hit) at
+                               which other paths end in a cache query (default 2)    */
+  uint32_t reserved[4];
 } aqua_integrator_cfg;
 
 void aqua_integrator_default(aqua_integrator_cfg* out);
@@ -186,7 +193,11 @@ typedef struct aqua_frame_stats {
   uint64_t tris_tested;        /* closest-hit rays only                    */
   uint64_t shadow_nodes_visited;
   uint64_t shadow_tris_tested;
-  uint64_t reserved[3];
+  uint32_t nrc_queries;        /* paths ended by a cache query          */
+  uint32_t nrc_train_records;  /* training samples produced this frame  */
+  float nrc_loss;              /* last training step's loss             */
+  float ms_nrc;                /* cache inference + targets + training  */
+  uint64_t reserved[1];
 } aqua_frame_stats;
 
 /* 32 bytes. */
@@ -300,7 +311,23 @@ int aqua_nrc_gradients(aqua_nrc* nrc, uint32_t n, const float* records, const fl
 int aqua_nrc_train_device(aqua_nrc* nrc, uint32_t n, const void* d_records, const void* d_targets,
                           uint32_t steps, aqua_nrc_stats* stats);
 
+/* The cache of an "nrc" renderer (NULL for "pt"); owned by the renderer. */
+aqua_nrc* aqua_renderer_nrc(aqua_renderer* r);
+/* 0 disables the cache (frames render as the plain path tracer, which is
+ * also the fallback when training diverges); 1 re-enables it. */
+int aqua_renderer_set_nrc_enabled(aqua_renderer* r, int enabled);
+/* The last frame's training batch, device-resident on the renderer's device:
+ * records (4 float4 per row) and targets (float4 per row). Valid until the
+ * next frame. */
+int aqua_renderer_nrc_batch(aqua_renderer* r, uint32_t* n, const void** d_records, const void** d_targets);
+
 /* ---- misc ---------------------------------------------------------------- */
+
+/* Camera of frame `frame` of an `n_frames` fly-through: a closed Catmull-Rom
+ * spline through 8 seeded control points inside the scene bounds, looking at
+ * the bounds' centre; fov and resolution as given. */
+int aqua_camera_flythrough(const aqua_scene* scene, uint32_t seed, uint32_t frame, uint32_t n_frames,
+                           float fov_deg, uint32_t width, uint32_t height, aqua_camera* out);
 
 const char* aqua_last_error(void);
 int aqua_abi_version(void);

@@ -1,0 +1,132 @@
+"""Online neural radiance cache inside the wavefront renderer."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import aqua, small_camera
+
+pytestmark = pytest.mark.gpu
+
+
+def nrc_cfg(max_depth=5, train_paths=1024, precision=aqua.NRC_F16, query_vertex=2, lr=1e-2):
+    c = aqua.integrator_default()
+    c.type = aqua.INTEGRATOR_NRC
+    c.max_depth = max_depth
+    c.nrc_train_paths = train_paths
+    c.nrc_precision = precision
+    c.nrc_query_vertex = query_vertex
+    c.learning_rate = lr
+    return c
+
+
+def furnace_scene(rho, w=48, h=48):
+    """Closed cube [-1,1]^3, every wall a Lambertian emitter (Le = 1, albedo
+    rho) facing inward; the camera sits inside."""
+    tris = []
+    for axis in range(3):
+        for s in (-1.0, 1.0):
+            def P(a, b):
+                p = [0.0, 0.0, 0.0]
+                p[axis] = s
+                p[(axis + 1) % 3] = a
+                p[(axis + 2) % 3] = b
+                return p
+            q = [P(-1, -1), P(1, -1), P(1, 1), P(-1, 1)]
+            for t in ((q[0], q[1], q[2]), (q[0], q[2], q[3])):
+                v = np.array(t, np.float32)
+                nrm = np.cross(v[1] - v[0], v[2] - v[0])
+                if nrm @ (-v.mean(0)) < 0:
+                    v = v[[0, 2, 1]]
+                tris.append(v.reshape(-1))
+    m = aqua.Material()
+    m.base_color[:] = [rho] * 3
+    m.emission[:] = [1.0] * 3
+    m.roughness = 1.0
+    cam = aqua.Camera()
+    cam.cam_to_world[:] = [1, 0, 0, 0.1, 0, 1, 0, -0.2, 0, 0, 1, 0.3]
+    cam.tan_half_fov = float(np.float32(math.tan(math.radians(40))))
+    cam.width, cam.height, cam.aspect = w, h, 1.0
+    return aqua.Scene.from_arrays(np.array(tris), np.zeros(len(tris), np.uint32), [m], cam)
+
+
+def test_cache_disabled_is_the_plain_tracer(gpu, box_room):
+    cam = small_camera(box_room.camera(), 64, 64)
+    pt = aqua.integrator_default()
+    pt.max_depth = 5
+    a = aqua.Renderer(box_room, pt)
+    a.render(4, camera=cam)
+    r = aqua.Renderer(box_room, nrc_cfg())
+    r.set_nrc_enabled(False)
+    st = r.render(4, camera=cam)
+    assert st.nrc_queries == 0
+    np.testing.assert_array_equal(r.film(cam), a.film(cam))
+
+
+def test_furnace_through_the_cache(gpu):
+    # Query paths: emission at vertex 1, NEE / BSDF to vertex 2, emission
+    # there, then the cache for the rest; the cache must learn the reflected
+    # radiance sum_{k=1}^{D-1} rho^k, so the image converges to sum_{k<=D} rho^k.
+    rho, D = 0.5, 5
+    sc = furnace_scene(rho)
+    r = aqua.Renderer(sc, nrc_cfg(max_depth=D, train_paths=512, lr=1e-2))
+    for f in range(60):
+        st = r.render(1, frame=f)
+    assert st.nrc_queries > 0 and st.nrc_train_records > 0
+    imgs = []
+    for f in range(60, 70):
+        r.render(1, frame=f)
+        imgs.append(r.film())
+    mean = float(np.mean(imgs))
+    expect = sum(rho ** k for k in range(D + 1))
+    assert abs(mean - expect) < 0.04 * expect, (mean, expect)
+
+
+def test_training_targets_in_the_furnace(gpu):
+    # query vertex 1, depth 1: every training sample sits at a first hit and
+    # its target is the one-bounce reflected radiance, rho in expectation.
+    rho = 0.6
+    sc = furnace_scene(rho)
+    r = aqua.Renderer(sc, nrc_cfg(max_depth=1, train_paths=2048, query_vertex=1))
+    r.render(1, frame=0)
+    n, _, _ = r.nrc_batch()
+    assert n > 1000
+    st = r.nrc_train_batch(steps=0)
+    assert st.rows == 0  # zero steps: nothing trained
+    # the cache then learns targets whose mean is rho
+    for f in range(1, 40):
+        r.render(1, frame=f)
+    nrc = aqua.Nrc.__new__(aqua.Nrc)
+    nrc._h = aqua.C.c_void_p(r.nrc_handle())
+    rec = np.zeros((256, 16), np.float32)
+    rng = np.random.default_rng(0)
+    rec[:, 0:3] = rng.random((256, 3))
+    rec[:, 3] = 1.0
+    d = rng.standard_normal((256, 3))
+    rec[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rec[:, 8:11] = rec[:, 4:7]
+    rec[:, 12:15] = rho
+    # points on the walls only: project to the nearest face
+    y = nrc.infer(rec)
+    nrc._h = None  # owned by the renderer
+    assert abs(float(np.median(y)) - rho) < 0.15, float(np.median(y))
+
+
+@pytest.mark.parametrize("prec", [aqua.NRC_F16, aqua.NRC_BF16, aqua.NRC_FP8])
+def test_box_room_cache_reduces_error(gpu, box_room, prec):
+    cam = small_camera(box_room.camera(), 96, 96)
+    pt = aqua.integrator_default()
+    pt.max_depth = 5
+    ref = aqua.Renderer(box_room, pt)
+    ref.render_oracle(0, 512, camera=cam)
+    ref.render(1, camera=cam, frame=1000)
+    pt_err, _ = ref.film_error()
+    r = aqua.Renderer(box_room, nrc_cfg(max_depth=5, train_paths=2048, precision=prec))
+    for f in range(40):
+        st = r.render(1, camera=cam, frame=f)
+        assert np.isfinite(st.nrc_loss)
+    img = r.film(cam)
+    assert np.all(np.isfinite(img))
+    oracle = ref.oracle_film(cam)
+    rel = float(np.mean((img - oracle) ** 2 / (oracle ** 2 + 1e-2)))
+    assert rel < pt_err, (rel, pt_err)
