@@ -52,11 +52,6 @@ struct Grid {
   uint32_t log2T;
 };
 
-struct Scales {  // FP8 inference: per-layer weight and activation scales
-  float w[5];
-  float a[5];
-};
-
 __device__ __forceinline__ int kappa(int g, int ks, int i) { return 16 * (2 * ks + (i >> 2)) + 4 * g + (i & 3); }
 
 __device__ __forceinline__ uint32_t grid_index(const Grid& G, int lv, int cx, int cy, int cz) {
@@ -192,15 +187,17 @@ struct FwdArgs {
   const float4* rec;        // 4 per row
   const float4* tgt;        // 1 per row (training)
   uint32_t n, npad;
+  const uint32_t* n_ptr;    // inference: row count read on the device (else n)
   const uint32_t* table16;  // [level][T] two FP16 features
   const void* pack_fwd;     // A operands
   const void* pack_bwd;
-  Scales sc;                // FP8 only
+  const float* wscale;      // FP8: per-layer weight scales (device)
+  const float* ascale;      // FP8: per-layer activation scales (device)
   float* out;               // 3 per row (inference)
   // training
   float* table_grad;        // [level][T][2] FP32
-  uint16_t* HT;             // 5 x [64][npad] layer inputs (transposed)
-  uint16_t* DT;             // 4 x [64][npad] + [16][npad] pre-activation grads
+  uint16_t* HT;             // 5 x [npad][64] layer inputs
+  uint16_t* DT;             // 5 x [npad][64] pre-activation grads (last: 16 used)
   float* loss_acc;
   uint32_t* act_amax;       // 5 per-layer input amax (float bits)
 };
@@ -214,6 +211,74 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// Encodes this lane group's 16 input features of one row: f[ks*8 + i] is
+// feature kappa(g, ks, i) (hash levels 2g, 2g+1, 8+2g, 9+2g; SH 4g..4g+3;
+// extras 4g..4g+3).
+__device__ __forceinline__ void encode16(const FwdArgs& a, uint32_t T_, int g, bool valid, float px, float py,
+                                         float pz, float4 r0, float4 r1, float4 r2, float4 r3, float f[16]) {
+#pragma unroll 2
+  for (int q = 0; q < 4; ++q) {
+    int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
+    Cell c = cell_of(a.G, lv, px, py, pz);
+    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
+      float w = corner_w(c, k);
+      float e0, e1;
+      load_entry(a.table16, size_t(lv) * T_ + idx, &e0, &e1);
+      s0 += w * e0;
+      s1 += w * e1;
+    }
+    f[2 * q] = valid ? s0 : 0.0f;
+    f[2 * q + 1] = valid ? s1 : 0.0f;
+  }
+  float sh[16];
+  sh16(r1.x, r1.y, r1.z, sh);
+  float ex[4];
+  if (g == 0) {
+    ex[0] = r2.x; ex[1] = r2.y; ex[2] = r2.z; ex[3] = r0.w;
+  } else if (g == 1) {
+    ex[0] = r3.x; ex[1] = r3.y; ex[2] = r3.z; ex[3] = 1.0f;
+  } else {
+    ex[0] = ex[1] = ex[2] = ex[3] = 0.0f;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float s = g == 0 ? sh[i] : (g == 1 ? sh[4 + i] : (g == 2 ? sh[8 + i] : sh[12 + i]));
+    f[8 + i] = valid ? s : 0.0f;
+    f[12 + i] = valid ? ex[i] : 0.0f;
+  }
+}
+
+// Row-major [row][64] activation / gradient buffers for the weight
+// gradients: a lane stores its features 16 mt + 4 g + {0..3} as one 8-byte
+// write per mt (p points at feature 4 g of the row).
+template <class O>
+__device__ __forceinline__ void store4(uint16_t* p, const float v[4]) {
+  uint2 u;
+  u.x = uint32_t(O::bits(v[0])) | (uint32_t(O::bits(v[1])) << 16);
+  u.y = uint32_t(O::bits(v[2])) | (uint32_t(O::bits(v[3])) << 16);
+  *reinterpret_cast<uint2*>(p) = u;
+}
+template <class O>
+__device__ __forceinline__ void store_row4x4(uint16_t* p, const float h[4][4]) {
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) store4<O>(p + 16 * mt, h[mt]);
+}
+
+// v0 / v1: the two K halves of a layer input held as h[mt][j] (feature
+// 16 mt + 4 g + j) -> MFMA B operands.
+__device__ __forceinline__ void split_b(const float h[4][4], float v0[8], float v1[8]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    v0[i] = h[0][i];
+    v0[4 + i] = h[1][i];
+    v1[i] = h[2][i];
+    v1[4 + i] = h[3][i];
+  }
+}
+
 template <int P, bool TRAIN>
 __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
   typedef Op<P> O;
@@ -223,14 +288,19 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
   const int lane = int(__lane_id());
   const int g = lane >> 4, r16 = lane & 15;
   const uint32_t T_ = 1u << a.G.log2T;
-  const uint32_t tiles = a.npad / 16u;
+  const uint32_t n = (!TRAIN && a.n_ptr) ? *a.n_ptr : a.n;
+  const uint32_t tiles = TRAIN ? a.npad / 16u : (n + 15u) / 16u;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
   float loss_lane = 0.0f;
   float amax_lane[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
   for (uint32_t tile = wave; tile < tiles; tile += nwaves) {
+    // The weight operands are loop-invariant; without this barrier the
+    // compiler hoists all 40 of them out of the tile loop into ~160 VGPRs.
+    // Re-reading them per tile hits L1 and keeps occupancy up.
+    asm volatile("" : "+s"(pf));
     const uint32_t row = tile * 16u + uint32_t(r16);
-    const bool valid = row < a.n;
+    const bool valid = row < n;
     float4 r0 = make_float4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
     if (valid) {
       r0 = a.rec[4 * size_t(row)];
@@ -240,75 +310,40 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
     }
     float px = fminf(fmaxf(r0.x, 0.0f), 1.0f), py = fminf(fmaxf(r0.y, 0.0f), 1.0f),
           pz = fminf(fmaxf(r0.z, 0.0f), 1.0f);
-    // ---- encoding: this lane group's 16 features, f[ks*8 + i] = feature kappa(g, ks, i)
-    float f[16];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
-      Cell c = cell_of(a.G, lv, px, py, pz);
-      float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
-        float w = corner_w(c, k);
-        float e0, e1;
-        load_entry(a.table16, size_t(lv) * T_ + idx, &e0, &e1);
-        s0 += w * e0;
-        s1 += w * e1;
-      }
-      f[2 * q] = valid ? s0 : 0.0f;
-      f[2 * q + 1] = valid ? s1 : 0.0f;
-    }
+    float cur[4][4];  // current layer input, feature 16 mt + 4 g + j
     {
-      float sh[16];
-      sh16(r1.x, r1.y, r1.z, sh);
-      float ex[4];
-      if (g == 0) {
-        ex[0] = r2.x; ex[1] = r2.y; ex[2] = r2.z; ex[3] = r0.w;
-      } else if (g == 1) {
-        ex[0] = r3.x; ex[1] = r3.y; ex[2] = r3.z; ex[3] = 1.0f;
-      } else {
-        ex[0] = ex[1] = ex[2] = ex[3] = 0.0f;
-      }
+      float f[16];
+      encode16(a, T_, g, valid, px, py, pz, r0, r1, r2, r3, f);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float s = g == 0 ? sh[i] : (g == 1 ? sh[4 + i] : (g == 2 ? sh[8 + i] : sh[12 + i]));
-        f[8 + i] = valid ? s : 0.0f;
-        f[12 + i] = valid ? ex[i] : 0.0f;
-      }
+      for (int i = 0; i < 16; ++i) cur[2 * (i >> 3) + ((i & 7) >> 2)][i & 3] = f[i];
     }
-    // h[L][mt][j]: input of layer L in the accumulator layout (feature 16*mt + 4*g + j)
-    float h[5][4][4];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      int ks = i >> 3, ii = i & 7;
-      int mt = 2 * ks + (ii >> 2), j = ii & 3;
-      h[0][mt][j] = f[i];
-    }
+    uint32_t mask[kHidden + 1];  // training: bit (4 mt + j) = layer input > 0
+    float y[4];
     // ---- forward
 #pragma unroll
     for (int L = 0; L <= kHidden; ++L) {
+      asm volatile("" : "+s"(pf));  // one layer's weights live at a time
       float v0[8], v1[8];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        v0[i] = h[L][0][i];
-        v0[4 + i] = h[L][1][i];
-        v1[i] = h[L][2][i];
-        v1[4 + i] = h[L][3][i];
+      split_b(cur, v0, v1);
+      float sa = 1.0f, deq = 1.0f;
+      if constexpr (P == AQUA_NRC_FP8) {
+        sa = a.ascale[L];
+        deq = 1.0f / (sa * a.wscale[L]);
       }
-      float sa = 1.0f;
-      if (P == AQUA_NRC_FP8) sa = a.sc.a[L];
       T b0 = O::make(v0, sa), b1 = O::make(v1, sa);
       if constexpr (TRAIN) {
         float m = 0.0f;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) m = fmaxf(m, fmaxf(fabsf(v0[i]), fabsf(v1[i])));
-        amax_lane[L] = fmaxf(amax_lane[L], m);
-        uint16_t* HTL = a.HT + size_t(L) * 64 * a.npad;
+        uint32_t bits = 0;
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) HTL[size_t(16 * mt + 4 * g + j) * a.npad + row] = O::bits(h[L][mt][j]);
+          for (int j = 0; j < 4; ++j) {
+            m = fmaxf(m, fabsf(cur[mt][j]));
+            bits |= (cur[mt][j] > 0.0f ? 1u : 0u) << (4 * mt + j);
+          }
+        mask[L] = bits;
+        amax_lane[L] = fmaxf(amax_lane[L], m);
+        store_row4x4<O>(a.HT + (size_t(L) * a.npad + row) * 64 + 4 * g, cur);
       }
       if (L < kHidden) {
 #pragma unroll
@@ -316,105 +351,87 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
           f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
           acc = O::mma(O::unpack(pf[((L * 4 + mt) * 2 + 0) * 64 + lane]), b0, acc);
           acc = O::mma(O::unpack(pf[((L * 4 + mt) * 2 + 1) * 64 + lane]), b1, acc);
-          float deq = 1.0f;
-          if (P == AQUA_NRC_FP8) deq = 1.0f / (a.sc.a[L] * a.sc.w[L]);
 #pragma unroll
-          for (int j = 0; j < 4; ++j) h[L + 1][mt][j] = fmaxf(acc[j] * deq, 0.0f);
+          for (int j = 0; j < 4; ++j) cur[mt][j] = fmaxf(acc[j] * deq, 0.0f);
         }
       } else {
         f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
         acc = O::mma(O::unpack(pf[kHidden * 8 * 64 + 0 * 64 + lane]), b0, acc);
         acc = O::mma(O::unpack(pf[kHidden * 8 * 64 + 1 * 64 + lane]), b1, acc);
-        float deq = 1.0f;
-        if (P == AQUA_NRC_FP8) deq = 1.0f / (a.sc.a[L] * a.sc.w[L]);
-        float y[4] = {acc[0] * deq, acc[1] * deq, acc[2] * deq, acc[3] * deq};
-        if constexpr (!TRAIN) {
-          if (g == 0 && valid) {
-            a.out[3 * size_t(row) + 0] = y[0];
-            a.out[3 * size_t(row) + 1] = y[1];
-            a.out[3 * size_t(row) + 2] = y[2];
-          }
-        } else {
-          // ---- loss gradient (unscaled: the 1 / (3 n) factor is applied in Adam)
-          float dy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-          if (g == 0 && valid) {
-            float4 t = a.tgt[row];
-            float tv[3] = {t.x, t.y, t.z};
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-              float den = y[j] * y[j] + 0.01f;
-              float d = y[j] - tv[j];
-              loss_lane += d * d / den;
-              dy[j] = 2.0f * d / den;
-            }
-          }
-          uint16_t* DT4 = a.DT + size_t(kHidden) * 64 * a.npad;
+        for (int j = 0; j < 4; ++j) y[j] = acc[j] * deq;
+      }
+    }
+    if constexpr (!TRAIN) {
+      if (g == 0 && valid) {
+        a.out[3 * size_t(row) + 0] = y[0];
+        a.out[3 * size_t(row) + 1] = y[1];
+        a.out[3 * size_t(row) + 2] = y[2];
+      }
+    } else {
+      // ---- loss gradient (unscaled: the 1 / (3 n) factor is applied in Adam)
+      float dy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (g == 0 && valid) {
+        float4 t = a.tgt[row];
+        float tv[3] = {t.x, t.y, t.z};
 #pragma unroll
-          for (int j = 0; j < 4; ++j) DT4[size_t(4 * g + j) * a.npad + row] = O::bits(dy[j]);
-          // ---- backward
-          const Pk* pb = reinterpret_cast<const Pk*>(a.pack_bwd);
-          float dz[4][4];
-          {
-            float v[8] = {dy[0], dy[1], dy[2], dy[3], 0.0f, 0.0f, 0.0f, 0.0f};
-            T bd = O::make(v, 1.0f);
+        for (int j = 0; j < 3; ++j) {
+          float den = y[j] * y[j] + 0.01f;
+          float d = y[j] - tv[j];
+          loss_lane += d * d / den;
+          dy[j] = 2.0f * d / den;
+        }
+      }
+      store4<O>(a.DT + (size_t(kHidden) * a.npad + row) * 64 + 4 * g, dy);
+      // ---- backward: dz in the accumulator layout, masks from the forward
+      const Pk* pb = reinterpret_cast<const Pk*>(a.pack_bwd);
+      asm volatile("" : "+s"(pb));  // as for pf: keep the loads in the loop
+      float dz[4][4];
+      {
+        float v[8] = {dy[0], dy[1], dy[2], dy[3], 0.0f, 0.0f, 0.0f, 0.0f};
+        T bd = O::make(v, 1.0f);
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt) {
-              f4v acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
-              acc2 = O::mma(O::unpack(pb[kHidden * 8 * 64 + mt * 64 + lane]), bd, acc2);
+        for (int mt = 0; mt < 4; ++mt) {
+          f4v acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
+          acc2 = O::mma(O::unpack(pb[kHidden * 8 * 64 + mt * 64 + lane]), bd, acc2);
 #pragma unroll
-              for (int j = 0; j < 4; ++j) dz[mt][j] = h[kHidden][mt][j] > 0.0f ? acc2[j] : 0.0f;
-            }
-          }
+          for (int j = 0; j < 4; ++j) dz[mt][j] = ((mask[kHidden] >> (4 * mt + j)) & 1u) ? acc2[j] : 0.0f;
+        }
+      }
 #pragma unroll
-          for (int LL = kHidden - 1; LL >= 0; --LL) {
-            uint16_t* DTL = a.DT + size_t(LL) * 64 * a.npad;
-            float v0b[8], v1b[8];
+      for (int LL = kHidden - 1; LL >= 0; --LL) {
+        asm volatile("" : "+s"(pb));
+        store_row4x4<O>(a.DT + (size_t(LL) * a.npad + row) * 64 + 4 * g, dz);
+        float v0b[8], v1b[8];
+        split_b(dz, v0b, v1b);
+        T bd0 = O::make(v0b, 1.0f), bd1 = O::make(v1b, 1.0f);
 #pragma unroll
-            for (int mt = 0; mt < 4; ++mt)
+        for (int mt = 0; mt < 4; ++mt) {
+          f4v acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
+          acc2 = O::mma(O::unpack(pb[((LL * 4 + mt) * 2 + 0) * 64 + lane]), bd0, acc2);
+          acc2 = O::mma(O::unpack(pb[((LL * 4 + mt) * 2 + 1) * 64 + lane]), bd1, acc2);
 #pragma unroll
-              for (int j = 0; j < 4; ++j) DTL[size_t(16 * mt + 4 * g + j) * a.npad + row] = O::bits(dz[mt][j]);
+          for (int j = 0; j < 4; ++j)
+            dz[mt][j] = (LL == 0 || ((mask[LL] >> (4 * mt + j)) & 1u)) ? acc2[j] : 0.0f;
+        }
+      }
+      // ---- hash-grid gradient scatter: dz now holds d loss / d input
+      // feature; level q's two features are dz[q < 2 ? 0 : 1][2 (q % 2) + {0,1}]
+      if (valid) {
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+          int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
+          float d0 = q < 2 ? dz[0][2 * (q & 1)] : dz[1][2 * (q & 1)];
+          float d1 = q < 2 ? dz[0][2 * (q & 1) + 1] : dz[1][2 * (q & 1) + 1];
+          if (d0 == 0.0f && d1 == 0.0f) continue;
+          Cell c = cell_of(a.G, lv, px, py, pz);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              v0b[i] = dz[0][i];
-              v0b[4 + i] = dz[1][i];
-              v1b[i] = dz[2][i];
-              v1b[4 + i] = dz[3][i];
-            }
-            T bd0 = O::make(v0b, 1.0f), bd1 = O::make(v1b, 1.0f);
-            float dh[4][4];
-#pragma unroll
-            for (int mt = 0; mt < 4; ++mt) {
-              f4v acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
-              acc2 = O::mma(O::unpack(pb[((LL * 4 + mt) * 2 + 0) * 64 + lane]), bd0, acc2);
-              acc2 = O::mma(O::unpack(pb[((LL * 4 + mt) * 2 + 1) * 64 + lane]), bd1, acc2);
-#pragma unroll
-              for (int j = 0; j < 4; ++j) dh[mt][j] = acc2[j];
-            }
-            if (LL > 0) {
-#pragma unroll
-              for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) dz[mt][j] = h[LL][mt][j] > 0.0f ? dh[mt][j] : 0.0f;
-            } else if (valid) {
-              // ---- hash-grid gradient scatter: features of level q live in
-              // dh[q < 2 ? 0 : 1][2*(q%2) + {0,1}]
-#pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
-                float d0 = dh[q >> 1][2 * (q & 1)], d1 = dh[q >> 1][2 * (q & 1) + 1];
-                if (d0 == 0.0f && d1 == 0.0f) continue;
-                Cell c = cell_of(a.G, lv, px, py, pz);
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                  uint32_t idx =
-                      grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
-                  float w = corner_w(c, k);
-                  float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
-                  atomicAdd(gp, w * d0);
-                  atomicAdd(gp + 1, w * d1);
-                }
-              }
-            }
+          for (int k = 0; k < 8; ++k) {
+            uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
+            float w = corner_w(c, k);
+            float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
+            atomicAdd(gp, w * d0);
+            atomicAdd(gp + 1, w * d1);
           }
         }
       }
@@ -432,32 +449,55 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
 }
 
 // ---- weight gradients: dW_L = H_L^T dZ_L, one partial per row chunk ----------
+//
+// H_L and dZ_L are row-major [row][64] 16-bit buffers. Each 32-row step the
+// block stages both tiles in LDS, transposed to [feature][row], so that a
+// lane's MFMA operand (8 consecutive rows of one feature) is one 16-byte LDS
+// read. Wave w owns input features 16 w .. 16 w + 15.
+
+constexpr int kWgLds = 40;  // LDS row stride in halves (16-byte aligned rows)
 
 template <int P>
 __global__ void __launch_bounds__(kBlock) k_nrc_wgrad(const uint16_t* HT, const uint16_t* DT, uint32_t npad,
                                                       float* partial) {
   typedef Op<P> O;
+  __shared__ __align__(16) uint16_t sH[64 * kWgLds];
+  __shared__ __align__(16) uint16_t sD[64 * kWgLds];
   const int L = blockIdx.y;
   const uint32_t chunk = blockIdx.x;
+  const int t = threadIdx.x;
   const int lane = int(__lane_id());
-  const int mt = threadIdx.x >> 6;   // wave -> 16-row block of the input features
+  const int mt = t >> 6;
   const int nout = L < kHidden ? 64 : 16;
   const int nts = nout / 16;
   const uint16_t* H = HT + size_t(L) * 64 * npad;
   const uint16_t* D = DT + size_t(L) * 64 * npad;
+  const int lr = t >> 3, lc = 8 * (t & 7);  // staging: row, first feature
+  const bool dload = lc < nout;
   uint32_t b_begin = chunk * kChunkRows, b_end = min(npad, b_begin + uint32_t(kChunkRows));
   f4v acc[4];
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) acc[nt] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
   for (uint32_t b0 = b_begin; b0 < b_end; b0 += 32) {
-    uint32_t kk = b0 + 8u * uint32_t(lane >> 4);
-    uint4 ua = *reinterpret_cast<const uint4*>(H + size_t(16 * mt + (lane & 15)) * npad + kk);
+    uint4 h = *reinterpret_cast<const uint4*>(H + size_t(b0 + lr) * 64 + lc);
+    uint4 d = make_uint4(0, 0, 0, 0);
+    if (dload) d = *reinterpret_cast<const uint4*>(D + size_t(b0 + lr) * 64 + lc);
+    uint32_t hw[4] = {h.x, h.y, h.z, h.w}, dw[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      sH[(lc + i) * kWgLds + lr] = uint16_t(hw[i >> 1] >> (16 * (i & 1)));
+      if (dload) sD[(lc + i) * kWgLds + lr] = uint16_t(dw[i >> 1] >> (16 * (i & 1)));
+    }
+    __syncthreads();
+    const int kq = 8 * (lane >> 4);
+    uint4 ua = *reinterpret_cast<const uint4*>(&sH[(16 * mt + (lane & 15)) * kWgLds + kq]);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
       if (nt >= nts) break;
-      uint4 ub = *reinterpret_cast<const uint4*>(D + size_t(16 * nt + (lane & 15)) * npad + kk);
+      uint4 ub = *reinterpret_cast<const uint4*>(&sD[(16 * nt + (lane & 15)) * kWgLds + kq]);
       acc[nt] = O::mma(O::unpack(ua), O::unpack(ub), acc[nt]);
     }
+    __syncthreads();
   }
   size_t base = size_t(chunk) * kMlp + size_t(L) * 4096;
 #pragma unroll
@@ -741,25 +781,21 @@ struct aqua_nrc {
 
 namespace aqua {
 
-// For the renderer: device-side query of n records already on the NRC's
-// device; writes 3 floats per record to d_out. Stream-ordered on `s`.
-int nrc_infer_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, float* d_out, uint32_t precision,
-                     hipStream_t s) {
-  if (n == 0) return AQUA_OK;
-  FwdArgs a = nrc->args(n, d_rec, nullptr);
+// For the renderer: device-side query of up to `n_max` records already on the
+// NRC's device; the actual count is read by the kernel from `d_count` (or is
+// n_max when d_count is null). Writes 3 floats per record to d_out.
+// Stream-ordered on `s`, no host synchronisation.
+int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* d_out, uint32_t precision,
+                     hipStream_t s, const uint32_t* d_count) {
+  if (n_max == 0) return AQUA_OK;
+  FwdArgs a = nrc->args(n_max, d_rec, nullptr);
   a.out = d_out;
+  a.n_ptr = d_count;
+  a.wscale = nrc->wscale.p;
+  a.ascale = nrc->ascale.p;
   uint32_t npad = a.npad;
   if (precision == AQUA_NRC_FP8) {
     a.pack_fwd = nrc->pack_fp8.p;
-    // scales are read from device memory by the host once per call
-    float ws[5], as[5];
-    HIPCHK(hipMemcpyAsync(ws, nrc->wscale.p, sizeof ws, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(as, nrc->ascale.p, sizeof as, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    for (int L = 0; L < 5; ++L) {
-      a.sc.w[L] = ws[L];
-      a.sc.a[L] = as[L];
-    }
     k_nrc_fwd<AQUA_NRC_FP8, false><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
   } else {
     bool native = precision == nrc->cfg.train_precision;
@@ -1048,7 +1084,8 @@ int aqua_nrc_infer(aqua_nrc* nrc, uint32_t n, const float* records, uint32_t pre
   HIPCHK(hipSetDevice(nrc->device));
   if (int e = upload_records(nrc, n, records, nullptr)) return e;
   if (int e = nrc->out_buf.alloc(size_t(n) * 3)) return e;
-  if (int e = aqua::nrc_infer_device(nrc, n, nrc->rec_buf.p, nrc->out_buf.p, precision, nrc->stream)) return e;
+  if (int e = aqua::nrc_infer_device(nrc, n, nrc->rec_buf.p, nrc->out_buf.p, precision, nrc->stream, nullptr))
+    return e;
   HIPCHK(hipMemcpyAsync(rgb, nrc->out_buf.p, size_t(n) * 12, hipMemcpyDeviceToHost, nrc->stream));
   HIPCHK(hipStreamSynchronize(nrc->stream));
   return AQUA_OK;

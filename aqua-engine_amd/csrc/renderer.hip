@@ -338,14 +338,15 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
 
 // Adds throughput x max(cache, 0) to the radiance of each querying path.
 __global__ void k_nrc_apply(const float4* qaux, const float* qout, const uint32_t* count, float4* Lbuf) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= *count) return;
-  float4 a = qaux[i];
-  uint32_t path = __float_as_uint(a.w);
-  float r = fmaxf(qout[3 * size_t(i)], 0.0f), g = fmaxf(qout[3 * size_t(i) + 1], 0.0f),
-        bl = fmaxf(qout[3 * size_t(i) + 2], 0.0f);
-  float4 l = Lbuf[path];
-  Lbuf[path] = make_float4(l.x + a.x * r, l.y + a.y * g, l.z + a.z * bl, 0.0f);
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 a = qaux[i];
+    uint32_t path = __float_as_uint(a.w);
+    float r = fmaxf(qout[3 * size_t(i)], 0.0f), g = fmaxf(qout[3 * size_t(i) + 1], 0.0f),
+          bl = fmaxf(qout[3 * size_t(i) + 2], 0.0f);
+    float4 l = Lbuf[path];
+    Lbuf[path] = make_float4(l.x + a.x * r, l.y + a.y * g, l.z + a.z * bl, 0.0f);
+  }
 }
 
 // Training targets: the radiance a training path gathered after a recorded
@@ -620,6 +621,7 @@ struct aqua_renderer {
   DevBuf<double> err_partial;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> kev;
+  hipEvent_t nev[2] = {nullptr, nullptr};  // around the cache queries
   // neural radiance cache
   aqua_nrc* nrc = nullptr;
   bool nrc_enabled = true;
@@ -635,6 +637,8 @@ struct aqua_renderer {
     }
     if (nrc) aqua_nrc_destroy(nrc);
     for (hipEvent_t e : kev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : nev)
+      if (e) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -702,8 +706,8 @@ struct aqua_renderer {
 };
 
 namespace aqua {
-int nrc_infer_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, float* d_out, uint32_t precision,
-                     hipStream_t s);
+int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* d_out, uint32_t precision,
+                     hipStream_t s, const uint32_t* d_count);
 int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
                      aqua_nrc_stats* stats);
 }  // namespace aqua
@@ -796,28 +800,13 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   uint32_t n_query = 0, n_train_rec = 0;
   float nrc_loss = 0.0f;
   if (pp.nrc) {
-    // cache queries: infer, then add throughput x prediction to each path
-    uint32_t nq = 0;
-    HIPCHK(hipMemcpyAsync(&nq, q.counters + kCntQuery, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    n_query = nq;
+    // cache queries: infer, then add throughput x prediction to each path;
+    // the query count stays on the device (no host round trip)
     uint32_t prec = r->cfg.nrc_precision;
-    hipEvent_t a, b2;
-    HIPCHK(hipEventCreate(&a));
-    HIPCHK(hipEventCreate(&b2));
-    HIPCHK(hipEventRecord(a, s));
-    if (nq) {
-      if (int e = aqua::nrc_infer_device(r->nrc, nq, r->qrec.p, r->qout.p, prec, s)) return e;
-      k_nrc_apply<<<int((nq + kBlock - 1) / kBlock), kBlock, 0, s>>>(r->qaux.p, r->qout.p, q.counters + kCntQuery,
-                                                                     q.Lbuf);
-    }
-    HIPCHK(hipEventRecord(b2, s));
-    HIPCHK(hipEventSynchronize(b2));
-    float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, a, b2));
-    ms_nrc += ms;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b2);
+    if (timing) HIPCHK(hipEventRecord(r->nev[0], s));
+    if (int e = aqua::nrc_infer_device(r->nrc, n, r->qrec.p, r->qout.p, prec, s, q.counters + kCntQuery)) return e;
+    k_nrc_apply<<<grid_for(n), kBlock, 0, s>>>(r->qaux.p, r->qout.p, q.counters + kCntQuery, q.Lbuf);
+    if (timing) HIPCHK(hipEventRecord(r->nev[1], s));
   }
   int gf = int((pp.npix + kBlock - 1) / kBlock);
   if (fp64)
@@ -858,10 +847,18 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   uint32_t hc[kCntWords];
   HIPCHK(hipMemcpyAsync(hc, q.counters, sizeof hc, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (pp.nrc) {
+    n_query = hc[kCntQuery];
+    if (timing) {
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, r->nev[0], r->nev[1]));
+      ms_nrc += ms;
+    }
+  }
   if (st) {
     st->nrc_queries += n_query;
     st->nrc_train_records += n_train_rec;
-    if (pp.nrc) st->nrc_loss = nrc_loss;
+    if (pp.nrc && n_slots) st->nrc_loss = nrc_loss;
     st->ms_nrc += ms_nrc;
     st->camera_rays += hc[kCntRays];
     for (uint32_t b = 1; b <= pp.max_depth; ++b) st->extension_rays += hc[kCntRays + b];
@@ -912,10 +909,12 @@ int prepare_frame(aqua_renderer* r, const aqua_camera* cam_in, uint32_t spp, aqu
 // Cache parameters of one pass of `n` paths: every stride-th path (from a
 // per-pass offset) is a training path, the rest query at the query vertex.
 int nrc_pass_params(aqua_renderer* r, uint32_t n, uint32_t frame, uint32_t sample0, PassParams* pp) {
+  // The cache trains once per frame, on the frame's first pass; later
+  // passes of the same frame only query.
   uint32_t want = std::max(1u, r->cfg.nrc_train_paths);
   uint32_t stride = std::max(1u, n / want);
   uint32_t offset = uint32_t((uint64_t(frame) * 2654435761ull + uint64_t(sample0) * 40503ull) % stride);
-  uint32_t n_train = std::min(want, (n - offset + stride - 1) / stride);
+  uint32_t n_train = sample0 == 0 ? std::min(want, (n - offset + stride - 1) / stride) : 0u;
   pp->nrc = 1;
   pp->query_depth = r->cfg.nrc_query_vertex >= 1 ? r->cfg.nrc_query_vertex - 1 : 0;
   pp->train_stride = stride;
@@ -970,6 +969,7 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   HIPCHK(hipEventCreate(&r->ev1));
   r->kev.resize(4 * (kMaxBounces + 1));
   for (hipEvent_t& e : r->kev) HIPCHK(hipEventCreate(&e));
+  for (hipEvent_t& e : r->nev) HIPCHK(hipEventCreate(&e));
 
   // Geometry in BVH leaf order.
   const uint32_t n = uint32_t(scene->mat_ids.size());

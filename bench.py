@@ -72,6 +72,121 @@ def cpu_baseline(scene, aqua, max_depth, target_s=12.0):
     }
 
 
+def nrc_train_leg(aqua, rows=1 << 18, steps=20, precision=0):
+    """Cache training step alone on samples the path tracer produced: box room,
+    1920x1080, training suffix paths only, then `steps` Adam steps on 2^18 of
+    those samples resident in HBM."""
+    with tempfile.TemporaryDirectory() as d:
+        scene = aqua.Scene.load(aqua.procgen(aqua.PROCGEN_BOX_ROOM, d, width=1920, height=1080))
+    cfg = aqua.integrator_default()
+    cfg.type = aqua.INTEGRATOR_NRC
+    cfg.max_depth = 8
+    cfg.learning_rate = 1e-2
+    cfg.nrc_precision = precision
+    cfg.nrc_train_paths = 1 << 18
+    r = aqua.Renderer(scene, cfg)
+    n = 0
+    for f in range(4):
+        r.render(1, frame=f)
+        n, _, _ = r.nrc_batch()
+        if n >= rows:
+            break
+    n_use = min(n, rows)
+    r.nrc_train_batch(steps=2)  # warm-up
+    n0, a, b = r.nrc_batch()
+    st = aqua.NrcStats()
+    aqua._check(aqua.lib().aqua_nrc_train_device(r.nrc_handle(), n_use, a, b, steps, aqua.C.byref(st)))
+    ms = st.ms_total / steps
+    table_params = 16 * (1 << 19) * 2
+    mlp_params = 4 * 64 * 64 + 64 * 16
+    # bytes: records 64 B + target 16 B in, 16 levels x 8 corners x 4 B gathered
+    # and x 8 B scattered per sample; Adam reads param/grad/m/v (16 B) and
+    # writes param/grad/m/v (16 B) + the FP16 copy (2 B) per table parameter.
+    bytes_step = n_use * (80 + 16 * 8 * 4 + 16 * 8 * 8) + table_params * 34 + mlp_params * 34
+    flops_step = 3 * 2 * n_use * (4 * 64 * 64 + 64 * 16)
+    return {
+        "rows": n_use,
+        "precision": ["f16", "bf16"][min(precision, 1)],
+        "ms_per_step": round(ms, 4),
+        "samples_per_s": round(n_use / (ms * 1e-3), 1),
+        "ms_fwd_bwd": round(st.ms_fwd_bwd / steps, 4),
+        "ms_wgrad": round(st.ms_wgrad / steps, 4),
+        "ms_adam": round(st.ms_adam / steps, 4),
+        "loss": round(float(st.loss), 5),
+        "roofline_hbm": {"achieved": round(bytes_step / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "bytes_per_step": bytes_step},
+        "mfma_tflops": round(flops_step / (ms * 1e-3) / 1e12, 3),
+    }
+
+
+def _relmse(img, ref):
+    import numpy as np
+    return float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def nrc_eval_leg(aqua, frames=(60, 120), n_frames=240, oracle_spp=1024, width=1920, height=1080,
+                 max_spp=256, precision=0):
+    """ms/frame to relMSE <= 1e-2 against an FP64 oracle, plain path tracer vs
+    cache, on the box room fly-through (1 spp + cache query at vertex 2, 2^16
+    training suffix paths per frame, plain tracing to depth 8 otherwise)."""
+    with tempfile.TemporaryDirectory() as d:
+        scene = aqua.Scene.load(aqua.procgen(aqua.PROCGEN_BOX_ROOM, d, width=width, height=height))
+    pt = aqua.integrator_default()
+    pt.max_depth = 8
+    ref = aqua.Renderer(scene, pt)
+    ncfg = aqua.integrator_default()
+    ncfg.type = aqua.INTEGRATOR_NRC
+    ncfg.max_depth = 8
+    ncfg.learning_rate = 1e-2
+    ncfg.nrc_precision = precision
+    ncfg.nrc_train_paths = 1 << 16
+    nr = aqua.Renderer(scene, ncfg)
+    out = []
+    next_frame = 0
+    for fe in frames:
+        cam = aqua.camera_flythrough(scene, fe, n_frames, width=width, height=height)
+        ref.render_oracle(0, oracle_spp, frame=fe, camera=cam)
+        oracle = ref.oracle_film(cam)
+        # plain path tracer: smallest spp on the ladder reaching relMSE <= 1e-2
+        pt_res = None
+        spp = 1
+        while spp <= max_spp:
+            st = ref.render(spp, frame=fe + 100000, camera=cam)
+            e = _relmse(ref.film(cam), oracle)
+            pt_res = {"spp": spp, "relmse": round(e, 5), "ms_per_frame": round(st.ms_frame, 3)}
+            if e <= 1e-2:
+                break
+            spp *= 2
+        # cache: fly through the frames before the evaluated one (online
+        # training, 1 spp each), then the smallest spp reaching the target
+        for f in range(next_frame, fe):
+            nr.render(1, frame=f, camera=aqua.camera_flythrough(scene, f, n_frames, width=width, height=height))
+        next_frame = fe + 1
+        nrc_res = None
+        spp = 1
+        while spp <= max_spp:
+            st = nr.render(spp, frame=fe, camera=cam)
+            e = _relmse(nr.film(cam), oracle)
+            nrc_res = {"spp": spp, "relmse": round(e, 5), "ms_per_frame": round(st.ms_frame, 3),
+                       "ms_cache": round(float(st.ms_nrc), 3)}
+            if e <= 1e-2:
+                break
+            spp *= 2
+        out.append({"frame": fe, "pt": pt_res, "nrc": nrc_res})
+    pt_ms = sum(o["pt"]["ms_per_frame"] for o in out) / len(out)
+    nrc_ms = sum(o["nrc"]["ms_per_frame"] for o in out) / len(out)
+    return {
+        "scene": "box room (36 tris), fly-through of %d frames, %dx%d, max depth 8" % (n_frames, width, height),
+        "oracle_spp": oracle_spp,
+        "precision": ["f16", "bf16", "fp8"][precision],
+        "frames": out,
+        "pt_ms_to_1e-2": round(pt_ms, 3),
+        "nrc_ms_to_1e-2": round(nrc_ms, 3),
+        "nrc_over_pt": round(nrc_ms / pt_ms, 4) if pt_ms > 0 else None,
+        "reached": all(o["pt"]["relmse"] <= 1e-2 and o["nrc"]["relmse"] <= 1e-2 for o in out),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +199,9 @@ def main():
     ap.add_argument("--tris", type=int, default=1000000)
     ap.add_argument("--max-depth", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nrc-train", action="store_true", help="add the cache training-step leg")
+    ap.add_argument("--nrc-eval", action="store_true", help="add the ms/frame-to-relMSE leg")
+    ap.add_argument("--only-nrc", action="store_true", help="skip the path-tracer workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -94,6 +212,14 @@ def main():
     if aqua.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.only_nrc:
+        out = {}
+        if args.nrc_train:
+            out["nrc_train"] = nrc_train_leg(aqua)
+        if args.nrc_eval:
+            out["nrc_eval"] = nrc_eval_leg(aqua)
+        print(json.dumps(out))
+        return
 
     with tempfile.TemporaryDirectory() as d:
         path = aqua.procgen(aqua.PROCGEN_INTERIOR, d, seed=0, n_meshes=args.meshes,
@@ -178,6 +304,11 @@ def main():
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, aqua, args.max_depth)
+    del r
+    if args.nrc_train:
+        out["nrc_train"] = nrc_train_leg(aqua)
+    if args.nrc_eval:
+        out["nrc_eval"] = nrc_eval_leg(aqua)
     print(json.dumps(out))
 
 
