@@ -188,6 +188,7 @@ struct FwdArgs {
   const float4* tgt;        // 1 per row (training)
   uint32_t n, npad;
   const uint32_t* n_ptr;    // inference: row count read on the device (else n)
+  const uint32_t* perm;     // inference: row j reads / writes slot perm[j] (else j)
   const uint32_t* table16;  // [level][T] two FP16 features
   const void* pack_fwd;     // A operands
   const void* pack_bwd;
@@ -290,23 +291,34 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
   const uint32_t T_ = 1u << a.G.log2T;
   const uint32_t n = (!TRAIN && a.n_ptr) ? *a.n_ptr : a.n;
   const uint32_t tiles = TRAIN ? a.npad / 16u : (n + 15u) / 16u;
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
+  // XCD x (blocks x, x + 8, ...) takes the x-th contiguous eighth of the
+  // tiles; with spatially sorted rows each XCD's L2 then serves one region
+  // of the grid. The grid is a multiple of 8 blocks.
+  const uint32_t xcd = blockIdx.x & 7u;
+  const uint32_t waves_x = (gridDim.x >> 3) * (blockDim.x >> 6);
+  const uint32_t per_x = (tiles + 7u) >> 3;
+  const uint32_t t_end = min(tiles, (xcd + 1u) * per_x);
   float loss_lane = 0.0f;
   float amax_lane[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-  for (uint32_t tile = wave; tile < tiles; tile += nwaves) {
+  for (uint32_t tile = xcd * per_x + (blockIdx.x >> 3) * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < t_end;
+       tile += waves_x) {
     // The weight operands are loop-invariant; without this barrier the
     // compiler hoists all 40 of them out of the tile loop into ~160 VGPRs.
     // Re-reading them per tile hits L1 and keeps occupancy up.
     asm volatile("" : "+s"(pf));
     const uint32_t row = tile * 16u + uint32_t(r16);
     const bool valid = row < n;
+    uint32_t src = row;
+    if constexpr (!TRAIN) {
+      if (a.perm && valid) src = a.perm[row];
+    }
     float4 r0 = make_float4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
     if (valid) {
-      r0 = a.rec[4 * size_t(row)];
-      r1 = a.rec[4 * size_t(row) + 1];
-      r2 = a.rec[4 * size_t(row) + 2];
-      r3 = a.rec[4 * size_t(row) + 3];
+      r0 = a.rec[4 * size_t(src)];
+      r1 = a.rec[4 * size_t(src) + 1];
+      r2 = a.rec[4 * size_t(src) + 2];
+      r3 = a.rec[4 * size_t(src) + 3];
     }
     float px = fminf(fmaxf(r0.x, 0.0f), 1.0f), py = fminf(fmaxf(r0.y, 0.0f), 1.0f),
           pz = fminf(fmaxf(r0.z, 0.0f), 1.0f);
@@ -364,9 +376,9 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
     }
     if constexpr (!TRAIN) {
       if (g == 0 && valid) {
-        a.out[3 * size_t(row) + 0] = y[0];
-        a.out[3 * size_t(row) + 1] = y[1];
-        a.out[3 * size_t(row) + 2] = y[2];
+        a.out[3 * size_t(src) + 0] = y[0];
+        a.out[3 * size_t(src) + 1] = y[1];
+        a.out[3 * size_t(src) + 2] = y[2];
       }
     } else {
       // ---- loss gradient (unscaled: the 1 / (3 n) factor is applied in Adam)
@@ -772,10 +784,13 @@ struct aqua_nrc {
     return a;
   }
 
+  // Blocks of 4 waves, one wave per 16-row tile up to 32 waves per CU; a
+  // multiple of 8 blocks (the kernel splits the tiles by XCD).
   int fwd_grid(uint32_t npad) const {
     int tiles = int(npad / 16);
     int waves = std::min(tiles, cus * 32);
-    return std::max(1, (waves + 3) / 4);
+    int blocks = (waves + 3) / 4;
+    return std::max(8, (blocks + 7) & ~7);
   }
 };
 
@@ -783,14 +798,17 @@ namespace aqua {
 
 // For the renderer: device-side query of up to `n_max` records already on the
 // NRC's device; the actual count is read by the kernel from `d_count` (or is
-// n_max when d_count is null). Writes 3 floats per record to d_out.
+// n_max when d_count is null). Writes 3 floats per record to d_out. With
+// d_perm, row j of the pass is record d_perm[j] (a spatial order of the
+// records; output still goes to the record's own slot).
 // Stream-ordered on `s`, no host synchronisation.
 int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* d_out, uint32_t precision,
-                     hipStream_t s, const uint32_t* d_count) {
+                     hipStream_t s, const uint32_t* d_count, const uint32_t* d_perm) {
   if (n_max == 0) return AQUA_OK;
   FwdArgs a = nrc->args(n_max, d_rec, nullptr);
   a.out = d_out;
   a.n_ptr = d_count;
+  a.perm = d_perm;
   a.wscale = nrc->wscale.p;
   a.ascale = nrc->ascale.p;
   uint32_t npad = a.npad;
@@ -1084,7 +1102,8 @@ int aqua_nrc_infer(aqua_nrc* nrc, uint32_t n, const float* records, uint32_t pre
   HIPCHK(hipSetDevice(nrc->device));
   if (int e = upload_records(nrc, n, records, nullptr)) return e;
   if (int e = nrc->out_buf.alloc(size_t(n) * 3)) return e;
-  if (int e = aqua::nrc_infer_device(nrc, n, nrc->rec_buf.p, nrc->out_buf.p, precision, nrc->stream, nullptr))
+  if (int e = aqua::nrc_infer_device(nrc, n, nrc->rec_buf.p, nrc->out_buf.p, precision, nrc->stream, nullptr,
+                                     nullptr))
     return e;
   HIPCHK(hipMemcpyAsync(rgb, nrc->out_buf.p, size_t(n) * 12, hipMemcpyDeviceToHost, nrc->stream));
   HIPCHK(hipStreamSynchronize(nrc->stream));

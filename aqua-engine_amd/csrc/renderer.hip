@@ -565,6 +565,12 @@ __global__ void k_fp_probe(const float* a, const float* b, const float* cc, cons
                         __FILE__, __LINE__);                                               \
   } while (0)
 
+namespace aqua {
+hipError_t query_sort_storage(uint32_t n, size_t* bytes);
+hipError_t query_sort(void* temp, size_t temp_bytes, const float4* rec, const uint32_t* d_count, uint32_t n,
+                      uint32_t* keys, uint32_t* idx, hipStream_t s, const uint32_t** perm);
+}  // namespace aqua
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -628,6 +634,9 @@ struct aqua_renderer {
   float bmin[3] = {0, 0, 0}, bscale[3] = {1, 1, 1};
   DevBuf<float4> qrec, qaux, trec, taux, brec, btgt;
   DevBuf<float> qout;
+  DevBuf<uint32_t> skey, sidx;  // query sort: Morton keys, slot order (double buffers)
+  DevBuf<uint8_t> stemp;
+  size_t stemp_bytes = 0;
   uint32_t batch_n = 0;
 
   ~aqua_renderer() {
@@ -699,6 +708,12 @@ struct aqua_renderer {
       if (int e = qrec.ensure(4 * n)) return e;
       if (int e = qaux.ensure(n)) return e;
       if (int e = qout.ensure(3 * n)) return e;
+      if (int e = skey.ensure(2 * n)) return e;
+      if (int e = sidx.ensure(2 * n)) return e;
+      size_t bytes = 0;
+      HIPCHK(aqua::query_sort_storage(uint32_t(n), &bytes));
+      if (int e = stemp.ensure(bytes)) return e;
+      stemp_bytes = bytes;
     }
     cap = n;
     return AQUA_OK;
@@ -707,7 +722,7 @@ struct aqua_renderer {
 
 namespace aqua {
 int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* d_out, uint32_t precision,
-                     hipStream_t s, const uint32_t* d_count);
+                     hipStream_t s, const uint32_t* d_count, const uint32_t* d_perm);
 int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
                      aqua_nrc_stats* stats);
 }  // namespace aqua
@@ -800,11 +815,20 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   uint32_t n_query = 0, n_train_rec = 0;
   float nrc_loss = 0.0f;
   if (pp.nrc) {
-    // cache queries: infer, then add throughput x prediction to each path;
-    // the query count stays on the device (no host round trip)
+    // cache queries: sort by position, infer, then add throughput x
+    // prediction to each path; the query count stays on the device (no host
+    // round trip)
     uint32_t prec = r->cfg.nrc_precision;
     if (timing) HIPCHK(hipEventRecord(r->nev[0], s));
-    if (int e = aqua::nrc_infer_device(r->nrc, n, r->qrec.p, r->qout.p, prec, s, q.counters + kCntQuery)) return e;
+    const uint32_t* perm = nullptr;
+    size_t need = 0;
+    HIPCHK(aqua::query_sort_storage(n, &need));
+    if (need > r->stemp_bytes || 2 * size_t(n) > r->skey.n)
+      return aqua::fail(AQUA_ERR_HIP, "query sort buffers too small for %u queries", n);
+    HIPCHK(aqua::query_sort(r->stemp.p, r->stemp_bytes, r->qrec.p, q.counters + kCntQuery, n, r->skey.p, r->sidx.p,
+                            s, &perm));
+    if (int e = aqua::nrc_infer_device(r->nrc, n, r->qrec.p, r->qout.p, prec, s, q.counters + kCntQuery, perm))
+      return e;
     k_nrc_apply<<<grid_for(n), kBlock, 0, s>>>(r->qaux.p, r->qout.p, q.counters + kCntQuery, q.Lbuf);
     if (timing) HIPCHK(hipEventRecord(r->nev[1], s));
   }
