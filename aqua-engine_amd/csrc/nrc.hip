@@ -36,10 +36,20 @@ namespace {
 constexpr int kLevels = 16;
 constexpr int kHidden = 4;
 constexpr int kMlp = 4 * 64 * 64 + 64 * 16;    // 17408 floats
+static_assert(kMlp % 64 == 0, "partial sums run 64 weights per block");
 constexpr int kFwdPack = kHidden * 8 * 64 + 2 * 64;  // A operands, forward
 constexpr int kBwdPack = kHidden * 8 * 64 + 4 * 64;  // A operands, backward
 constexpr int kBlock = 256;
-constexpr int kChunkRows = 4096;               // rows per weight-gradient partial
+// Rows per weight-gradient partial: about 256 partials per step, so the
+// weight-gradient launch fills the chip, each 256..4096 rows (a multiple of
+// the kernel's 32-row step).
+inline uint32_t chunk_rows(uint32_t npad) {
+  uint32_t c = ((npad / 256u) + 31u) & ~31u;
+  return c < 256u ? 256u : (c > 4096u ? 4096u : c);
+}
+inline uint32_t n_chunks(uint32_t npad) { return (npad + chunk_rows(npad) - 1) / chunk_rows(npad); }
+// Upper bound of n_chunks over every row count up to npad (partials buffer).
+inline uint32_t max_chunks(uint32_t npad) { return std::max(257u, (npad + 4095u) / 4096u); }
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
@@ -188,7 +198,7 @@ struct FwdArgs {
   const float4* tgt;        // 1 per row (training)
   uint32_t n, npad;
   const uint32_t* n_ptr;    // inference: row count read on the device (else n)
-  const uint32_t* perm;     // inference: row j reads / writes slot perm[j] (else j)
+  const uint32_t* perm;     // row j is record perm[j] (else j): spatial order
   const uint32_t* table16;  // [level][T] two FP16 features
   const void* pack_fwd;     // A operands
   const void* pack_bwd;
@@ -309,10 +319,7 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
     asm volatile("" : "+s"(pf));
     const uint32_t row = tile * 16u + uint32_t(r16);
     const bool valid = row < n;
-    uint32_t src = row;
-    if constexpr (!TRAIN) {
-      if (a.perm && valid) src = a.perm[row];
-    }
+    const uint32_t src = (a.perm && valid) ? a.perm[row] : row;
     float4 r0 = make_float4(0, 0, 0, 0), r1 = r0, r2 = r0, r3 = r0;
     if (valid) {
       r0 = a.rec[4 * size_t(src)];
@@ -384,7 +391,7 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
       // ---- loss gradient (unscaled: the 1 / (3 n) factor is applied in Adam)
       float dy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
       if (g == 0 && valid) {
-        float4 t = a.tgt[row];
+        float4 t = a.tgt[src];
         float tv[3] = {t.x, t.y, t.z};
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
@@ -429,14 +436,43 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
       }
       // ---- hash-grid gradient scatter: dz now holds d loss / d input
       // feature; level q's two features are dz[q < 2 ? 0 : 1][2 (q % 2) + {0,1}]
-      if (valid) {
 #pragma unroll 1
-        for (int q = 0; q < 4; ++q) {
-          int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
-          float d0 = q < 2 ? dz[0][2 * (q & 1)] : dz[1][2 * (q & 1)];
-          float d1 = q < 2 ? dz[0][2 * (q & 1) + 1] : dz[1][2 * (q & 1) + 1];
+      for (int q = 0; q < 4; ++q) {
+        int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
+        float d0 = q < 2 ? dz[0][2 * (q & 1)] : dz[1][2 * (q & 1)];
+        float d1 = q < 2 ? dz[0][2 * (q & 1) + 1] : dz[1][2 * (q & 1) + 1];
+        if (!valid) d0 = d1 = 0.0f;
+        Cell c = cell_of(a.G, lv, px, py, pz);
+        if (q < 2) {
+          // Levels 0..7: the 16 rows of a tile (spatially sorted by the
+          // caller) often share an entry. Runs of equal entries across the
+          // lane group's 16 lanes are summed first (segmented suffix sum),
+          // and only the first lane of a run issues the atomics.
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
+            float w = corner_w(c, k);
+            float v0 = w * d0, v1 = w * d1;
+            uint32_t prev = __shfl_up(idx, 1, 16);
+            bool head = r16 == 0 || prev != idx;
+            uint32_t hm = uint32_t(__ballot(head) >> (16 * g)) & 0xFFFFu;
+            uint32_t after = hm >> (r16 + 1);  // bit j: lane r16 + 1 + j starts a run
+#pragma unroll
+            for (int st = 1; st < 16; st <<= 1) {
+              float o0 = __shfl_down(v0, st, 16), o1 = __shfl_down(v1, st, 16);
+              if (r16 + st < 16 && (after & ((1u << st) - 1u)) == 0u) {
+                v0 += o0;
+                v1 += o1;
+              }
+            }
+            if (head && (v0 != 0.0f || v1 != 0.0f)) {
+              float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
+              atomicAdd(gp, v0);
+              atomicAdd(gp + 1, v1);
+            }
+          }
+        } else {
           if (d0 == 0.0f && d1 == 0.0f) continue;
-          Cell c = cell_of(a.G, lv, px, py, pz);
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
             uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
@@ -471,7 +507,7 @@ constexpr int kWgLds = 40;  // LDS row stride in halves (16-byte aligned rows)
 
 template <int P>
 __global__ void __launch_bounds__(kBlock) k_nrc_wgrad(const uint16_t* HT, const uint16_t* DT, uint32_t npad,
-                                                      float* partial) {
+                                                      uint32_t rows_per_chunk, float* partial) {
   typedef Op<P> O;
   __shared__ __align__(16) uint16_t sH[64 * kWgLds];
   __shared__ __align__(16) uint16_t sD[64 * kWgLds];
@@ -486,7 +522,7 @@ __global__ void __launch_bounds__(kBlock) k_nrc_wgrad(const uint16_t* HT, const 
   const uint16_t* D = DT + size_t(L) * 64 * npad;
   const int lr = t >> 3, lc = 8 * (t & 7);  // staging: row, first feature
   const bool dload = lc < nout;
-  uint32_t b_begin = chunk * kChunkRows, b_end = min(npad, b_begin + uint32_t(kChunkRows));
+  uint32_t b_begin = chunk * rows_per_chunk, b_end = min(npad, b_begin + rows_per_chunk);
   f4v acc[4];
 #pragma unroll
   for (int nt = 0; nt < 4; ++nt) acc[nt] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
@@ -540,21 +576,32 @@ __device__ __forceinline__ float adam_one(float p, float gr, float* m, float* v,
   return p - A.lr * mh / (sqrtf(vh) + A.eps);
 }
 
-__global__ void k_nrc_adam_mlp(const float* partial, uint32_t nchunks, float* master, float* m, float* v,
-                               AdamArgs A) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= uint32_t(kMlp)) return;
+// Sum of the weight-gradient partials of weight blockIdx.x * 64 + (t % 64):
+// the block's 4 waves each add every 4th chunk, then the four sums are
+// combined in a fixed order (deterministic). Launch kMlp / 64 blocks of 256.
+__device__ __forceinline__ float sum_partials(const float* partial, uint32_t nchunks, uint32_t* p_out) {
+  __shared__ float red[4][64];
+  const uint32_t t = threadIdx.x, p = blockIdx.x * 64u + (t & 63u);
   float g = 0.0f;
-  for (uint32_t c = 0; c < nchunks; ++c) g += partial[size_t(c) * kMlp + p];
-  master[p] = adam_one(master[p], g * A.gscale, &m[p], &v[p], A);
+  for (uint32_t c = t >> 6; c < nchunks; c += 4) g += partial[size_t(c) * kMlp + p];
+  red[t >> 6][t & 63u] = g;
+  __syncthreads();
+  *p_out = p;
+  return ((red[0][t & 63u] + red[1][t & 63u]) + red[2][t & 63u]) + red[3][t & 63u];
 }
 
-__global__ void k_nrc_reduce_partials(const float* partial, uint32_t nchunks, float* out, float gscale) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= uint32_t(kMlp)) return;
-  float g = 0.0f;
-  for (uint32_t c = 0; c < nchunks; ++c) g += partial[size_t(c) * kMlp + p];
-  out[p] = g * gscale;
+__global__ void __launch_bounds__(256) k_nrc_adam_mlp(const float* partial, uint32_t nchunks, float* master,
+                                                      float* m, float* v, AdamArgs A) {
+  uint32_t p;
+  float g = sum_partials(partial, nchunks, &p);
+  if (threadIdx.x < 64) master[p] = adam_one(master[p], g * A.gscale, &m[p], &v[p], A);
+}
+
+__global__ void __launch_bounds__(256) k_nrc_reduce_partials(const float* partial, uint32_t nchunks, float* out,
+                                                             float gscale) {
+  uint32_t p;
+  float g = sum_partials(partial, nchunks, &p);
+  if (threadIdx.x < 64) out[p] = g * gscale;
 }
 
 __global__ void k_nrc_adam_table(float* master, float* grad, float* m, float* v, uint32_t* table16, size_t n_entries,
@@ -704,6 +751,12 @@ int blocks_for(size_t n, int per = kBlock) { return int((n + per - 1) / per); }
 
 }  // namespace
 
+namespace aqua {
+hipError_t query_sort_storage(uint32_t n, size_t* bytes);
+hipError_t query_sort(void* temp, size_t temp_bytes, const float4* rec, const uint32_t* d_count, uint32_t n,
+                      uint32_t* keys, uint32_t* idx, hipStream_t s, const uint32_t** perm);
+}  // namespace aqua
+
 struct aqua_nrc {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -721,6 +774,8 @@ struct aqua_nrc {
   Buf<float> wscale, ascale, loss;
   Buf<uint32_t> act_amax;
   Buf<uint16_t> HT, DT;
+  Buf<uint32_t> skey, sidx;  // training batch sort: Morton keys, row order (double buffers)
+  Buf<uint8_t> stemp;
   uint32_t cap_rows = 0;
   Buf<float4> rec_buf, tgt_buf;
   Buf<float> out_buf;
@@ -741,8 +796,10 @@ struct aqua_nrc {
     if (npad <= cap_rows) return AQUA_OK;
     if (int e = HT.alloc(size_t(5) * 64 * npad)) return e;
     if (int e = DT.alloc(size_t(5) * 64 * npad)) return e;
-    uint32_t nchunks = (npad + kChunkRows - 1) / kChunkRows;
+    uint32_t nchunks = max_chunks(npad);
     if (int e = partial.alloc(size_t(nchunks) * kMlp)) return e;
+    if (int e = skey.alloc(size_t(2) * npad)) return e;
+    if (int e = sidx.alloc(size_t(2) * npad)) return e;
     cap_rows = npad;
     return AQUA_OK;
   }
@@ -835,8 +892,16 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
   FwdArgs a = nrc->args(n, d_rec, d_tgt);
   a.pack_fwd = nrc->pack_fwd.p;
   a.pack_bwd = nrc->pack_bwd.p;
+  // rows in Morton order: the scatter merges rows that share coarse entries
+  {
+    size_t need = 0;
+    HIPCHK(aqua::query_sort_storage(n, &need));
+    if (int e = nrc->stemp.alloc(need)) return e;  // rocPRIM's need is not monotone in n
+    HIPCHK(aqua::query_sort(nrc->stemp.p, nrc->stemp.n, d_rec, nullptr, n, nrc->skey.p, nrc->sidx.p, s,
+                            &a.perm));
+  }
   uint32_t npad = a.npad;
-  uint32_t nchunks = (npad + kChunkRows - 1) / kChunkRows;
+  uint32_t nchunks = n_chunks(npad);
   double t_fb = 0, t_wg = 0, t_ad = 0;
   float last_loss = 0.0f;
   HIPCHK(hipEventRecord(nrc->ev[0], s));
@@ -861,12 +926,13 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
     HIPCHK(hipEventRecord(nrc->ev[2], s));
     dim3 gw(nchunks, 5);
     if (c.train_precision == AQUA_NRC_BF16)
-      k_nrc_wgrad<AQUA_NRC_BF16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, nrc->partial.p);
+      k_nrc_wgrad<AQUA_NRC_BF16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),
+                                                        nrc->partial.p);
     else
-      k_nrc_wgrad<AQUA_NRC_F16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, nrc->partial.p);
+      k_nrc_wgrad<AQUA_NRC_F16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),
+                                                        nrc->partial.p);
     HIPCHK(hipEventRecord(nrc->ev[3], s));
-    k_nrc_adam_mlp<<<blocks_for(kMlp), kBlock, 0, s>>>(nrc->partial.p, nchunks, nrc->mmaster.p, nrc->mm.p,
-                                                       nrc->mv.p, A);
+    k_nrc_adam_mlp<<<kMlp / 64, 256, 0, s>>>(nrc->partial.p, nchunks, nrc->mmaster.p, nrc->mm.p, nrc->mv.p, A);
     k_nrc_adam_table<<<blocks_for(nrc->entries), kBlock, 0, s>>>(nrc->tmaster.p, nrc->tgrad.p, nrc->tm.p,
                                                                  nrc->tv.p, nrc->t16.p, nrc->entries, A);
     if (int e = nrc->repack()) return e;
@@ -1132,7 +1198,7 @@ int aqua_nrc_gradients(aqua_nrc* nrc, uint32_t n, const float* records, const fl
   FwdArgs a = nrc->args(n, nrc->rec_buf.p, nrc->tgt_buf.p);
   a.pack_fwd = nrc->pack_fwd.p;
   a.pack_bwd = nrc->pack_bwd.p;
-  uint32_t npad = a.npad, nchunks = (npad + kChunkRows - 1) / kChunkRows;
+  uint32_t npad = a.npad, nchunks = n_chunks(npad);
   float gscale = float(1.0 / (3.0 * double(n)));
   k_fill_f32<<<1, 64, 0, s>>>(nrc->loss.p, 1, 0.0f);
   k_fill_u32<<<1, 64, 0, s>>>(nrc->act_amax.p, 5, 0u);
@@ -1142,12 +1208,14 @@ int aqua_nrc_gradients(aqua_nrc* nrc, uint32_t n, const float* records, const fl
     k_nrc_fwd<AQUA_NRC_F16, true><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
   dim3 gw(nchunks, 5);
   if (nrc->cfg.train_precision == AQUA_NRC_BF16)
-    k_nrc_wgrad<AQUA_NRC_BF16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, nrc->partial.p);
+    k_nrc_wgrad<AQUA_NRC_BF16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),
+                                                        nrc->partial.p);
   else
-    k_nrc_wgrad<AQUA_NRC_F16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, nrc->partial.p);
+    k_nrc_wgrad<AQUA_NRC_F16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),
+                                                        nrc->partial.p);
   Buf<float> gm;
   if (int e = gm.alloc(kMlp)) return e;
-  k_nrc_reduce_partials<<<blocks_for(kMlp), kBlock, 0, s>>>(nrc->partial.p, nchunks, gm.p, gscale);
+  k_nrc_reduce_partials<<<kMlp / 64, 256, 0, s>>>(nrc->partial.p, nchunks, gm.p, gscale);
   HIPCHK(hipGetLastError());
   float lsum = 0.0f;
   HIPCHK(hipMemcpyAsync(&lsum, nrc->loss.p, sizeof lsum, hipMemcpyDeviceToHost, s));

@@ -636,7 +636,6 @@ struct aqua_renderer {
   DevBuf<float> qout;
   DevBuf<uint32_t> skey, sidx;  // query sort: Morton keys, slot order (double buffers)
   DevBuf<uint8_t> stemp;
-  size_t stemp_bytes = 0;
   uint32_t batch_n = 0;
 
   ~aqua_renderer() {
@@ -710,10 +709,6 @@ struct aqua_renderer {
       if (int e = qout.ensure(3 * n)) return e;
       if (int e = skey.ensure(2 * n)) return e;
       if (int e = sidx.ensure(2 * n)) return e;
-      size_t bytes = 0;
-      HIPCHK(aqua::query_sort_storage(uint32_t(n), &bytes));
-      if (int e = stemp.ensure(bytes)) return e;
-      stemp_bytes = bytes;
     }
     cap = n;
     return AQUA_OK;
@@ -823,9 +818,10 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     const uint32_t* perm = nullptr;
     size_t need = 0;
     HIPCHK(aqua::query_sort_storage(n, &need));
-    if (need > r->stemp_bytes || 2 * size_t(n) > r->skey.n)
+    if (2 * size_t(n) > r->skey.n || 2 * size_t(n) > r->sidx.n)
       return aqua::fail(AQUA_ERR_HIP, "query sort buffers too small for %u queries", n);
-    HIPCHK(aqua::query_sort(r->stemp.p, r->stemp_bytes, r->qrec.p, q.counters + kCntQuery, n, r->skey.p, r->sidx.p,
+    if (int e = r->stemp.ensure(need)) return e;  // rocPRIM's need is not monotone in n
+    HIPCHK(aqua::query_sort(r->stemp.p, r->stemp.n, r->qrec.p, q.counters + kCntQuery, n, r->skey.p, r->sidx.p,
                             s, &perm));
     if (int e = aqua::nrc_infer_device(r->nrc, n, r->qrec.p, r->qout.p, prec, s, q.counters + kCntQuery, perm))
       return e;

@@ -41,7 +41,7 @@ __device__ __forceinline__ uint32_t cell10(float x) {
 // rec: 4 float4 per row, row 0's xyz = position in [0,1]^3.
 __global__ void k_query_keys(const float4* rec, const uint32_t* count, uint32_t n, uint32_t* key,
                              uint32_t* idx) {
-  const uint32_t m = *count;
+  const uint32_t m = count ? *count : n;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t k = 0xFFFFFFFFu;
     if (i < m) {
@@ -63,10 +63,11 @@ hipError_t query_sort_storage(uint32_t n, size_t* bytes) {
 }
 
 // Sorts the first n query slots by Morton key (slots at or past *d_count
-// last). keys and idx hold 2 n words each (double buffers); *perm receives
-// the sorted slot order (n entries, the first *d_count of them real
-// queries; LSD radix sort is stable, so the real queries precede the
-// padding even where a key saturates). Stream-ordered, no host sync.
+// last; a null d_count means all n are real). keys and idx hold 2 n words
+// each (double buffers); *perm receives the sorted slot order (n entries,
+// the real queries first: LSD radix sort is stable, so they precede the
+// padding even where a key saturates). temp holds temp_bytes of at least
+// query_sort_storage(n). Stream-ordered, no host sync.
 hipError_t query_sort(void* temp, size_t temp_bytes, const float4* rec, const uint32_t* d_count, uint32_t n,
                       uint32_t* keys, uint32_t* idx, hipStream_t s, const uint32_t** perm) {
   size_t blocks = (size_t(n) + 255) / 256;
