@@ -19,11 +19,16 @@ constexpr uint32_t kMaxBounces = 64;
 // counters[] layout
 constexpr uint32_t kCntRays = 0;        // [0, 65): rays in the queue of bounce b
 constexpr uint32_t kCntShadow = 80;     // [80, 144): shadow rays emitted at bounce b
-constexpr uint32_t kCntFetch = 160;     // [160, 225): closest-hit work cursor of bounce b
-constexpr uint32_t kCntFetchSh = 240;   // [240, 304): any-hit work cursor of bounce b
 constexpr uint32_t kCntQuery = 304;     // cache queries emitted this pass
 constexpr uint32_t kCntTrain = 305;     // cache training samples emitted this pass
 constexpr uint32_t kCntWords = 320;
+// cursors[] layout: the persistent traversal kernels' work cursors, 8 per
+// launch (one per XCD range), each on its own 128-byte line.
+constexpr uint32_t kCursorStride = 32;                 // words between cursors
+constexpr uint32_t kCursorSet = 8 * kCursorStride;     // one launch
+constexpr uint32_t kCursorWords = 2 * kMaxBounces * kCursorSet;
+// cursor set of the closest-hit (kind 0) or any-hit (kind 1) launch of bounce b
+__host__ __device__ constexpr uint32_t cursor_set(uint32_t b, uint32_t kind) { return (2 * b + kind) * kCursorSet; }
 // stats64[] layout
 enum : uint32_t {
   kStNodes = 0,
@@ -81,6 +86,7 @@ struct Queues {
   float4* shc;     // shadow contribution
   float4* Lbuf;    // radiance of finished paths, by path id
   uint32_t* counters;
+  uint32_t* cursors;
   unsigned long long* stats;
   // cache queries (compacted) and training slots (train index x depth)
   float4* qrec;    // 4 per query: (pos, roughness), (dir, -), (normal, -), (albedo, -)

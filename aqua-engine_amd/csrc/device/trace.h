@@ -96,6 +96,7 @@ struct DevBvh {
   const float4* nodes;  // 8 per 4-wide node
   const float4* tris;   // 3 per triangle slot: (v0, id), (v1, material), (v2, 0)
   float delta;
+  uint32_t* spill;      // traversal spill stacks: kSpillStack x (grid threads) words
 };
 
 struct TraceCount {
@@ -103,25 +104,34 @@ struct TraceCount {
 };
 
 // ---- traversal stack: a 16-entry ring in LDS (one column per thread) whose
-// oldest entries spill to a private array when it is full. With at most three
+// oldest entries spill to global memory when it is full. With at most three
 // pushes per visited node the host bounds the total depth (3 x tree depth)
 // by kLdsStack + kSpillStack.
+//
+// The spill area is a global buffer the renderer owns, entry i of grid
+// thread t at spill[i * threads + t] (lanes at the same depth coalesce), not
+// a private array: private arrays put the kernel on scratch memory, whose
+// per-launch set-up cost (~0.1 ms at the traversal grids) exceeds the whole
+// traversal of a short queue.
 constexpr int kLdsStack = 16;
 constexpr int kSpillStack = 64;
 
-// Only scalars live in the struct, so they stay in registers; the spill
-// array is a separate private array owned by the kernel.
+// Only scalars live in the struct, so they stay in registers.
 struct TStack {
   uint32_t* lds;    // this thread's column
-  uint32_t* spill;  // kSpillStack entries of private memory
+  uint32_t* spill;  // this thread's first spill entry
   int stride;       // threads per block
+  uint32_t sstride; // threads per grid
   int top, cnt, gsp;
 };
 
-AQ_DEV void st_init(TStack& s, uint32_t* lds_base, uint32_t* spill, int tid, int stride) {
+// Thread (blockIdx, threadIdx) of a grid of gridDim x blockDim threads.
+AQ_DEV void st_init(TStack& s, uint32_t* lds_base, uint32_t* spill_base) {
+  const uint32_t tid = threadIdx.x;
   s.lds = lds_base + tid;
-  s.spill = spill;
-  s.stride = stride;
+  s.sstride = gridDim.x * blockDim.x;
+  s.spill = spill_base + size_t(blockIdx.x) * blockDim.x + tid;
+  s.stride = int(blockDim.x);
   s.top = kLdsStack - 1;
   s.cnt = 0;
   s.gsp = 0;
@@ -129,7 +139,7 @@ AQ_DEV void st_init(TStack& s, uint32_t* lds_base, uint32_t* spill, int tid, int
 AQ_DEV void st_push(TStack& s, uint32_t v) {
   int slot = s.top + 1 == kLdsStack ? 0 : s.top + 1;
   if (s.cnt == kLdsStack)
-    s.spill[s.gsp++] = s.lds[slot * s.stride];
+    s.spill[size_t(s.gsp++) * s.sstride] = s.lds[slot * s.stride];
   else
     ++s.cnt;
   s.lds[slot * s.stride] = v;
@@ -143,7 +153,7 @@ AQ_DEV uint32_t st_pop(TStack& s) {
     --s.cnt;
     return v;
   }
-  return s.spill[--s.gsp];
+  return s.spill[size_t(--s.gsp) * s.sstride];
 }
 
 AQ_DEV int sel4(int4 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }

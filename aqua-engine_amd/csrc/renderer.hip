@@ -41,9 +41,16 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
   return base + below;
 }
 
-__global__ void k_reset(uint32_t* counters, uint32_t initial_rays) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kCntWords) counters[i] = (i == kCntRays) ? initial_rays : 0u;
+// Queue counters of a pass (bounce 0 holds initial_rays) and the first
+// cursor_words work cursors.
+__global__ void k_reset(uint32_t* counters, uint32_t initial_rays, uint32_t* cursors, uint32_t cursor_words) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kCntWords + cursor_words;
+       i += gridDim.x * blockDim.x) {
+    if (i < kCntWords)
+      counters[i] = (i == kCntRays) ? initial_rays : 0u;
+    else
+      cursors[i - kCntWords] = 0u;
+  }
 }
 
 __global__ void k_zero_stats(unsigned long long* st) {
@@ -87,12 +94,57 @@ __global__ void k_camera(PassParams pp, Queues q) {
   }
 }
 
-// Persistent waves: each wave takes the next 64 rays from a work cursor
-// until the queue is exhausted, so long rays do not hold up a static share.
-__device__ __forceinline__ uint32_t wave_fetch(uint32_t* cursor) {
-  uint32_t base = 0;
-  if (__lane_id() == 0) base = atomicAdd(cursor, 64u);
-  return __shfl(base, 0);
+// Persistent traversal waves. The queue [0, n) is cut into 8 contiguous
+// 64-aligned ranges; range x belongs to the blocks b with b % 8 == x, which
+// the hardware deals to XCD x, so a range's rays share that XCD's L2. A wave
+// first takes its static 64-ray chunk of its own range, then further chunks
+// from its range's cursor, then helps the other ranges through theirs. A
+// cursor is read before it is incremented, so exhausted ranges cost no
+// atomics and a short queue costs none at all; each cursor has its own
+// 128-byte line. The grid is a multiple of 8 blocks.
+struct WaveWork {
+  uint32_t n, per, waves_x, wave_x, x;
+  uint32_t* cur;  // 8 cursors, kCursorStride words apart
+  uint32_t phase; // 0: static chunk; 1..8: cursor of range (x + phase - 1) % 8
+};
+
+__device__ __forceinline__ WaveWork ww_init(uint32_t n, uint32_t* cursors) {
+  WaveWork w;
+  w.n = n;
+  w.per = (((n + 7u) >> 3) + 63u) & ~63u;
+  w.x = blockIdx.x & 7u;
+  w.waves_x = (gridDim.x >> 3) * (blockDim.x >> 6);
+  w.wave_x = (blockIdx.x >> 3) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  w.cur = cursors;
+  w.phase = 0;
+  return w;
+}
+
+// Base of the wave's next chunk, or kMiss when every range is done.
+__device__ __forceinline__ uint32_t ww_next(WaveWork& w) {
+  if (w.phase == 0) {
+    w.phase = 1;
+    uint64_t lo = min(uint64_t(w.n), uint64_t(w.x) * w.per), hi = min(uint64_t(w.n), lo + w.per);
+    uint64_t base = lo + 64ull * w.wave_x;
+    if (base < hi) return uint32_t(base);
+  }
+  while (w.phase <= 8u) {
+    const uint32_t r = (w.x + w.phase - 1u) & 7u;
+    const uint64_t lo = min(uint64_t(w.n), uint64_t(r) * w.per), hi = min(uint64_t(w.n), lo + w.per);
+    uint32_t c = kMiss;
+    if (__lane_id() == 0) {
+      uint32_t* p = w.cur + r * kCursorStride;
+      uint32_t seen = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (lo + 64ull * (uint64_t(w.waves_x) + seen) < hi) c = atomicAdd(p, 1u);
+    }
+    c = __shfl(c, 0);
+    if (c != kMiss) {
+      uint64_t base = lo + 64ull * (uint64_t(w.waves_x) + c);
+      if (base < hi) return uint32_t(base);
+    }
+    ++w.phase;
+  }
+  return kMiss;
 }
 
 template <bool COUNT>
@@ -101,14 +153,14 @@ __global__ void __launch_bounds__(kBlock) k_intersect(DevBvh bvh, const float4* 
                                                       const uint32_t* count, uint32_t* cursor, uint2* hit,
                                                       unsigned long long* stats) {
   __shared__ uint32_t lds_stack[kLdsStack * kBlock];
-  uint32_t spill[kSpillStack];
   TStack st;
-  st_init(st, lds_stack, spill, threadIdx.x, kBlock);
+  st_init(st, lds_stack, bvh.spill);
   const uint32_t n = *count;
   TraceCount tc{0u, 0u};
+  WaveWork ww = ww_init(n, cursor);
   while (true) {
-    uint32_t base = wave_fetch(cursor);
-    if (base >= n) break;
+    uint32_t base = ww_next(ww);
+    if (base == kMiss) break;
     uint32_t i = base + __lane_id();
     if (i < n) {
       float4 o = ro[i], d = rd[i];
@@ -390,16 +442,15 @@ __global__ void k_clear_taux(float4* taux, uint32_t n_slots) {
 template <bool COUNT>
 __global__ void __launch_bounds__(kBlock) k_shadow(DevBvh bvh, Queues q, uint32_t b) {
   __shared__ uint32_t lds_stack[kLdsStack * kBlock];
-  uint32_t spill[kSpillStack];
   TStack st;
-  st_init(st, lds_stack, spill, threadIdx.x, kBlock);
+  st_init(st, lds_stack, bvh.spill);
   const uint32_t nx = (b & 1u) ^ 1u;
   const uint32_t n = q.counters[kCntShadow + b];
-  uint32_t* cursor = q.counters + kCntFetchSh + b;
   TraceCount tc{0u, 0u};
+  WaveWork ww = ww_init(n, q.cursors + cursor_set(b, 1));
   while (true) {
-    uint32_t base = wave_fetch(cursor);
-    if (base >= n) break;
+    uint32_t base = ww_next(ww);
+    if (base == kMiss) break;
     uint32_t i = base + __lane_id();
     if (i >= n) continue;
     float4 o = q.sho[i], d = q.shd[i];
@@ -483,14 +534,13 @@ __global__ void k_hits_out(const uint2* hit, const float4* tris, uint32_t n, uin
 __global__ void __launch_bounds__(kBlock) k_occluded(DevBvh bvh, const float4* ro, const float4* rd,
                                                      uint32_t n, uint8_t* out) {
   __shared__ uint32_t lds_stack[kLdsStack * kBlock];
-  uint32_t spill[kSpillStack];
   TStack st;
-  st_init(st, lds_stack, spill, threadIdx.x, kBlock);
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  st_init(st, lds_stack, bvh.spill);
   TraceCount tc;
-  float4 o = ro[i], d = rd[i];
-  out[i] = trace_any<false>(bvh, st, xyz(o), xyz(d), o.w, &tc) ? 1 : 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 o = ro[i], d = rd[i];
+    out[i] = trace_any<false>(bvh, st, xyz(o), xyz(d), o.w, &tc) ? 1 : 0;
+  }
 }
 
 // relMSE / relRMSE partial sums in FP64, one triple per block.
@@ -612,13 +662,14 @@ struct aqua_renderer {
   int persistent_blocks = 2048;  // resident blocks of the traversal kernels
   std::vector<uint32_t> tri_order;
   DevBuf<float4> tris, nodes, mats, lights;
+  DevBuf<uint32_t> spill;  // traversal spill stacks (kSpillStack words per grid thread)
   DevBuf<float> cdf, pmf;
   // path queues
   size_t cap = 0;
   DevBuf<float4> ro[2], rd[2], tp[2], L[2], sho, shd, shc, Lbuf;
   DevBuf<uint4> rng[2];
   DevBuf<uint2> hit;
-  DevBuf<uint32_t> counters;
+  DevBuf<uint32_t> counters, cursors;
   DevBuf<unsigned long long> stats;
   // films
   uint32_t film_w = 0, film_h = 0, film_spp = 0, oracle_spp = 0;
@@ -652,11 +703,19 @@ struct aqua_renderer {
     if (stream) (void)hipStreamDestroy(stream);
   }
 
+  // Blocks of a persistent traversal launch over n rays: a multiple of 8
+  // (one range per XCD), at most the resident count.
+  int trav_grid(size_t n) const {
+    int g = int(std::min<size_t>((n + kBlock - 1) / kBlock, size_t(persistent_blocks)));
+    return std::min(std::max(8, (g + 7) & ~7), persistent_blocks);
+  }
+
   DevScene dev_scene() const {
     DevScene s;
     s.bvh.nodes = nodes.p;
     s.bvh.tris = tris.p;
     s.bvh.delta = delta;
+    s.bvh.spill = spill.p;
     s.mats = mats.p;
     s.lights = lights.p;
     s.light_cdf = cdf.p;
@@ -681,6 +740,7 @@ struct aqua_renderer {
     q.shc = shc.p;
     q.Lbuf = Lbuf.p;
     q.counters = counters.p;
+    q.cursors = cursors.p;
     q.stats = stats.p;
     q.qrec = qrec.p;
     q.qaux = qaux.p;
@@ -778,16 +838,16 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   DevScene sc = r->dev_scene();
   Queues q = r->queues();
   int g = grid_for(n);
-  int gp = std::min(g, r->persistent_blocks);
+  int gp = r->trav_grid(n);
   const uint32_t n_slots = pp.nrc ? pp.n_train * (pp.max_depth + 1u) : 0u;
-  k_reset<<<1, kCntWords, 0, s>>>(q.counters, n);
+  k_reset<<<64, kBlock, 0, s>>>(q.counters, n, q.cursors, cursor_set(pp.max_depth + 1, 0));
   if (n_slots) k_clear_taux<<<int((n_slots + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.taux, n_slots);
   k_camera<<<g, kBlock, 0, s>>>(pp, q);
   bool timing = r->opts.timing != 0;
   for (uint32_t b = 0; b <= pp.max_depth; ++b) {
     uint32_t c = b & 1u;
     const uint32_t* cnt = q.counters + kCntRays + b;
-    uint32_t* cur = q.counters + kCntFetch + b;
+    uint32_t* cur = q.cursors + cursor_set(b, 0);
     hipEvent_t* ev = &r->kev[4 * b];
     if (timing) HIPCHK(hipEventRecord(ev[0], s));
     if (count)
@@ -1073,7 +1133,10 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   if (int e = r->cdf.upload(cdf.data(), cdf.size(), s)) return e;
   if (int e = r->pmf.upload(pmf.data(), pmf.size(), s)) return e;
   if (int e = r->counters.ensure(kCntWords)) return e;
+  if (int e = r->cursors.ensure(kCursorWords)) return e;
   if (int e = r->stats.ensure(kStWords)) return e;
+  // traversal spill stacks of every thread of the (capped) traversal grids
+  if (int e = r->spill.ensure(size_t(kSpillStack) * r->persistent_blocks * kBlock)) return e;
   if (cfg->type == AQUA_INTEGRATOR_NRC) {
     if (cfg->nrc_precision > AQUA_NRC_FP8)
       return aqua::fail(AQUA_ERR_INVALID_ARG, "unknown nrc_precision %u", cfg->nrc_precision);
@@ -1254,10 +1317,10 @@ int aqua_primary_hits(aqua_renderer* r, const aqua_camera* cam_in, uint32_t fram
   if (int e = ids.ensure(n)) return e;
   if (int e = ts.ensure(n)) return e;
   hipStream_t s = r->stream;
-  k_reset<<<1, kCntWords, 0, s>>>(q.counters, n);
+  k_reset<<<8, kBlock, 0, s>>>(q.counters, n, q.cursors, kCursorSet);
   k_camera<<<grid_for(n), kBlock, 0, s>>>(pp, q);
-  k_intersect<false><<<std::min(grid_for(n), r->persistent_blocks), kBlock, 0, s>>>(
-      sc.bvh, q.ro[0], q.rd[0], q.counters, q.counters + kCntFetch, q.hit, q.stats);
+  k_intersect<false><<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, q.ro[0], q.rd[0], q.counters, q.cursors, q.hit,
+                                                         q.stats);
   k_hits_out<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.hit, sc.bvh.tris, n, ids.p, ts.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(prim_ids, ids.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1297,9 +1360,9 @@ int aqua_trace_closest(aqua_renderer* r, uint32_t n, const float* rays, uint32_t
   if (int e = ids.ensure(n)) return e;
   if (int e = ts.ensure(n)) return e;
   hipStream_t s = r->stream;
-  k_reset<<<1, kCntWords, 0, s>>>(q.counters, n);
-  k_intersect<false><<<std::min(grid_for(n), r->persistent_blocks), kBlock, 0, s>>>(
-      sc.bvh, q.ro[0], q.rd[0], q.counters, q.counters + kCntFetch, q.hit, q.stats);
+  k_reset<<<8, kBlock, 0, s>>>(q.counters, n, q.cursors, kCursorSet);
+  k_intersect<false><<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, q.ro[0], q.rd[0], q.counters, q.cursors, q.hit,
+                                                         q.stats);
   k_hits_out<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.hit, sc.bvh.tris, n, ids.p, ts.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(prim_ids, ids.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1317,7 +1380,7 @@ int aqua_trace_occluded(aqua_renderer* r, uint32_t n, const float* rays, uint8_t
   DevBuf<uint8_t> out;
   if (int e = out.ensure(n)) return e;
   hipStream_t s = r->stream;
-  k_occluded<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(sc.bvh, r->ro[0].p, r->rd[0].p, n, out.p);
+  k_occluded<<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, r->ro[0].p, r->rd[0].p, n, out.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(occluded, out.p, n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
