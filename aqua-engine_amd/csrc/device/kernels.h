@@ -16,9 +16,13 @@ namespace aqd {
 
 constexpr uint32_t kMiss = 0xFFFFFFFFu;
 constexpr uint32_t kMaxBounces = 64;
-// counters[] layout
-constexpr uint32_t kCntRays = 0;        // [0, 65): rays in the queue of bounce b
-constexpr uint32_t kCntShadow = 80;     // [80, 144): shadow rays emitted at bounce b
+// counters[] layout. Words [0, 2 kMaxBounces + 4) hold the queue lengths:
+// rays of bounce b at 2 b, shadow rays emitted at bounce b at 2 b + 3, so
+// the ray count of bounce b + 1 and the shadow count of bounce b form one
+// 8-byte-aligned pair that the shading kernel bumps with a single 64-bit
+// atomic (ray count in the low word).
+__host__ __device__ constexpr uint32_t cnt_rays(uint32_t b) { return 2 * b; }
+__host__ __device__ constexpr uint32_t cnt_shadow(uint32_t b) { return 2 * b + 3; }
 constexpr uint32_t kCntQuery = 304;     // cache queries emitted this pass
 constexpr uint32_t kCntTrain = 305;     // cache training samples emitted this pass
 constexpr uint32_t kCntWords = 320;

@@ -30,15 +30,58 @@ constexpr int kBlock = 256;
 
 // Wave-aggregated queue append: every lane of the wave must call it; lanes
 // with `want` get consecutive slots starting at *counter.
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
-  unsigned long long mask = __ballot(want);
-  uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32),
-                                             __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
-  uint32_t total = uint32_t(__popcll(mask));
-  uint32_t base = 0;
-  if (total != 0u && __lane_id() == 0) base = atomicAdd(counter, total);
-  base = __shfl(base, 0);
-  return base + below;
+// Block-wide queue appends: each thread that wants a slot gets base +
+// (wanting threads before it in the block); one atomic per block. Every
+// thread of the block must call (the calls synchronise the block).
+__device__ __forceinline__ uint32_t block_prefix(unsigned long long mask, const uint32_t* wave_tot, uint32_t w) {
+  uint32_t p = __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+  for (uint32_t k = 0; k < w; ++k) p += wave_tot[k];
+  return p;
+}
+
+// Two queues at once: `pair` holds the first queue's length in its low word
+// and the second's in its high word.
+__device__ __forceinline__ void block_append2(bool a, bool b, unsigned long long* pair, uint32_t* ja,
+                                              uint32_t* jb) {
+  __shared__ uint32_t ta[kBlock / 64], tb[kBlock / 64];
+  __shared__ unsigned long long base;
+  const unsigned long long ma = __ballot(a), mb = __ballot(b);
+  const uint32_t w = threadIdx.x >> 6;
+  if (__lane_id() == 0) {
+    ta[w] = uint32_t(__popcll(ma));
+    tb[w] = uint32_t(__popcll(mb));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sa = 0, sb = 0;
+    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+      sa += ta[k];
+      sb += tb[k];
+    }
+    base = (sa | sb) ? atomicAdd(pair, sa | (sb << 32)) : 0ull;
+  }
+  __syncthreads();
+  *ja = uint32_t(base) + block_prefix(ma, ta, w);
+  *jb = uint32_t(base >> 32) + block_prefix(mb, tb, w);
+  __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t block_append(bool a, uint32_t* counter) {
+  __shared__ uint32_t ta[kBlock / 64];
+  __shared__ uint32_t base;
+  const unsigned long long ma = __ballot(a);
+  const uint32_t w = threadIdx.x >> 6;
+  if (__lane_id() == 0) ta[w] = uint32_t(__popcll(ma));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sa = 0;
+    for (uint32_t k = 0; k < kBlock / 64; ++k) sa += ta[k];
+    base = sa ? atomicAdd(counter, sa) : 0u;
+  }
+  __syncthreads();
+  uint32_t j = base + block_prefix(ma, ta, w);
+  __syncthreads();
+  return j;
 }
 
 // Queue counters of a pass (bounce 0 holds initial_rays) and the first
@@ -47,7 +90,7 @@ __global__ void k_reset(uint32_t* counters, uint32_t initial_rays, uint32_t* cur
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kCntWords + cursor_words;
        i += gridDim.x * blockDim.x) {
     if (i < kCntWords)
-      counters[i] = (i == kCntRays) ? initial_rays : 0u;
+      counters[i] = (i == cnt_rays(0)) ? initial_rays : 0u;
     else
       cursors[i - kCntWords] = 0u;
   }
@@ -191,7 +234,7 @@ __device__ __forceinline__ uint32_t pick_light(const float* cdf, uint32_t n, flo
 // depth limit, next-event estimation, BSDF sampling and Russian roulette.
 __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, uint32_t b, Queues q) {
   const uint32_t c = b & 1u, nx = c ^ 1u;
-  const uint32_t n = q.counters[kCntRays + b];
+  const uint32_t n = q.counters[cnt_rays(b)];
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
@@ -357,7 +400,8 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
         }
       }
     }
-    uint32_t j = wave_append(alive, &q.counters[kCntRays + b + 1]);
+    uint32_t j, k;
+    block_append2(alive, shadow, reinterpret_cast<unsigned long long*>(q.counters + cnt_rays(b + 1)), &j, &k);
     if (alive) {
       q.ro[nx][j] = n_ro;
       q.rd[nx][j] = n_rd;
@@ -367,7 +411,6 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
     } else if (valid) {
       q.Lbuf[path_id] = make_float4(L.x, L.y, L.z, 0.0f);
     }
-    uint32_t k = wave_append(shadow, &q.counters[kCntShadow + b]);
     if (shadow) {
       uint32_t dst = alive ? j : (0x80000000u | path_id);
       s_d.w = __uint_as_float(dst);
@@ -376,7 +419,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
       q.shc[k] = s_c;
     }
     if (pp.nrc) {
-      uint32_t qi = wave_append(query, &q.counters[kCntQuery]);
+      uint32_t qi = block_append(query, &q.counters[kCntQuery]);
       if (query) {
         q.qrec[4 * size_t(qi) + 0] = qr0;
         q.qrec[4 * size_t(qi) + 1] = qr1;
@@ -426,7 +469,7 @@ __global__ void k_nrc_targets(const float4* trec, const float4* taux, uint32_t n
         tgt = make_float4(o[0], o[1], o[2], 0.0f);
       }
     }
-    uint32_t j = wave_append(ok, counter);
+    uint32_t j = block_append(ok, counter);
     if (ok) {
       for (int k = 0; k < 4; ++k) out_rec[4 * size_t(j) + k] = trec[4 * size_t(s) + k];
       out_tgt[j] = tgt;
@@ -445,7 +488,7 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevBvh bvh, Queues q, uint32_
   TStack st;
   st_init(st, lds_stack, bvh.spill);
   const uint32_t nx = (b & 1u) ^ 1u;
-  const uint32_t n = q.counters[kCntShadow + b];
+  const uint32_t n = q.counters[cnt_shadow(b)];
   TraceCount tc{0u, 0u};
   WaveWork ww = ww_init(n, q.cursors + cursor_set(b, 1));
   while (true) {
@@ -846,7 +889,7 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   bool timing = r->opts.timing != 0;
   for (uint32_t b = 0; b <= pp.max_depth; ++b) {
     uint32_t c = b & 1u;
-    const uint32_t* cnt = q.counters + kCntRays + b;
+    const uint32_t* cnt = q.counters + cnt_rays(b);
     uint32_t* cur = q.cursors + cursor_set(b, 0);
     hipEvent_t* ev = &r->kev[4 * b];
     if (timing) HIPCHK(hipEventRecord(ev[0], s));
@@ -940,9 +983,9 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     st->nrc_train_records += n_train_rec;
     if (pp.nrc && n_slots) st->nrc_loss = nrc_loss;
     st->ms_nrc += ms_nrc;
-    st->camera_rays += hc[kCntRays];
-    for (uint32_t b = 1; b <= pp.max_depth; ++b) st->extension_rays += hc[kCntRays + b];
-    for (uint32_t b = 0; b < pp.max_depth; ++b) st->shadow_rays += hc[kCntShadow + b];
+    st->camera_rays += hc[cnt_rays(0)];
+    for (uint32_t b = 1; b <= pp.max_depth; ++b) st->extension_rays += hc[cnt_rays(b)];
+    for (uint32_t b = 0; b < pp.max_depth; ++b) st->shadow_rays += hc[cnt_shadow(b)];
     st->n_intersect_launches += pp.max_depth + 1;
     if (timing) {
       for (uint32_t b = 0; b <= pp.max_depth; ++b) {
