@@ -108,16 +108,17 @@ int aqua_camera_flythrough(const aqua_scene* s, uint32_t seed, uint32_t frame, u
   if (!s || !out || n_frames == 0 || width == 0 || height == 0 || width > 16384 || height > 16384 ||
       !(fov_deg > 0.0f && fov_deg < 180.0f))
     return aqua::fail(AQUA_ERR_INVALID_ARG, "aqua_camera_flythrough: bad argument");
-  // 8 control points in the inner part of the bounds (PCG32, stream 0x666c79);
-  // closed uniform Catmull-Rom spline through them; the camera looks at the
-  // centre of the bounds with +y up.
+  // 8 control points (PCG32, stream 0x666c79) on a jittered orbit around the
+  // centre of the bounds: angle 2 pi i / 8 +- 0.3 rad, radius 25-40 % of the
+  // half extent in x and z, height 70-80 % up the bounds (above floor-standing
+  // objects, below a ceiling light); closed uniform Catmull-Rom spline through
+  // them; the camera looks at the centre of the bounds with +y up.
   const aqua_scene_info& in = s->info;
-  double lo[3], hi[3], c[3];
+  double c[3], he[3];
   for (int k = 0; k < 3; ++k) {
     double a = in.bounds_lo[k], b = in.bounds_hi[k];
     c[k] = 0.5 * (a + b);
-    lo[k] = a + 0.2 * (b - a);
-    hi[k] = b - 0.2 * (b - a);
+    he[k] = 0.5 * (b - a);
   }
   uint64_t state = 0, inc = (0x666c79ull << 1u) | 1u;
   auto next = [&]() {
@@ -126,12 +127,19 @@ int aqua_camera_flythrough(const aqua_scene* s, uint32_t seed, uint32_t frame, u
     uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u), rot = uint32_t(old >> 59u);
     return (xs >> rot) | (xs << ((32u - rot) & 31u));
   };
+  auto u01 = [&]() { return double(next() >> 8) / 16777216.0; };
   next();
   state += seed;
   next();
   double P[8][3];
-  for (int i = 0; i < 8; ++i)
-    for (int k = 0; k < 3; ++k) P[i][k] = lo[k] + (hi[k] - lo[k]) * (double(next() >> 8) / 16777216.0);
+  for (int i = 0; i < 8; ++i) {
+    double ang = 2.0 * M_PI * i / 8.0 + 0.6 * (u01() - 0.5);
+    double rad = 0.25 + 0.15 * u01();
+    double hy = 0.7 + 0.1 * u01();
+    P[i][0] = c[0] + rad * he[0] * std::cos(ang);
+    P[i][1] = in.bounds_lo[1] + hy * 2.0 * he[1];
+    P[i][2] = c[2] + rad * he[2] * std::sin(ang);
+  }
   double u = double(frame % n_frames) / double(n_frames) * 8.0;
   int seg = int(u);
   double t = u - seg;

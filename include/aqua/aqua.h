@@ -324,8 +324,9 @@ int aqua_renderer_nrc_batch(aqua_renderer* r, uint32_t* n, const void** d_record
 /* ---- misc ---------------------------------------------------------------- */
 
 /* Camera of frame `frame` of an `n_frames` fly-through: a closed Catmull-Rom
- * spline through 8 seeded control points inside the scene bounds, looking at
- * the bounds' centre; fov and resolution as given. */
+ * spline through 8 seeded control points on a jittered orbit around the
+ * centre of the scene bounds (25-40 % of the half extent out, 70-80 % of the
+ * way up), looking at the bounds' centre; fov and resolution as given. */
 int aqua_camera_flythrough(const aqua_scene* scene, uint32_t seed, uint32_t frame, uint32_t n_frames,
                            float fov_deg, uint32_t width, uint32_t height, aqua_camera* out);
 
