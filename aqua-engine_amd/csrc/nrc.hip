@@ -884,11 +884,13 @@ int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* 
   return AQUA_OK;
 }
 
+// `steps` Adam steps on n samples already on the device, stream-ordered on
+// `s`. With stats, the call waits for the steps and reports their timing and
+// the last loss; without, it returns at once (the loss stays on the device).
 int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
-                     aqua_nrc_stats* stats) {
+                     aqua_nrc_stats* stats, hipStream_t s) {
   if (n == 0 || steps == 0) return AQUA_OK;
   if (int e = nrc->ensure_rows(n)) return e;
-  hipStream_t s = nrc->stream;
   FwdArgs a = nrc->args(n, d_rec, d_tgt);
   a.pack_fwd = nrc->pack_fwd.p;
   a.pack_bwd = nrc->pack_bwd.p;
@@ -950,9 +952,9 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
     }
   }
   HIPCHK(hipEventRecord(nrc->ev[5], s));
-  HIPCHK(hipMemcpyAsync(&last_loss, nrc->loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
   if (stats) {
+    HIPCHK(hipMemcpyAsync(&last_loss, nrc->loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     memset(stats, 0, sizeof *stats);
     float ms;
     HIPCHK(hipEventElapsedTime(&ms, nrc->ev[0], nrc->ev[5]));
@@ -1183,7 +1185,7 @@ int aqua_nrc_train(aqua_nrc* nrc, uint32_t n, const float* records, const float*
   HIPCHK(hipSetDevice(nrc->device));
   if (int e = upload_records(nrc, n, records, targets)) return e;
   aqua_nrc_stats st;
-  if (int e = aqua::nrc_train_device(nrc, n, nrc->rec_buf.p, nrc->tgt_buf.p, steps, &st)) return e;
+  if (int e = aqua::nrc_train_device(nrc, n, nrc->rec_buf.p, nrc->tgt_buf.p, steps, &st, nrc->stream)) return e;
   if (stats) *stats = st;
   return AQUA_OK;
 }
@@ -1234,8 +1236,11 @@ int aqua_nrc_train_device(aqua_nrc* nrc, uint32_t n, const void* d_records, cons
                           aqua_nrc_stats* stats) {
   if (!nrc || (n && (!d_records || !d_targets))) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
   HIPCHK(hipSetDevice(nrc->device));
-  return aqua::nrc_train_device(nrc, n, static_cast<const float4*>(d_records), static_cast<const float4*>(d_targets),
-                                steps, stats);
+  if (int e = aqua::nrc_train_device(nrc, n, static_cast<const float4*>(d_records),
+                                     static_cast<const float4*>(d_targets), steps, stats, nrc->stream))
+    return e;
+  HIPCHK(hipStreamSynchronize(nrc->stream));
+  return AQUA_OK;
 }
 
 }  // extern "C"

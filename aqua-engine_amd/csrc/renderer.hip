@@ -721,7 +721,7 @@ struct aqua_renderer {
   DevBuf<double> err_partial;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<hipEvent_t> kev;
-  hipEvent_t nev[2] = {nullptr, nullptr};  // around the cache queries
+  hipEvent_t nev[4] = {};  // around the cache queries, around the cache training
   // neural radiance cache
   aqua_nrc* nrc = nullptr;
   bool nrc_enabled = true;
@@ -822,7 +822,7 @@ namespace aqua {
 int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* d_out, uint32_t precision,
                      hipStream_t s, const uint32_t* d_count, const uint32_t* d_perm);
 int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
-                     aqua_nrc_stats* stats);
+                     aqua_nrc_stats* stats, hipStream_t s);
 }  // namespace aqua
 
 namespace {
@@ -910,6 +910,7 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   }
   HIPCHK(hipGetLastError());
   float ms_nrc = 0.0f;
+  bool trained = false;
   uint32_t n_query = 0, n_train_rec = 0;
   float nrc_loss = 0.0f;
   if (pp.nrc) {
@@ -937,8 +938,9 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   else
     k_film32<<<gf, kBlock, 0, s>>>(pp, q.Lbuf, r->film.p, q.stats);
   if (pp.nrc && n_slots) {
-    // training: targets of this pass's suffix paths, then Adam steps
-    auto t0 = std::chrono::steady_clock::now();
+    // training: targets of this pass's suffix paths, then Adam steps, all on
+    // the renderer's stream; the sample count is read back to size them
+    if (timing) HIPCHK(hipEventRecord(r->nev[2], s));
     k_nrc_targets<<<std::min(grid_for(n_slots), 4096), kBlock, 0, s>>>(
         q.trec, q.taux, n_slots, pp.max_depth + 1u, pp.train_stride, pp.train_offset, q.Lbuf, r->brec.p,
         r->btgt.p, q.counters + kCntTrain);
@@ -948,17 +950,26 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     HIPCHK(hipStreamSynchronize(s));
     n_train_rec = nt;
     r->batch_n = nt;
-    // four Adam steps per pass, each on a quarter of the samples
+    // four Adam steps per pass, each on a quarter of the samples; only the
+    // last waits, for its loss
     const uint32_t steps = 4;
     uint32_t per = (nt + steps - 1) / steps;
+    uint32_t last = steps;
     for (uint32_t k = 0; k < steps && per > 0; ++k) {
+      uint32_t lo = k * per, cnt = std::min(per, nt - std::min(nt, lo));
+      if (cnt >= 32) last = k;
+    }
+    for (uint32_t k = 0; k <= last && last < steps; ++k) {
       uint32_t lo = k * per, cnt = std::min(per, nt - std::min(nt, lo));
       if (cnt < 32) continue;
       aqua_nrc_stats ns;
-      if (int e = aqua::nrc_train_device(r->nrc, cnt, r->brec.p + 4 * size_t(lo), r->btgt.p + lo, 1, &ns)) return e;
-      nrc_loss = ns.loss;
+      if (int e = aqua::nrc_train_device(r->nrc, cnt, r->brec.p + 4 * size_t(lo), r->btgt.p + lo, 1,
+                                         k == last ? &ns : nullptr, s))
+        return e;
+      if (k == last) nrc_loss = ns.loss;
     }
-    ms_nrc += float(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    if (timing) HIPCHK(hipEventRecord(r->nev[3], s));
+    trained = true;
     if (!std::isfinite(nrc_loss)) {
       // a diverged cache is not used again: frames fall back to plain tracing
       r->nrc_enabled = false;
@@ -976,6 +987,10 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
       float ms = 0.0f;
       HIPCHK(hipEventElapsedTime(&ms, r->nev[0], r->nev[1]));
       ms_nrc += ms;
+      if (trained) {
+        HIPCHK(hipEventElapsedTime(&ms, r->nev[2], r->nev[3]));
+        ms_nrc += ms;
+      }
     }
   }
   if (st) {

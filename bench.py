@@ -207,6 +207,67 @@ def nrc_eval_leg(aqua, frames=tuple(range(15, 240, 30)), n_frames=240, oracle_sp
     return res
 
 
+def large_leg(aqua, meshes=400, tris=5000000, width=3840, height=2160, frames=60, oracle_spp=64):
+    """Large interior (about 5 M triangles, 3840x2160): 1 spp per frame with an
+    online cache over a `frames`-frame fly-through, once with FP8 (e4m3)
+    inference + BF16 training and once with FP16 throughout; then the last
+    frame's relMSE of both against an FP64 oracle, and of FP8 against FP16."""
+    import numpy as np
+    t0 = time.perf_counter()
+    with tempfile.TemporaryDirectory() as d:
+        path = aqua.procgen(aqua.PROCGEN_INTERIOR, d, seed=0, n_meshes=meshes, target_tris=tris,
+                            width=width, height=height)
+        scene = aqua.Scene.load(path)
+    load_s = time.perf_counter() - t0
+    info = scene.info()
+    pt = aqua.integrator_default()
+    pt.max_depth = 8
+    t0 = time.perf_counter()
+    ref = aqua.Renderer(scene, pt)
+    build_s = time.perf_counter() - t0
+    res = {"scene": "procedural interior, %d meshes, %d triangles, %dx%d, max depth 8" % (
+               meshes, info.n_triangles, width, height),
+           "load_s": round(load_s, 2), "bvh_build_s": round(build_s, 2), "frames": frames}
+    cam_last = aqua.camera_flythrough(scene, frames - 1, frames, width=width, height=height)
+    imgs = {}
+    for name, prec in (("fp8", aqua.NRC_FP8), ("f16", aqua.NRC_F16)):
+        c = aqua.integrator_default()
+        c.type = aqua.INTEGRATOR_NRC
+        c.max_depth = 8
+        c.learning_rate = 1e-2
+        c.nrc_precision = prec
+        c.nrc_train_paths = 1 << 16
+        r = aqua.Renderer(scene, c)
+        ms, ms_cache, rays, wall = [], [], 0, 0.0
+        for f in range(frames):
+            cam = aqua.camera_flythrough(scene, f, frames, width=width, height=height)
+            t1 = time.perf_counter()
+            st = r.render(1, frame=f, camera=cam)
+            wall += time.perf_counter() - t1
+            if f >= 2:
+                ms.append(st.ms_frame)
+                ms_cache.append(st.ms_nrc)
+            rays += st.rays
+        imgs[name] = r.film(cam_last).astype(np.float64)
+        res[name] = {"ms_per_frame": round(float(np.mean(ms)), 3),
+                     "ms_cache_per_frame": round(float(np.mean(ms_cache)), 3),
+                     "mrays_per_s": round(rays / wall / 1e6, 1),
+                     "queries_last_frame": int(st.nrc_queries),
+                     "loss_last_frame": round(float(st.nrc_loss), 5)}
+        del r
+    st = ref.render(1, frame=frames - 1 + 100000, camera=cam_last)
+    pt1 = ref.film(cam_last).astype(np.float64)
+    res["pt_1spp_ms"] = round(st.ms_frame, 3)
+    ref.render_oracle(0, oracle_spp, frame=frames - 1, camera=cam_last)
+    oracle = ref.oracle_film(cam_last)
+    res["oracle_spp"] = oracle_spp
+    res["relmse_vs_oracle"] = {"pt_1spp": round(_relmse(pt1, oracle), 5),
+                               "nrc_fp8": round(_relmse(imgs["fp8"], oracle), 5),
+                               "nrc_f16": round(_relmse(imgs["f16"], oracle), 5)}
+    res["relmse_fp8_vs_f16"] = round(_relmse(imgs["fp8"], imgs["f16"]), 6)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +283,8 @@ def main():
     ap.add_argument("--nrc-train", action="store_true", help="add the cache training-step leg")
     ap.add_argument("--nrc-eval", action="store_true", help="add the ms/frame-to-relMSE leg")
     ap.add_argument("--only-nrc", action="store_true", help="skip the path-tracer workload")
+    ap.add_argument("--large", action="store_true",
+                    help="add the 5 M-triangle 3840x2160 leg (FP8 vs FP16 cache)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -238,6 +301,8 @@ def main():
             out["nrc_train"] = nrc_train_leg(aqua)
         if args.nrc_eval:
             out["nrc_eval"] = nrc_eval_leg(aqua)
+        if args.large:
+            out["large"] = large_leg(aqua)
         print(json.dumps(out))
         return
 
@@ -329,6 +394,8 @@ def main():
         out["nrc_train"] = nrc_train_leg(aqua)
     if args.nrc_eval:
         out["nrc_eval"] = nrc_eval_leg(aqua)
+    if args.large:
+        out["large"] = large_leg(aqua)
     print(json.dumps(out))
 
 

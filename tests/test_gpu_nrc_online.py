@@ -98,15 +98,26 @@ def test_training_targets_in_the_furnace(gpu):
         r.render(1, frame=f)
     nrc = aqua.Nrc.__new__(aqua.Nrc)
     nrc._h = aqua.C.c_void_p(r.nrc_handle())
+    # states the cache was trained on: points on the walls (the renderer maps
+    # the bounds [-1, 1]^3, padded by 1e-3 of the extent, to [0, 1]^3), the
+    # inward normal, an outgoing direction in the normal's hemisphere
     rec = np.zeros((256, 16), np.float32)
     rng = np.random.default_rng(0)
-    rec[:, 0:3] = rng.random((256, 3))
-    rec[:, 3] = 1.0
+    rec[:, 0:3] = rng.uniform(0.05, 0.95, (256, 3))
+    axis = rng.integers(0, 3, 256)
+    side = rng.integers(0, 2, 256)
+    pad = 1e-3 / (1.0 + 2e-3)
+    rows = np.arange(256)
+    rec[rows, axis] = np.where(side == 0, pad, 1.0 - pad)
+    nrm = np.zeros((256, 3))
+    nrm[rows, axis] = np.where(side == 0, 1.0, -1.0)
     d = rng.standard_normal((256, 3))
-    rec[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
-    rec[:, 8:11] = rec[:, 4:7]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d *= np.sign(np.sum(d * nrm, axis=1, keepdims=True))
+    rec[:, 3] = 1.0
+    rec[:, 4:7] = d
+    rec[:, 8:11] = nrm
     rec[:, 12:15] = rho
-    # points on the walls only: project to the nearest face
     y = nrc.infer(rec)
     nrc._h = None  # owned by the renderer
     assert abs(float(np.median(y)) - rho) < 0.15, float(np.median(y))
