@@ -651,7 +651,9 @@ __global__ void k_nrc_repack(const float* master, E* fwd, E* bwd, uint8_t* fwd8,
     __syncthreads();
   }
   float sw = red[0] > 0.0f ? 448.0f / red[0] : 1.0f;
-  if (threadIdx.x == 0 && fwd8) {
+  // blocks (L, 0 .. gridDim.y - 1) share layer L's elements
+  const int e0 = int(blockIdx.y * blockDim.x + threadIdx.x), es = int(gridDim.y * blockDim.x);
+  if (threadIdx.x == 0 && blockIdx.y == 0 && fwd8) {
     wscale[L] = sw;
     float am = __uint_as_float(act_amax[L]);
     ascale[L] = am > 0.0f ? 448.0f / am : 1.0f;
@@ -659,7 +661,7 @@ __global__ void k_nrc_repack(const float* master, E* fwd, E* bwd, uint8_t* fwd8,
   // forward
   const int n_mt = L < kHidden ? 4 : 1;
   const int fbase = L * 8 * 64 * 8;  // elements (8 per lane per (mt, ks))
-  for (int e = threadIdx.x; e < n_mt * 2 * 64 * 8; e += blockDim.x) {
+  for (int e = e0; e < n_mt * 2 * 64 * 8; e += es) {
     int i = e & 7, lane = (e >> 3) & 63, ks = (e >> 9) & 1, mt = e >> 10;
     int kin = kappa(lane >> 4, ks, i), out = 16 * mt + (lane & 15);
     float w = W[size_t(kin) * nout + out];
@@ -673,13 +675,13 @@ __global__ void k_nrc_repack(const float* master, E* fwd, E* bwd, uint8_t* fwd8,
   if (!bwd) return;
   // backward
   if (L < kHidden) {
-    for (int e = threadIdx.x; e < 4 * 2 * 64 * 8; e += blockDim.x) {
+    for (int e = e0; e < 4 * 2 * 64 * 8; e += es) {
       int i = e & 7, lane = (e >> 3) & 63, ks = (e >> 9) & 1, mt = e >> 10;
       int in = 16 * mt + (lane & 15), kout = kappa(lane >> 4, ks, i);
       bwd[L * 8 * 64 * 8 + e] = cvt_e<E>(W[size_t(in) * 64 + kout]);
     }
   } else {
-    for (int e = threadIdx.x; e < 4 * 64 * 8; e += blockDim.x) {
+    for (int e = e0; e < 4 * 64 * 8; e += es) {
       int i = e & 7, lane = (e >> 3) & 63, mt = e >> 9;
       int in = 16 * mt + (lane & 15), out = 4 * (lane >> 4) + i;
       bwd[kHidden * 8 * 64 * 8 + e] = cvt_e<E>(i < 4 ? W[size_t(in) * 16 + out] : 0.0f);
@@ -806,19 +808,21 @@ struct aqua_nrc {
 
   // Rebuilds the MFMA operand packs from the FP32 master weights: forward +
   // backward + FP8 in the training precision, forward only in the other one.
-  int repack() {
+  // Stream-ordered on s (the stream the weights were updated on).
+  int repack(hipStream_t s) {
+    const dim3 g(5, 8);  // layer x 8 slices of its elements
     if (cfg.train_precision == AQUA_NRC_BF16) {
-      k_nrc_repack<__bf16><<<5, kBlock, 0, stream>>>(mmaster.p, reinterpret_cast<__bf16*>(pack_fwd.p),
-                                                      reinterpret_cast<__bf16*>(pack_bwd.p), pack_fp8.p,
-                                                      wscale.p, act_amax.p, ascale.p);
-      k_nrc_repack<_Float16><<<5, kBlock, 0, stream>>>(mmaster.p, reinterpret_cast<_Float16*>(pack_fwd_other.p),
-                                                        nullptr, nullptr, wscale.p, act_amax.p, ascale.p);
+      k_nrc_repack<__bf16><<<g, kBlock, 0, s>>>(mmaster.p, reinterpret_cast<__bf16*>(pack_fwd.p),
+                                                reinterpret_cast<__bf16*>(pack_bwd.p), pack_fp8.p, wscale.p,
+                                                act_amax.p, ascale.p);
+      k_nrc_repack<_Float16><<<g, kBlock, 0, s>>>(mmaster.p, reinterpret_cast<_Float16*>(pack_fwd_other.p),
+                                                  nullptr, nullptr, wscale.p, act_amax.p, ascale.p);
     } else {
-      k_nrc_repack<_Float16><<<5, kBlock, 0, stream>>>(mmaster.p, reinterpret_cast<_Float16*>(pack_fwd.p),
-                                                        reinterpret_cast<_Float16*>(pack_bwd.p), pack_fp8.p,
-                                                        wscale.p, act_amax.p, ascale.p);
-      k_nrc_repack<__bf16><<<5, kBlock, 0, stream>>>(mmaster.p, reinterpret_cast<__bf16*>(pack_fwd_other.p),
-                                                      nullptr, nullptr, wscale.p, act_amax.p, ascale.p);
+      k_nrc_repack<_Float16><<<g, kBlock, 0, s>>>(mmaster.p, reinterpret_cast<_Float16*>(pack_fwd.p),
+                                                  reinterpret_cast<_Float16*>(pack_bwd.p), pack_fp8.p, wscale.p,
+                                                  act_amax.p, ascale.p);
+      k_nrc_repack<__bf16><<<g, kBlock, 0, s>>>(mmaster.p, reinterpret_cast<__bf16*>(pack_fwd_other.p), nullptr,
+                                                nullptr, wscale.p, act_amax.p, ascale.p);
     }
     if (hipGetLastError() != hipSuccess) return aqua::fail(AQUA_ERR_HIP, "repack launch failed");
     return AQUA_OK;
@@ -937,7 +941,7 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
     k_nrc_adam_mlp<<<kMlp / 64, 256, 0, s>>>(nrc->partial.p, nchunks, nrc->mmaster.p, nrc->mm.p, nrc->mv.p, A);
     k_nrc_adam_table<<<blocks_for(nrc->entries), kBlock, 0, s>>>(nrc->tmaster.p, nrc->tgrad.p, nrc->tm.p,
                                                                  nrc->tv.p, nrc->t16.p, nrc->entries, A);
-    if (int e = nrc->repack()) return e;
+    if (int e = nrc->repack(s)) return e;
     HIPCHK(hipEventRecord(nrc->ev[4], s));
     HIPCHK(hipGetLastError());
     if (stats) {
@@ -1083,7 +1087,7 @@ int aqua_nrc_create(int device, const aqua_nrc_cfg* cfg, aqua_nrc** out) {
   std::vector<float> w;
   init_mlp_host(cfg->seed, &w);
   HIPCHK(hipMemcpyAsync(r->mmaster.p, w.data(), kMlp * sizeof(float), hipMemcpyHostToDevice, s));
-  if (int e = r->repack()) return e;
+  if (int e = r->repack(r->stream)) return e;
   HIPCHK(hipStreamSynchronize(s));
   *out = r.release();
   return AQUA_OK;
@@ -1139,7 +1143,7 @@ int aqua_nrc_set_params(aqua_nrc* nrc, const float* table, const float* mlp) {
   k_fill_f32<<<blocks_for(kMlp), kBlock, 0, s>>>(nrc->mm.p, kMlp, 0.0f);
   k_fill_f32<<<blocks_for(kMlp), kBlock, 0, s>>>(nrc->mv.p, kMlp, 0.0f);
   nrc->step = 0;
-  if (int e = nrc->repack()) return e;
+  if (int e = nrc->repack(nrc->stream)) return e;
   HIPCHK(hipStreamSynchronize(s));
   return AQUA_OK;
 }

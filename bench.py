@@ -72,6 +72,18 @@ def cpu_baseline(scene, aqua, max_depth, target_s=12.0):
     }
 
 
+def _traffic():
+    """HBM bytes per k_intersect launch of this workload, from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary (counters scaled by a plain copy
+    of known size), or None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_intersect_interior1m_1080p_64spp_traffic.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def nrc_train_leg(aqua, rows=1 << 18, steps=20, precision=0):
     """Cache training step alone on samples the path tracer produced: box room,
     1920x1080, training suffix paths only, then `steps` Adam steps on 2^18 of
@@ -285,11 +297,24 @@ def main():
     ap.add_argument("--only-nrc", action="store_true", help="skip the path-tracer workload")
     ap.add_argument("--large", action="store_true",
                     help="add the 5 M-triangle 3840x2160 leg (FP8 vs FP16 cache)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 256 MiB device copy (the reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the path tracer shards by pixel tiles across GPUs; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        # A copy of known size: 256 MiB read + 256 MiB written. Profiled with
+        # the workload, its FETCH_SIZE / WRITE_SIZE turn the counters of the
+        # traversal kernel into bytes.
+        import torch
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
 
     aqua = _load("aqua_amd", os.path.join(ROOT, "aqua-engine_amd", "aqua.py"))
     if aqua.device_count() < 1:
@@ -379,7 +404,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": _traffic(),
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/k_intersect_interior1m_1080p_64spp_traffic.json",
+            "algorithmic_bytes_per_launch": round(bytes_per_launch),
             "bytes_per_ray": round(bytes_per_ray, 1),
             "nodes_per_ray": round(cst.nodes_visited / closest_rays, 2),
             "tris_per_ray": round(cst.tris_tested / closest_rays, 2),
