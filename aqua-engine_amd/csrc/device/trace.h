@@ -176,116 +176,161 @@ AQ_DEV void cswap(uint32_t& a, uint32_t& b) {
   b = hi;
 }
 
-// Closest hit. Returns the triangle slot (0xFFFFFFFF on a miss) and t.
+// ---- resumable traversal: the state of one ray between node visits, so a
+// persistent wave can give a lane a new ray as soon as its last one ends
+// instead of idling until the wave's slowest ray is done.
+
+struct ClosestRay {
+  RayPre r;
+  float tmax, tbest;
+  uint32_t best_id, best_slot, node;
+};
+
+AQ_DEV void closest_begin(ClosestRay& c, v3 o, v3 d, float tmax) {
+  c.r = ray_pre(o, d);
+  c.tmax = tmax;
+  c.tbest = tmax;
+  c.best_id = 0xFFFFFFFFu;
+  c.best_slot = 0xFFFFFFFFu;
+  c.node = 0;
+}
+
+// One node visit of a closest-hit ray; true once the ray is finished (its
+// result is (best_slot, tbest); best_slot 0xFFFFFFFF = miss). The stack is
+// empty again when it returns true.
 template <bool COUNT>
-AQ_DEV uint32_t trace_closest(const DevBvh& bvh, TStack& st, v3 o, v3 d, float tmax, float* t_out,
-                              TraceCount* cnt) {
-  RayPre r = ray_pre(o, d);
-  float tbest = tmax;
-  uint32_t best_id = 0xFFFFFFFFu, best_slot = 0xFFFFFFFFu;
-  uint32_t node = 0;
-  while (true) {
-    float tn[4];
-    int4 child, count;
-    node_test4(r, bvh.nodes + 8 * node, tbest, bvh.delta, tn, &child, &count);
-    if (COUNT) cnt->nodes++;
-    // Order hit children near to far: key = entry distance bits with the
-    // child index in the low two bits (entry distances are >= 0).
-    uint32_t key[4];
+AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCount* cnt) {
+  float tn[4];
+  int4 child, count;
+  node_test4(c.r, bvh.nodes + 8 * c.node, c.tbest, bvh.delta, tn, &child, &count);
+  if (COUNT) cnt->nodes++;
+  // Order hit children near to far: key = entry distance bits with the
+  // child index in the low two bits (entry distances are >= 0).
+  uint32_t key[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      key[k] = tn[k] != __builtin_inff() ? ((__float_as_uint(tn[k]) & ~3u) | uint32_t(k)) : 0xFFFFFFFFu;
-    cswap(key[0], key[1]);
-    cswap(key[2], key[3]);
-    cswap(key[0], key[2]);
-    cswap(key[1], key[3]);
-    cswap(key[1], key[2]);
-    // Leaves are tested in near-to-far order; the nearest inner child is
-    // visited next and the other inner children are pushed far-first.
-    // (Indices stay compile-time so nothing lands in scratch memory.)
-    int cj[4];
-    bool inner[4];
+  for (int k = 0; k < 4; ++k)
+    key[k] = tn[k] != __builtin_inff() ? ((__float_as_uint(tn[k]) & ~3u) | uint32_t(k)) : 0xFFFFFFFFu;
+  cswap(key[0], key[1]);
+  cswap(key[2], key[3]);
+  cswap(key[0], key[2]);
+  cswap(key[1], key[3]);
+  cswap(key[1], key[2]);
+  // Leaves are tested in near-to-far order; the nearest inner child is
+  // visited next and the other inner children are pushed far-first.
+  // (Indices stay compile-time so nothing lands in scratch memory.)
+  int cj[4];
+  bool inner[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      bool valid = key[j] != 0xFFFFFFFFu;
-      int k = int(key[j] & 3u);
-      int c = sel4(child, k), n = sel4(count, k);
-      cj[j] = c;
-      inner[j] = valid && n == 0;
-      if (valid && n > 0) {
-        for (int s = c; s < c + n; ++s) {
-          const float4* tp = bvh.tris + 3 * s;
-          float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-          if (COUNT) cnt->tris++;
-          float t;
-          if (tri_hit(r, xyz(p0), xyz(p1), xyz(p2), &t)) {
-            uint32_t id = __float_as_uint(p0.w);
-            if (t > 0.0f && t < tmax && (t < tbest || (t == tbest && id < best_id))) {
-              tbest = t;
-              best_id = id;
-              best_slot = uint32_t(s);
-            }
+  for (int j = 0; j < 4; ++j) {
+    bool valid = key[j] != 0xFFFFFFFFu;
+    int k = int(key[j] & 3u);
+    int ch = sel4(child, k), n = sel4(count, k);
+    cj[j] = ch;
+    inner[j] = valid && n == 0;
+    if (valid && n > 0) {
+      for (int s = ch; s < ch + n; ++s) {
+        const float4* tp = bvh.tris + 3 * s;
+        float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+        if (COUNT) cnt->tris++;
+        float t;
+        if (tri_hit(c.r, xyz(p0), xyz(p1), xyz(p2), &t)) {
+          uint32_t id = __float_as_uint(p0.w);
+          if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
+            c.tbest = t;
+            c.best_id = id;
+            c.best_slot = uint32_t(s);
           }
         }
       }
     }
-    int first = inner[0] ? 0 : (inner[1] ? 1 : (inner[2] ? 2 : (inner[3] ? 3 : -1)));
+  }
+  int first = inner[0] ? 0 : (inner[1] ? 1 : (inner[2] ? 2 : (inner[3] ? 3 : -1)));
 #pragma unroll
-    for (int j = 3; j >= 1; --j)
-      if (inner[j] && j != first) st_push(st, uint32_t(cj[j]));
-    if (first >= 0) {
-      node = uint32_t(first == 0 ? cj[0] : (first == 1 ? cj[1] : (first == 2 ? cj[2] : cj[3])));
+  for (int j = 3; j >= 1; --j)
+    if (inner[j] && j != first) st_push(st, uint32_t(cj[j]));
+  if (first >= 0) {
+    c.node = uint32_t(first == 0 ? cj[0] : (first == 1 ? cj[1] : (first == 2 ? cj[2] : cj[3])));
+    return false;
+  }
+  if (st_empty(st)) return true;
+  c.node = st_pop(st);
+  return false;
+}
+
+// Closest hit. Returns the triangle slot (0xFFFFFFFF on a miss) and t.
+template <bool COUNT>
+AQ_DEV uint32_t trace_closest(const DevBvh& bvh, TStack& st, v3 o, v3 d, float tmax, float* t_out,
+                              TraceCount* cnt) {
+  ClosestRay c;
+  closest_begin(c, o, d, tmax);
+  while (!closest_step<COUNT>(bvh, st, c, cnt)) {
+  }
+  *t_out = c.tbest;
+  return c.best_slot;
+}
+
+struct AnyRay {
+  RayPre r;
+  float tmax;
+  uint32_t node;
+};
+
+AQ_DEV void any_begin(AnyRay& a, v3 o, v3 d, float tmax) {
+  a.r = ray_pre(o, d);
+  a.tmax = tmax;
+  a.node = 0;
+}
+
+// One node visit of an any-hit ray in (0, tmax): 0 = not finished,
+// 1 = occluded, 2 = unoccluded. The stack is empty once it returns non-zero.
+template <bool COUNT>
+AQ_DEV int any_step(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
+  float tn[4];
+  int4 child, count;
+  node_test4(a.r, bvh.nodes + 8 * a.node, a.tmax, bvh.delta, tn, &child, &count);
+  if (COUNT) cnt->nodes++;
+  int next = -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (tn[k] == __builtin_inff()) continue;
+    int ch = sel4(child, k), n = sel4(count, k);
+    if (n > 0) {
+      for (int s = ch; s < ch + n; ++s) {
+        const float4* tp = bvh.tris + 3 * s;
+        float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+        if (COUNT) cnt->tris++;
+        float t;
+        if (tri_hit(a.r, xyz(p0), xyz(p1), xyz(p2), &t) && t > 0.0f && t < a.tmax) {
+          // drain the stack so the lane's next ray starts empty
+          st.cnt = 0;
+          st.gsp = 0;
+          return 1;
+        }
+      }
+    } else if (next < 0) {
+      next = ch;
     } else {
-      if (st_empty(st)) break;
-      node = st_pop(st);
+      st_push(st, uint32_t(ch));
     }
   }
-  *t_out = tbest;
-  return best_slot;
+  if (next >= 0) {
+    a.node = uint32_t(next);
+    return 0;
+  }
+  if (st_empty(st)) return 2;
+  a.node = st_pop(st);
+  return 0;
 }
 
 // Any hit in (0, tmax).
 template <bool COUNT>
 AQ_DEV bool trace_any(const DevBvh& bvh, TStack& st, v3 o, v3 d, float tmax, TraceCount* cnt) {
-  RayPre r = ray_pre(o, d);
-  uint32_t node = 0;
-  while (true) {
-    float tn[4];
-    int4 child, count;
-    node_test4(r, bvh.nodes + 8 * node, tmax, bvh.delta, tn, &child, &count);
-    if (COUNT) cnt->nodes++;
-    int next = -1;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (tn[k] == __builtin_inff()) continue;
-      int c = sel4(child, k), n = sel4(count, k);
-      if (n > 0) {
-        for (int s = c; s < c + n; ++s) {
-          const float4* tp = bvh.tris + 3 * s;
-          float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-          if (COUNT) cnt->tris++;
-          float t;
-          if (tri_hit(r, xyz(p0), xyz(p1), xyz(p2), &t) && t > 0.0f && t < tmax) {
-            // drain the stack so the caller's next ray starts empty
-            st.cnt = 0;
-            st.gsp = 0;
-            return true;
-          }
-        }
-      } else if (next < 0) {
-        next = c;
-      } else {
-        st_push(st, uint32_t(c));
-      }
-    }
-    if (next >= 0) {
-      node = uint32_t(next);
-    } else {
-      if (st_empty(st)) break;
-      node = st_pop(st);
-    }
+  AnyRay a;
+  any_begin(a, o, d, tmax);
+  int r;
+  while ((r = any_step<COUNT>(bvh, st, a, cnt)) == 0) {
   }
-  return false;
+  return r == 1;
 }
 
 }  // namespace aqd

@@ -190,6 +190,53 @@ __device__ __forceinline__ uint32_t ww_next(WaveWork& w) {
   return kMiss;
 }
 
+// Per-lane ray refill on top of WaveWork: the lanes that want a ray take the
+// next rays of the wave's current 64-ray chunk in lane order, and the wave
+// fetches further chunks as that one runs out. Every lane of the wave calls
+// it together (the bookkeeping is wave-uniform).
+struct LaneFeed {
+  WaveWork ww;
+  uint32_t pos, end;  // the wave's current chunk: rays [pos, end) not yet handed out
+  bool more;          // chunks may remain
+};
+
+__device__ __forceinline__ LaneFeed feed_init(uint32_t n, uint32_t* cursors) {
+  LaneFeed f;
+  f.ww = ww_init(n, cursors);
+  f.pos = 0;
+  f.end = 0;
+  f.more = true;
+  return f;
+}
+
+// This lane's new ray index, or kMiss (it did not ask, or the queue is done).
+__device__ __forceinline__ uint32_t feed_take(LaneFeed& f, bool wants) {
+  const unsigned long long m = __ballot(wants);
+  const uint32_t k = uint32_t(__popcll(m));
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+  uint32_t got = kMiss, given = 0;
+  while (given < k && f.more) {
+    if (f.pos >= f.end) {
+      uint32_t base = ww_next(f.ww);
+      if (base == kMiss) {
+        f.more = false;
+        break;
+      }
+      f.pos = base;
+      f.end = min(base + 64u, f.ww.n);
+    }
+    uint32_t take = min(k - given, f.end - f.pos);
+    if (wants && rank >= given && rank < given + take) got = f.pos + (rank - given);
+    f.pos += take;
+    given += take;
+  }
+  return got;
+}
+
+// A wave keeps traversing while more than this many of its lanes hold a ray,
+// then refills the idle ones (until the queue is drained).
+constexpr uint32_t kRefillBelow = 48;
+
 template <bool COUNT>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevBvh bvh, const float4* __restrict__ ro,
                                                       const float4* __restrict__ rd,
@@ -200,16 +247,29 @@ __global__ void __launch_bounds__(kBlock) k_intersect(DevBvh bvh, const float4* 
   st_init(st, lds_stack, bvh.spill);
   const uint32_t n = *count;
   TraceCount tc{0u, 0u};
-  WaveWork ww = ww_init(n, cursor);
+  LaneFeed feed = feed_init(n, cursor);
+  ClosestRay cr;
+  bool active = false;
+  uint32_t ray = 0;
   while (true) {
-    uint32_t base = ww_next(ww);
-    if (base == kMiss) break;
-    uint32_t i = base + __lane_id();
-    if (i < n) {
-      float4 o = ro[i], d = rd[i];
-      float t;
-      uint32_t slot = trace_closest<COUNT>(bvh, st, xyz(o), xyz(d), o.w, &t, &tc);
-      hit[i] = make_uint2(slot, __float_as_uint(t));
+    if (feed.more) {
+      uint32_t r = feed_take(feed, !active);
+      if (r != kMiss) {
+        float4 o = ro[r], d = rd[r];
+        closest_begin(cr, xyz(o), xyz(d), o.w);
+        ray = r;
+        active = true;
+      }
+    }
+    const uint32_t keep = feed.more ? kRefillBelow : 0u;
+    uint32_t live = uint32_t(__popcll(__ballot(active)));
+    if (live == 0) break;
+    while (live > keep) {
+      if (active && closest_step<COUNT>(bvh, st, cr, &tc)) {
+        hit[ray] = make_uint2(cr.best_slot, __float_as_uint(cr.tbest));
+        active = false;
+      }
+      live = uint32_t(__popcll(__ballot(active)));
     }
   }
   if (COUNT) {
@@ -490,19 +550,38 @@ __global__ void __launch_bounds__(kBlock) k_shadow(DevBvh bvh, Queues q, uint32_
   const uint32_t nx = (b & 1u) ^ 1u;
   const uint32_t n = q.counters[cnt_shadow(b)];
   TraceCount tc{0u, 0u};
-  WaveWork ww = ww_init(n, q.cursors + cursor_set(b, 1));
+  LaneFeed feed = feed_init(n, q.cursors + cursor_set(b, 1));
+  AnyRay ar;
+  bool active = false;
+  uint32_t ray = 0;
   while (true) {
-    uint32_t base = ww_next(ww);
-    if (base == kMiss) break;
-    uint32_t i = base + __lane_id();
-    if (i >= n) continue;
-    float4 o = q.sho[i], d = q.shd[i];
-    if (!trace_any<COUNT>(bvh, st, xyz(o), xyz(d), o.w, &tc)) {
-      float4 cc = q.shc[i];
-      uint32_t dst = __float_as_uint(d.w);
-      float4* Lp = (dst & 0x80000000u) ? &q.Lbuf[dst & 0x7FFFFFFFu] : &q.L[nx][dst];
-      float4 l = *Lp;
-      *Lp = make_float4(l.x + cc.x, l.y + cc.y, l.z + cc.z, 0.0f);
+    if (feed.more) {
+      uint32_t r = feed_take(feed, !active);
+      if (r != kMiss) {
+        float4 o = q.sho[r], d = q.shd[r];
+        any_begin(ar, xyz(o), xyz(d), o.w);
+        ray = r;
+        active = true;
+      }
+    }
+    const uint32_t keep = feed.more ? kRefillBelow : 0u;
+    uint32_t live = uint32_t(__popcll(__ballot(active)));
+    if (live == 0) break;
+    while (live > keep) {
+      if (active) {
+        int res = any_step<COUNT>(bvh, st, ar, &tc);
+        if (res != 0) {
+          active = false;
+          if (res == 2) {
+            float4 cc = q.shc[ray];
+            uint32_t dst = __float_as_uint(q.shd[ray].w);
+            float4* Lp = (dst & 0x80000000u) ? &q.Lbuf[dst & 0x7FFFFFFFu] : &q.L[nx][dst];
+            float4 l = *Lp;
+            *Lp = make_float4(l.x + cc.x, l.y + cc.y, l.z + cc.z, 0.0f);
+          }
+        }
+      }
+      live = uint32_t(__popcll(__ballot(active)));
     }
   }
   if (COUNT) {
@@ -1096,9 +1175,16 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   if (int e = select_device(device, &cus)) return e;
   auto r = std::make_unique<aqua_renderer>();
   r->device = device;
-  // 8 blocks of 256 per CU keep every SIMD busy at the traversal kernels'
-  // register and LDS footprint.
-  r->persistent_blocks = std::max(1, cus) * 8;
+  // Persistent traversal grids: as many blocks as are resident at once for
+  // both traversal kernels (their register / LDS footprint), so no block
+  // waits for another to exit; a multiple of 8 (one work range per XCD).
+  {
+    int a = 0, b = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_intersect<false>, kBlock, 0));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kBlock, 0));
+    int per_cu = std::max(1, std::min(a, b));
+    r->persistent_blocks = std::max(8, (std::max(1, cus) * per_cu) & ~7);
+  }
   r->cfg = *cfg;
   r->camera = scene->camera;
   r->opts.timing = 1;
