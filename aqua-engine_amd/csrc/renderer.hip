@@ -235,7 +235,7 @@ __device__ __forceinline__ uint32_t feed_take(LaneFeed& f, bool wants) {
 
 // A wave keeps traversing while more than this many of its lanes hold a ray,
 // then refills the idle ones (until the queue is drained).
-constexpr uint32_t kRefillBelow = 48;
+constexpr uint32_t kRefillBelow = 40;
 
 template <bool COUNT>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevBvh bvh, const float4* __restrict__ ro,
