@@ -87,8 +87,13 @@ AQ_DEV float box_enter(const RayPre& r, float lx, float hx, float ly, float hy, 
   float t0x = __builtin_fmaf(lx, r.inv.x, -r.oi.x), t1x = __builtin_fmaf(hx, r.inv.x, -r.oi.x);
   float t0y = __builtin_fmaf(ly, r.inv.y, -r.oi.y), t1y = __builtin_fmaf(hy, r.inv.y, -r.oi.y);
   float t0z = __builtin_fmaf(lz, r.inv.z, -r.oi.z), t1z = __builtin_fmaf(hz, r.inv.z, -r.oi.z);
-  float tn = fmax2(fmax2(fmin2(t0x, t1x), fmin2(t0y, t1y)), fmax2(fmin2(t0z, t1z), 0.0f));
-  float tf = fmin2(fmin2(fmax2(t0x, t1x), fmax2(t0y, t1y)), fmin2(fmax2(t0z, t1z), tbest));
+  // The t values are finite (|1/d| is clamped), so the hardware min / max
+  // (v_min3 / v_max3) apply; culling only has to be conservative, the hit
+  // itself does not depend on it.
+  float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(t0x, t1x), __builtin_fminf(t0y, t1y)),
+                             __builtin_fmaxf(__builtin_fminf(t0z, t1z), 0.0f));
+  float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(t0x, t1x), __builtin_fmaxf(t0y, t1y)),
+                             __builtin_fminf(__builtin_fmaxf(t0z, t1z), tbest));
   return tn <= tf * kFarScale + delta ? tn : __builtin_inff();
 }
 
