@@ -10,7 +10,11 @@ resident in HBM.
 Prints one JSON line: Mrays/s (camera + extension + shadow rays per second of
 wall time), ms/frame, the roofline of the closest-hit traversal kernel (HIP
 events on the renderer's stream, algorithmic bytes from an instrumented frame),
-and the CPU restatement's rate on a bounded sample of the same frame.
+and the CPU restatement's rate on a bounded sample of the same frame. Unless
+--no-nrc, two cache legs follow: one training step on 2^18 samples
+("nrc_train") and the ms/frame-to-relMSE-1e-2 comparison of the cache against
+the plain tracer on the box-room fly-through ("nrc_eval", whose "nrc_over_pt"
+is the north-star ratio).
 """
 import argparse
 import importlib.util
@@ -292,14 +296,18 @@ def main():
     ap.add_argument("--tris", type=int, default=1000000)
     ap.add_argument("--max-depth", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--nrc-train", action="store_true", help="add the cache training-step leg")
-    ap.add_argument("--nrc-eval", action="store_true", help="add the ms/frame-to-relMSE leg")
+    ap.add_argument("--no-nrc", action="store_true",
+                    help="skip the cache legs (training step, ms/frame-to-relMSE evaluation)")
+    ap.add_argument("--nrc-train", action="store_true", help="(default on) the cache training-step leg")
+    ap.add_argument("--nrc-eval", action="store_true", help="(default on) the ms/frame-to-relMSE leg")
     ap.add_argument("--only-nrc", action="store_true", help="skip the path-tracer workload")
     ap.add_argument("--large", action="store_true",
                     help="add the 5 M-triangle 3840x2160 leg (FP8 vs FP16 cache)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (the reference kernel for HBM counters)")
     args = ap.parse_args()
+    if not args.no_nrc and not args.only_nrc:
+        args.nrc_train = args.nrc_eval = True
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the path tracer shards by pixel tiles across GPUs; see DESIGN.md)", file=sys.stderr)
