@@ -94,7 +94,7 @@ AQ_DEV float box_enter(const RayPre& r, float lx, float hx, float ly, float hy, 
                              __builtin_fmaxf(__builtin_fminf(t0z, t1z), 0.0f));
   float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(t0x, t1x), __builtin_fmaxf(t0y, t1y)),
                              __builtin_fminf(__builtin_fmaxf(t0z, t1z), tbest));
-  return tn <= tf * kFarScale + delta ? tn : __builtin_inff();
+  return tn <= __builtin_fmaf(tf, kFarScale, delta) ? tn : __builtin_inff();
 }
 
 struct DevBvh {
@@ -161,18 +161,34 @@ AQ_DEV uint32_t st_pop(TStack& s) {
   return s.spill[size_t(--s.gsp) * s.sstride];
 }
 
-AQ_DEV int sel4(int4 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+// Branch-free choice of v[k], k in 0..3 (three selects).
+AQ_DEV uint32_t pick4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t k) {
+  uint32_t lo = (k & 1u) ? b : a, hi = (k & 1u) ? d : c;
+  return (k & 2u) ? hi : lo;
+}
 
-// Four slab tests of one node. Empty children get +inf.
-AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float delta, float tn[4], int4* child,
-                       int4* count) {
+// Four slab tests of one node, straight-line (all four boxes are tested, then
+// empty children are masked), so the node is eight 16-byte loads with no
+// branch between them. Empty children get +inf. Child references come back
+// packed: (index << 3) | triangle count (count 0 = inner node; counts are at
+// most 7).
+AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float delta, float tn[4],
+                       uint32_t ref[4]) {
   float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-  *child = *reinterpret_cast<const int4*>(np + 6);
-  *count = *reinterpret_cast<const int4*>(np + 7);
-  tn[0] = child->x >= 0 ? box_enter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tbest, delta) : __builtin_inff();
-  tn[1] = child->y >= 0 ? box_enter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tbest, delta) : __builtin_inff();
-  tn[2] = child->z >= 0 ? box_enter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tbest, delta) : __builtin_inff();
-  tn[3] = child->w >= 0 ? box_enter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tbest, delta) : __builtin_inff();
+  int4 child = *reinterpret_cast<const int4*>(np + 6);
+  int4 count = *reinterpret_cast<const int4*>(np + 7);
+  float b0 = box_enter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tbest, delta);
+  float b1 = box_enter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tbest, delta);
+  float b2 = box_enter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tbest, delta);
+  float b3 = box_enter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tbest, delta);
+  tn[0] = child.x >= 0 ? b0 : __builtin_inff();
+  tn[1] = child.y >= 0 ? b1 : __builtin_inff();
+  tn[2] = child.z >= 0 ? b2 : __builtin_inff();
+  tn[3] = child.w >= 0 ? b3 : __builtin_inff();
+  ref[0] = (uint32_t(child.x) << 3) | uint32_t(count.x);
+  ref[1] = (uint32_t(child.y) << 3) | uint32_t(count.y);
+  ref[2] = (uint32_t(child.z) << 3) | uint32_t(count.z);
+  ref[3] = (uint32_t(child.w) << 3) | uint32_t(count.w);
 }
 
 AQ_DEV void cswap(uint32_t& a, uint32_t& b) {
@@ -206,8 +222,8 @@ AQ_DEV void closest_begin(ClosestRay& c, v3 o, v3 d, float tmax) {
 template <bool COUNT>
 AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCount* cnt) {
   float tn[4];
-  int4 child, count;
-  node_test4(c.r, bvh.nodes + 8 * c.node, c.tbest, bvh.delta, tn, &child, &count);
+  uint32_t ref[4];
+  node_test4(c.r, bvh.nodes + 8 * c.node, c.tbest, bvh.delta, tn, ref);
   if (COUNT) cnt->nodes++;
   // Order hit children near to far: key = entry distance bits with the
   // child index in the low two bits (entry distances are >= 0).
@@ -223,17 +239,17 @@ AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCoun
   // Leaves are tested in near-to-far order; the nearest inner child is
   // visited next and the other inner children are pushed far-first.
   // (Indices stay compile-time so nothing lands in scratch memory.)
-  int cj[4];
+  uint32_t cj[4];
   bool inner[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    bool valid = key[j] != 0xFFFFFFFFu;
-    int k = int(key[j] & 3u);
-    int ch = sel4(child, k), n = sel4(count, k);
-    cj[j] = ch;
+    const bool valid = key[j] != 0xFFFFFFFFu;
+    const uint32_t p = pick4(ref[0], ref[1], ref[2], ref[3], key[j] & 3u);
+    const uint32_t n = p & 7u;
+    cj[j] = p >> 3;
     inner[j] = valid && n == 0;
     if (valid && n > 0) {
-      for (int s = ch; s < ch + n; ++s) {
+      for (uint32_t s = p >> 3, e = (p >> 3) + n; s < e; ++s) {
         const float4* tp = bvh.tris + 3 * s;
         float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
         if (COUNT) cnt->tris++;
@@ -243,7 +259,7 @@ AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCoun
           if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
             c.tbest = t;
             c.best_id = id;
-            c.best_slot = uint32_t(s);
+            c.best_slot = s;
           }
         }
       }
@@ -252,9 +268,9 @@ AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCoun
   int first = inner[0] ? 0 : (inner[1] ? 1 : (inner[2] ? 2 : (inner[3] ? 3 : -1)));
 #pragma unroll
   for (int j = 3; j >= 1; --j)
-    if (inner[j] && j != first) st_push(st, uint32_t(cj[j]));
+    if (inner[j] && j != first) st_push(st, cj[j]);
   if (first >= 0) {
-    c.node = uint32_t(first == 0 ? cj[0] : (first == 1 ? cj[1] : (first == 2 ? cj[2] : cj[3])));
+    c.node = first == 0 ? cj[0] : (first == 1 ? cj[1] : (first == 2 ? cj[2] : cj[3]));
     return false;
   }
   if (st_empty(st)) return true;
@@ -291,16 +307,16 @@ AQ_DEV void any_begin(AnyRay& a, v3 o, v3 d, float tmax) {
 template <bool COUNT>
 AQ_DEV int any_step(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
   float tn[4];
-  int4 child, count;
-  node_test4(a.r, bvh.nodes + 8 * a.node, a.tmax, bvh.delta, tn, &child, &count);
+  uint32_t ref[4];
+  node_test4(a.r, bvh.nodes + 8 * a.node, a.tmax, bvh.delta, tn, ref);
   if (COUNT) cnt->nodes++;
   int next = -1;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (tn[k] == __builtin_inff()) continue;
-    int ch = sel4(child, k), n = sel4(count, k);
+    const uint32_t n = ref[k] & 7u, ch = ref[k] >> 3;
     if (n > 0) {
-      for (int s = ch; s < ch + n; ++s) {
+      for (uint32_t s = ch; s < ch + n; ++s) {
         const float4* tp = bvh.tris + 3 * s;
         float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
         if (COUNT) cnt->tris++;
@@ -313,9 +329,9 @@ AQ_DEV int any_step(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
         }
       }
     } else if (next < 0) {
-      next = ch;
+      next = int(ch);
     } else {
-      st_push(st, uint32_t(ch));
+      st_push(st, ch);
     }
   }
   if (next >= 0) {
