@@ -249,17 +249,33 @@ AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCoun
     cj[j] = p >> 3;
     inner[j] = valid && n == 0;
     if (valid && n > 0) {
-      for (uint32_t s = p >> 3, e = (p >> 3) + n; s < e; ++s) {
+      // two triangles per round, both fetched before either is tested
+      for (uint32_t s = p >> 3, e = (p >> 3) + n; s < e; s += 2) {
+        const bool two = s + 1 < e;
         const float4* tp = bvh.tris + 3 * s;
-        float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-        if (COUNT) cnt->tris++;
+        float4 a0 = tp[0], a1 = tp[1], a2 = tp[2];
+        float4 b0 = a0, b1 = a1, b2 = a2;
+        if (two) {
+          b0 = tp[3];
+          b1 = tp[4];
+          b2 = tp[5];
+        }
+        if (COUNT) cnt->tris += two ? 2u : 1u;
         float t;
-        if (tri_hit(c.r, xyz(p0), xyz(p1), xyz(p2), &t)) {
-          uint32_t id = __float_as_uint(p0.w);
+        if (tri_hit(c.r, xyz(a0), xyz(a1), xyz(a2), &t)) {
+          uint32_t id = __float_as_uint(a0.w);
           if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
             c.tbest = t;
             c.best_id = id;
             c.best_slot = s;
+          }
+        }
+        if (two && tri_hit(c.r, xyz(b0), xyz(b1), xyz(b2), &t)) {
+          uint32_t id = __float_as_uint(b0.w);
+          if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
+            c.tbest = t;
+            c.best_id = id;
+            c.best_slot = s + 1;
           }
         }
       }
