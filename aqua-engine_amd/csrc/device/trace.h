@@ -332,12 +332,22 @@ AQ_DEV int any_step(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
     if (tn[k] == __builtin_inff()) continue;
     const uint32_t n = ref[k] & 7u, ch = ref[k] >> 3;
     if (n > 0) {
-      for (uint32_t s = ch; s < ch + n; ++s) {
+      // two triangles per round, both fetched before either is tested
+      for (uint32_t s = ch; s < ch + n; s += 2) {
+        const bool two = s + 1 < ch + n;
         const float4* tp = bvh.tris + 3 * s;
-        float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-        if (COUNT) cnt->tris++;
+        float4 a0 = tp[0], a1 = tp[1], a2 = tp[2];
+        float4 b0 = a0, b1 = a1, b2 = a2;
+        if (two) {
+          b0 = tp[3];
+          b1 = tp[4];
+          b2 = tp[5];
+        }
+        if (COUNT) cnt->tris += two ? 2u : 1u;
         float t;
-        if (tri_hit(a.r, xyz(p0), xyz(p1), xyz(p2), &t) && t > 0.0f && t < a.tmax) {
+        bool hit = tri_hit(a.r, xyz(a0), xyz(a1), xyz(a2), &t) && t > 0.0f && t < a.tmax;
+        if (!hit && two) hit = tri_hit(a.r, xyz(b0), xyz(b1), xyz(b2), &t) && t > 0.0f && t < a.tmax;
+        if (hit) {
           // drain the stack so the lane's next ray starts empty
           st.cnt = 0;
           st.gsp = 0;
