@@ -141,3 +141,26 @@ def test_box_room_cache_reduces_error(gpu, box_room, prec):
     oracle = ref.oracle_film(cam)
     rel = float(np.mean((img - oracle) ** 2 / (oracle ** 2 + 1e-2)))
     assert rel < pt_err, (rel, pt_err)
+
+
+def test_diverged_cache_falls_back_to_plain_tracing(gpu, box_room):
+    # A learning rate of 1e30 overflows the FP16 weights on the first step, so
+    # the next step's loss is not finite: the renderer switches its cache off
+    # and later frames are the plain tracer's, bit for bit.
+    cam = small_camera(box_room.camera(), 64, 64)
+    r = aqua.Renderer(box_room, nrc_cfg(max_depth=5, train_paths=1024, lr=1e30))
+    losses = []
+    for f in range(4):
+        st = r.render(1, camera=cam, frame=f)
+        losses.append(st.nrc_loss)
+        if st.nrc_queries == 0:
+            break
+    assert st.nrc_queries == 0, losses
+    assert not all(np.isfinite(losses)), losses
+    pt = aqua.integrator_default()
+    pt.max_depth = 5
+    a = aqua.Renderer(box_room, pt)
+    a.render(4, camera=cam, frame=9)
+    st = r.render(4, camera=cam, frame=9)
+    assert st.nrc_queries == 0 and st.nrc_train_records == 0
+    np.testing.assert_array_equal(r.film(cam), a.film(cam))
