@@ -143,3 +143,122 @@ def test_render_interior_ggx_parity(gpu, tmp_path):
     sc = aqua.Scene.load(path)
     g, c, _ = _render_match(sc, oracle_scene_of(sc), sc.camera(), spp=4, max_depth=8)
     np.testing.assert_array_equal(g, c)
+
+
+def test_pass_split_does_not_change_the_film(gpu, box_room, box_room_oracle):
+    # sample s of pixel p draws from a fixed position of stream p, so the film
+    # is the same however the samples are split into passes (here 3 + 3 + 1
+    # against 7 in one pass), at a resolution that is not a multiple of 64
+    cam = small_camera(box_room.camera(), 37, 23)
+    r = aqua.Renderer(box_room, cfg(spp=7))
+    r.set_opts(samples_per_pass=3)
+    st = r.render(7, frame=5, camera=cam)
+    assert st.n_passes == 3
+    split = r.film(cam)
+    r.set_opts(samples_per_pass=7)
+    st = r.render(7, frame=5, camera=cam)
+    assert st.n_passes == 1
+    np.testing.assert_array_equal(r.film(cam), split)
+    img_c, _ = box_room_oracle.render(aqua.camera_to_array(cam), 7, 5, frame=5)
+    np.testing.assert_array_equal(split, img_c)
+    # same frame again: identical; another frame: a different noise pattern
+    r.render(7, frame=5, camera=cam)
+    np.testing.assert_array_equal(r.film(cam), split)
+    r.render(7, frame=6, camera=cam)
+    assert not np.array_equal(r.film(cam), split)
+
+
+def test_single_triangle_and_empty_scenes(gpu):
+    m = aqua.Material()
+    m.base_color[:] = [0.8, 0.8, 0.8]
+    m.roughness = 0.5
+    cam = aqua.Camera()
+    cam.cam_to_world[:] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 3]
+    cam.tan_half_fov = np.float32(0.5)
+    cam.width, cam.height, cam.aspect = 33, 17, np.float32(33 / 17)
+    pl = aqua.PointLight()
+    pl.pos[:] = [0.0, 0.0, 2.0]
+    pl.intensity[:] = [4.0, 4.0, 4.0]
+    tri = np.array([[-1, -1, 0, 1, -1, 0, 0, 1, 0]], np.float32)
+    sc = aqua.Scene.from_arrays(tri, np.zeros(1, np.uint32), [m], cam, [pl])
+    orc = oracle_scene_of(sc)
+    _hits_match(sc, orc)
+    g, c, _ = _render_match(sc, orc, cam, spp=4, max_depth=3)
+    np.testing.assert_array_equal(g, c)
+    assert g.max() > 0.0
+    # no triangles at all: every primary ray misses and the film is black
+    empty = aqua.Scene.from_arrays(np.zeros((0, 9), np.float32), np.zeros(0, np.uint32), [m], cam, [pl])
+    r = aqua.Renderer(empty, cfg())
+    ids, _ = r.primary_hits()
+    assert np.all(ids == aqua.MISS)
+    st = r.render(2)
+    assert st.extension_rays == 0 and st.shadow_rays == 0
+    assert not np.any(r.film())
+
+
+def test_trace_degenerate_and_edge_rays(gpu, box_room, box_room_oracle):
+    v, _ = box_room.triangles()
+    v = v.reshape(-1, 3, 3)
+    rng = np.random.default_rng(11)
+    rays = []
+    inside = np.array([0.3, 1.5, 0.6], np.float32)  # above the short box, clear of the tall one
+    for t in v[::3]:
+        for p in (t[0], t[1], t[2], (t[0] + t[1]) / 2, (t[1] + t[2]) / 2, t.mean(0)):
+            d = p - inside
+            rays.append([*inside, 3.0e38, *(d / np.linalg.norm(d)), 0.0])   # at vertices / edges
+    for k in range(3):                                                  # axis-aligned both ways
+        for s in (-1.0, 1.0):
+            d = np.zeros(3, np.float32)
+            d[k] = s
+            rays.append([*inside, 3.0e38, *d, 0.0])
+    o = rng.uniform(-0.9, 0.9, (64, 3)).astype(np.float32)
+    o[:, 1] += 1.0
+    d = rng.standard_normal((64, 3)).astype(np.float32)
+    for i in range(64):
+        rays.append([*o[i], 0.0, *d[i], 0.0])                           # tmax 0: always a miss
+        rays.append([*o[i], 1e-6, *(d[i] * 1e-3), 0.0])                 # short, unnormalised
+    rays = np.array(rays, np.float32)
+    r = aqua.Renderer(box_room, cfg())
+    ids_g, t_g = r.trace_closest(rays)
+    ids_b, t_b = box_room_oracle.trace_closest(rays, brute=True)
+    np.testing.assert_array_equal(ids_g, ids_b)
+    np.testing.assert_array_equal(t_g, t_b)
+    assert np.all(ids_g[len(rays) - 128::2] == aqua.MISS)
+    np.testing.assert_array_equal(r.trace_occluded(rays), box_room_oracle.trace_occluded(rays, brute=True))
+
+
+def test_closed_cube_is_watertight(gpu):
+    # Rays from inside a closed cube aimed exactly at its vertices and edge
+    # midpoints (where two or three triangles meet) must all hit: a crack in
+    # the ray/triangle test would let some through.
+    c = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], np.float32)
+    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    tris = []
+    for a, b, cc, d in quads:
+        tris += [c[[a, b, cc]].reshape(-1), c[[a, cc, d]].reshape(-1)]
+    m = aqua.Material()
+    m.base_color[:] = [0.5, 0.5, 0.5]
+    cam = aqua.Camera()
+    cam.cam_to_world[:] = [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0]
+    cam.tan_half_fov = np.float32(0.5)
+    cam.width, cam.height, cam.aspect = 8, 8, np.float32(1)
+    sc = aqua.Scene.from_arrays(np.array(tris), np.zeros(len(tris), np.uint32), [m], cam)
+    targets = list(c)
+    for i in range(8):
+        for j in range(i + 1, 8):
+            if np.sum(c[i] != c[j]) == 1:
+                targets.append((c[i] + c[j]) / 2)
+    rng = np.random.default_rng(5)
+    rays = []
+    for o in rng.uniform(-0.9, 0.9, (50, 3)).astype(np.float32):
+        for p in targets:
+            d = np.asarray(p, np.float32) - o
+            rays.append([*o, 3.0e38, *d, 0.0])
+    rays = np.array(rays, np.float32)
+    r = aqua.Renderer(sc, cfg())
+    ids, t = r.trace_closest(rays)
+    assert np.all(ids != aqua.MISS), int(np.sum(ids == aqua.MISS))
+    orc = oracle_scene_of(sc)
+    ids_b, t_b = orc.trace_closest(rays, brute=True)
+    np.testing.assert_array_equal(ids, ids_b)
+    np.testing.assert_array_equal(t, t_b)
