@@ -296,6 +296,25 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
   typedef typename O::T T;
   typedef typename O::Packed Pk;
   const Pk* pf = reinterpret_cast<const Pk*>(a.pack_fwd);
+  // Inference: the block stages the forward weight operands in LDS once
+  // (34.8 KB in 16-bit, 17.4 KB in FP8), so each tile reads them at LDS
+  // bandwidth instead of re-fetching them through L1. `woff` is an opaque
+  // zero that keeps those reads inside the tile loop (hoisted, the 40
+  // operands would need ~160 VGPRs).
+  extern __shared__ __align__(16) unsigned char nrc_lds[];
+  const Pk* wl = reinterpret_cast<const Pk*>(nrc_lds);
+  int woff = 0;
+  if constexpr (!TRAIN) {
+    Pk* dst = reinterpret_cast<Pk*>(nrc_lds);
+    for (int i = int(threadIdx.x); i < kFwdPack; i += int(blockDim.x)) dst[i] = pf[i];
+    __syncthreads();
+  }
+  auto wfwd = [&](int idx) -> Pk {
+    if constexpr (TRAIN)
+      return pf[idx];
+    else
+      return wl[idx + woff];
+  };
   const int lane = int(__lane_id());
   const int g = lane >> 4, r16 = lane & 15;
   const uint32_t T_ = 1u << a.G.log2T;
@@ -316,7 +335,10 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
     // The weight operands are loop-invariant; without this barrier the
     // compiler hoists all 40 of them out of the tile loop into ~160 VGPRs.
     // Re-reading them per tile hits L1 and keeps occupancy up.
-    asm volatile("" : "+s"(pf));
+    if constexpr (TRAIN)
+      asm volatile("" : "+s"(pf));
+    else
+      asm volatile("" : "+s"(woff));
     const uint32_t row = tile * 16u + uint32_t(r16);
     const bool valid = row < n;
     const uint32_t src = (a.perm && valid) ? a.perm[row] : row;
@@ -341,7 +363,11 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
     // ---- forward
 #pragma unroll
     for (int L = 0; L <= kHidden; ++L) {
-      asm volatile("" : "+s"(pf));  // one layer's weights live at a time
+      // one layer's weights live at a time
+      if constexpr (TRAIN)
+        asm volatile("" : "+s"(pf));
+      else
+        asm volatile("" : "+s"(woff));
       float v0[8], v1[8];
       split_b(cur, v0, v1);
       float sa = 1.0f, deq = 1.0f;
@@ -368,15 +394,15 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
           f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-          acc = O::mma(O::unpack(pf[((L * 4 + mt) * 2 + 0) * 64 + lane]), b0, acc);
-          acc = O::mma(O::unpack(pf[((L * 4 + mt) * 2 + 1) * 64 + lane]), b1, acc);
+          acc = O::mma(O::unpack(wfwd(((L * 4 + mt) * 2 + 0) * 64 + lane)), b0, acc);
+          acc = O::mma(O::unpack(wfwd(((L * 4 + mt) * 2 + 1) * 64 + lane)), b1, acc);
 #pragma unroll
           for (int j = 0; j < 4; ++j) cur[mt][j] = fmaxf(acc[j] * deq, 0.0f);
         }
       } else {
         f4v acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        acc = O::mma(O::unpack(pf[kHidden * 8 * 64 + 0 * 64 + lane]), b0, acc);
-        acc = O::mma(O::unpack(pf[kHidden * 8 * 64 + 1 * 64 + lane]), b1, acc);
+        acc = O::mma(O::unpack(wfwd(kHidden * 8 * 64 + 0 * 64 + lane)), b0, acc);
+        acc = O::mma(O::unpack(wfwd(kHidden * 8 * 64 + 1 * 64 + lane)), b1, acc);
 #pragma unroll
         for (int j = 0; j < 4; ++j) y[j] = acc[j] * deq;
       }
@@ -875,14 +901,14 @@ int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* 
   uint32_t npad = a.npad;
   if (precision == AQUA_NRC_FP8) {
     a.pack_fwd = nrc->pack_fp8.p;
-    k_nrc_fwd<AQUA_NRC_FP8, false><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+    k_nrc_fwd<AQUA_NRC_FP8, false><<<nrc->fwd_grid(npad), kBlock, kFwdPack * 8, s>>>(a);
   } else {
     bool native = precision == nrc->cfg.train_precision;
     a.pack_fwd = native ? nrc->pack_fwd.p : nrc->pack_fwd_other.p;
     if (precision == AQUA_NRC_BF16)
-      k_nrc_fwd<AQUA_NRC_BF16, false><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+      k_nrc_fwd<AQUA_NRC_BF16, false><<<nrc->fwd_grid(npad), kBlock, kFwdPack * 16, s>>>(a);
     else
-      k_nrc_fwd<AQUA_NRC_F16, false><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+      k_nrc_fwd<AQUA_NRC_F16, false><<<nrc->fwd_grid(npad), kBlock, kFwdPack * 16, s>>>(a);
   }
   HIPCHK(hipGetLastError());
   return AQUA_OK;
