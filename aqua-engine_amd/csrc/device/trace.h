@@ -18,7 +18,6 @@ constexpr float kFarScale = 1.00000036f;
 struct RayPre {
   v3 o, d;
   v3 inv;     // 1 / d, with |d| clamped away from 0
-  v3 oi;      // o * inv
   int kx, ky, kz;
   float sx, sy, sz;
 };
@@ -35,7 +34,6 @@ AQ_DEV RayPre ray_pre(v3 o, v3 d) {
   r.o = o;
   r.d = d;
   r.inv = v3{safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
-  r.oi = mul(o, r.inv);
   // Woop, Benthin & Wald 2013, "Watertight Ray/Triangle Intersection":
   // kz = dominant axis of d, kx/ky the others (swapped if d[kz] < 0 to keep
   // the winding), shear constants Sx, Sy, Sz.
@@ -81,15 +79,13 @@ AQ_DEV bool tri_hit(const RayPre& r, v3 v0, v3 v1, v3 v2, float* t_out) {
   return true;
 }
 
-// Slab test of one box; returns the entry distance, or +inf when culled.
-AQ_DEV float box_enter(const RayPre& r, float lx, float hx, float ly, float hy, float lz, float hz,
-                       float tbest, float delta) {
-  float t0x = __builtin_fmaf(lx, r.inv.x, -r.oi.x), t1x = __builtin_fmaf(hx, r.inv.x, -r.oi.x);
-  float t0y = __builtin_fmaf(ly, r.inv.y, -r.oi.y), t1y = __builtin_fmaf(hy, r.inv.y, -r.oi.y);
-  float t0z = __builtin_fmaf(lz, r.inv.z, -r.oi.z), t1z = __builtin_fmaf(hz, r.inv.z, -r.oi.z);
-  // The t values are finite (|1/d| is clamped), so the hardware min / max
-  // (v_min3 / v_max3) apply; culling only has to be conservative, the hit
-  // itself does not depend on it.
+// Slab test of one box from its six plane distances along the ray; returns
+// the entry distance, or +inf when culled. The t values are finite (|1/d| is
+// clamped), so the hardware min / max (v_min3 / v_max3) apply. Culling only
+// has to be conservative: the far bound is widened by kFarScale and delta,
+// which also covers the rounding of the plane distances.
+AQ_DEV float slab_enter(float t0x, float t1x, float t0y, float t1y, float t0z, float t1z, float tbest,
+                        float delta) {
   float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(t0x, t1x), __builtin_fminf(t0y, t1y)),
                              __builtin_fmaxf(__builtin_fminf(t0z, t1z), 0.0f));
   float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(t0x, t1x), __builtin_fmaxf(t0y, t1y)),
@@ -98,7 +94,7 @@ AQ_DEV float box_enter(const RayPre& r, float lx, float hx, float ly, float hy, 
 }
 
 struct DevBvh {
-  const float4* nodes;  // 8 per 4-wide node
+  const float4* nodes;  // 4 per 4-wide node (64-byte quantised nodes)
   const float4* tris;   // 3 per triangle slot: (v0, id), (v1, material), (v2, 0)
   float delta;
   uint32_t* spill;      // traversal spill stacks: kSpillStack x (grid threads) words
@@ -167,28 +163,37 @@ AQ_DEV uint32_t pick4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t k
   return (k & 2u) ? hi : lo;
 }
 
-// Four slab tests of one node, straight-line (all four boxes are tested, then
-// empty children are masked), so the node is eight 16-byte loads with no
-// branch between them. Empty children get +inf. Child references come back
-// packed: (index << 3) | triangle count (count 0 = inner node; counts are at
-// most 7).
+// Byte k of a word as a float (v_cvt_f32_ubyteK).
+AQ_DEV float ubyte(uint32_t w, int k) { return float((w >> (8 * k)) & 0xFFu); }
+
+// Four slab tests of one 64-byte quantised node (four 16-byte loads issued
+// together, straight-line code). A child plane is origin + q * 2^e, so its
+// distance along the ray is q * (2^e / d) + (origin - o) / d: one FMA per
+// plane after six per-node products. Empty children get +inf. ref[k] is
+// the packed child reference (index << 3) | triangle count.
 AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float delta, float tn[4],
                        uint32_t ref[4]) {
-  float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-  int4 child = *reinterpret_cast<const int4*>(np + 6);
-  int4 count = *reinterpret_cast<const int4*>(np + 7);
-  float b0 = box_enter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tbest, delta);
-  float b1 = box_enter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tbest, delta);
-  float b2 = box_enter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tbest, delta);
-  float b3 = box_enter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tbest, delta);
-  tn[0] = child.x >= 0 ? b0 : __builtin_inff();
-  tn[1] = child.y >= 0 ? b1 : __builtin_inff();
-  tn[2] = child.z >= 0 ? b2 : __builtin_inff();
-  tn[3] = child.w >= 0 ? b3 : __builtin_inff();
-  ref[0] = (uint32_t(child.x) << 3) | uint32_t(count.x);
-  ref[1] = (uint32_t(child.y) << 3) | uint32_t(count.y);
-  ref[2] = (uint32_t(child.z) << 3) | uint32_t(count.z);
-  ref[3] = (uint32_t(child.w) << 3) | uint32_t(count.w);
+  const float4 h = np[0];
+  const uint4 q0 = *reinterpret_cast<const uint4*>(np + 1);  // lo.x hi.x lo.y hi.y
+  const uint4 q1 = *reinterpret_cast<const uint4*>(np + 2);  // lo.z hi.z ref0 ref1
+  const uint4 q2 = *reinterpret_cast<const uint4*>(np + 3);  // ref2 ref3 - -
+  const uint32_t e = __float_as_uint(h.w);
+  const float sx = __uint_as_float((e & 0xFFu) << 23), sy = __uint_as_float(((e >> 8) & 0xFFu) << 23),
+              sz = __uint_as_float(((e >> 16) & 0xFFu) << 23);
+  const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
+  const float bx = (h.x - r.o.x) * r.inv.x, by = (h.y - r.o.y) * r.inv.y, bz = (h.z - r.o.z) * r.inv.z;
+  ref[0] = q1.z;
+  ref[1] = q1.w;
+  ref[2] = q2.x;
+  ref[3] = q2.y;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float t = slab_enter(__builtin_fmaf(ubyte(q0.x, k), ax, bx), __builtin_fmaf(ubyte(q0.y, k), ax, bx),
+                         __builtin_fmaf(ubyte(q0.z, k), ay, by), __builtin_fmaf(ubyte(q0.w, k), ay, by),
+                         __builtin_fmaf(ubyte(q1.x, k), az, bz), __builtin_fmaf(ubyte(q1.y, k), az, bz), tbest,
+                         delta);
+    tn[k] = ref[k] != 0xFFFFFFFFu ? t : __builtin_inff();
+  }
 }
 
 AQ_DEV void cswap(uint32_t& a, uint32_t& b) {
@@ -223,7 +228,7 @@ template <bool COUNT>
 AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCount* cnt) {
   float tn[4];
   uint32_t ref[4];
-  node_test4(c.r, bvh.nodes + 8 * c.node, c.tbest, bvh.delta, tn, ref);
+  node_test4(c.r, bvh.nodes + 4 * c.node, c.tbest, bvh.delta, tn, ref);
   if (COUNT) cnt->nodes++;
   // Order hit children near to far: key = entry distance bits with the
   // child index in the low two bits (entry distances are >= 0).
@@ -324,7 +329,7 @@ template <bool COUNT>
 AQ_DEV int any_step(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
   float tn[4];
   uint32_t ref[4];
-  node_test4(a.r, bvh.nodes + 8 * a.node, a.tmax, bvh.delta, tn, ref);
+  node_test4(a.r, bvh.nodes + 4 * a.node, a.tmax, bvh.delta, tn, ref);
   if (COUNT) cnt->nodes++;
   int next = -1;
 #pragma unroll

@@ -364,6 +364,55 @@ uint32_t collapse_rec(const Bvh& bin, std::vector<Ref> refs, uint32_t depth, Bvh
 
 }  // namespace
 
+void quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out) {
+  out->assign(size_t(kNodeWords) * in.nodes.size(), 0u);
+  for (size_t i = 0; i < in.nodes.size(); ++i) {
+    const Bvh4Node& n = in.nodes[i];
+    uint32_t* w = out->data() + kNodeWords * i;
+    const float* lo[3] = {n.lo_x, n.lo_y, n.lo_z};
+    const float* hi[3] = {n.hi_x, n.hi_y, n.hi_z};
+    float origin[3];
+    int ex[3];
+    for (int a = 0; a < 3; ++a) {
+      float o = INFINITY, h = -INFINITY;
+      for (int k = 0; k < 4; ++k)
+        if (n.child[k] >= 0) {
+          o = std::min(o, lo[a][k]);
+          h = std::max(h, hi[a][k]);
+        }
+      if (!(o <= h)) o = h = 0.0f;  // no children
+      origin[a] = o;
+      // smallest power of two with 255 steps covering the extent
+      double ext = double(h) - double(o);
+      int e = -100;
+      if (ext > 0.0) {
+        e = std::ilogb(ext) - 8;
+        while (std::ldexp(255.0, e) < ext) ++e;
+        while (e > -100 && std::ldexp(255.0, e - 1) >= ext) --e;
+      }
+      ex[a] = std::max(-100, std::min(100, e));
+    }
+    for (int a = 0; a < 3; ++a) memcpy(&w[a], &origin[a], 4);
+    w[3] = uint32_t(ex[0] + 127) | (uint32_t(ex[1] + 127) << 8) | (uint32_t(ex[2] + 127) << 16);
+    for (int k = 0; k < 4; ++k) {
+      if (n.child[k] < 0) {
+        w[10 + k] = kNoChild;
+        continue;
+      }
+      for (int a = 0; a < 3; ++a) {
+        double sc = std::ldexp(1.0, ex[a]);
+        double ql = std::floor((double(lo[a][k]) - origin[a]) / sc);
+        double qh = std::ceil((double(hi[a][k]) - origin[a]) / sc);
+        uint32_t bl = uint32_t(std::max(0.0, std::min(255.0, ql)));
+        uint32_t bh = uint32_t(std::max(0.0, std::min(255.0, qh)));
+        w[4 + 2 * a] |= bl << (8 * k);
+        w[5 + 2 * a] |= bh << (8 * k);
+      }
+      w[10 + k] = (uint32_t(n.child[k]) << 3) | uint32_t(n.count[k]);
+    }
+  }
+}
+
 void collapse_bvh4(const Bvh& bin, Bvh4* out) {
   *out = Bvh4();
   out->nodes.reserve(bin.nodes.size() / 2 + 1);

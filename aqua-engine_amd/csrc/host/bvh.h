@@ -55,6 +55,20 @@ struct Bvh4 {
   uint32_t max_inner_depth = 0; // deepest inner node (root = 0)
 };
 
+// Device node format: 64 bytes = 16 words per 4-wide node.
+//   w0..2  origin (float): the low corner of the union of the child boxes
+//   w3     per-axis power-of-two scales as biased exponents (bits 0-7 x,
+//          8-15 y, 16-23 z): plane = origin + q * 2^(e - 127)
+//   w4..9  child planes as bytes (byte k = child k): lo.x, hi.x, lo.y, hi.y,
+//          lo.z, hi.z, each rounded outward (lo down, hi up)
+//   w10..13 child references (index << 3) | triangle count (count 0 = inner
+//          node, count <= 7); 0xFFFFFFFF = no child
+//   w14..15 zero
+// Decoded boxes enclose the original ones, so traversal stays conservative.
+constexpr uint32_t kNodeWords = 16;
+constexpr uint32_t kNoChild = 0xFFFFFFFFu;
+void quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out);
+
 // Collapses a binary tree into 4-wide nodes: each node repeatedly opens its
 // inner child of largest surface area until it has four children (the
 // usual greedy SAH-preserving collapse). Triangle slots are unchanged.

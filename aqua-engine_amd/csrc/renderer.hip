@@ -1228,8 +1228,10 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   }
   r->n_tris = n;
   r->tri_order = bvh.tri_order;
-  std::vector<float4> nodes(8 * bvh4.nodes.size());
-  memcpy(nodes.data(), bvh4.nodes.data(), bvh4.nodes.size() * sizeof(aqua::Bvh4Node));
+  std::vector<uint32_t> qnodes;
+  aqua::quantize_bvh4(bvh4, &qnodes);
+  std::vector<float4> nodes(qnodes.size() / 4);
+  memcpy(nodes.data(), qnodes.data(), qnodes.size() * sizeof(uint32_t));
   // Culling slack: 2^-16 of the largest coordinate magnitude.
   float mx = 0.0f;
   for (int k = 0; k < 3; ++k) {
@@ -1272,9 +1274,10 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   }
   if (n == 0) {
     // An empty scene still needs a valid root: one node, all children empty.
-    nodes.assign(8, make_float4(0, 0, 0, 0));
-    int32_t e4[4] = {-1, -1, -1, -1};
-    memcpy(&nodes[6], e4, 16);
+    nodes.assign(4, make_float4(0, 0, 0, 0));
+    uint32_t e4[4] = {aqua::kNoChild, aqua::kNoChild, 0u, 0u};
+    memcpy(&nodes[2].z, e4, 8);
+    memcpy(&nodes[3], e4, 8);
     tris.assign(3, make_float4(0, 0, 0, 0));
   }
   hipStream_t s = r->stream;

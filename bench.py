@@ -29,9 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "ms/frame to relMSE 1e-2 vs FP64 oracle (NRC vs plain PT); Mrays/s"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one closest-hit ray: the ray (origin+tmax, direction:
-# 32 B) read once, the hit record (slot, t: 8 B) written once, one 128-byte
-# 4-wide node per node visit and one 48-byte triangle per triangle test.
-RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 32, 8, 128, 48
+# 32 B) read once, the hit record (slot, t: 8 B) written once, one 64-byte
+# quantised 4-wide node per node visit and one 48-byte triangle per triangle
+# test.
+RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 32, 8, 64, 48
 
 
 def _load(name, path):

@@ -335,11 +335,12 @@ int aqua_abi_version(void);
 /* Number of HIP devices visible (0 on a host without a GPU). */
 int aqua_device_count(int* count);
 
-/* Diagnostics: the BVH the renderer builds for `scene`, in its device
- * layout: 128-byte 4-wide nodes of 32 floats -- lo.x[4], hi.x[4], lo.y[4],
- * hi.y[4], lo.z[4], hi.z[4], then int32 index[4] and count[4]; count > 0 =
- * leaf over triangle slots [index, index + count), count == 0 = inner node
- * `index`, index < 0 = empty. Call with nodes == NULL to get *n_nodes, then
+/* Diagnostics: the BVH the renderer builds for `scene`, as exact 4-wide
+ * nodes of 32 floats (128 bytes) -- lo.x[4], hi.x[4], lo.y[4], hi.y[4],
+ * lo.z[4], hi.z[4], then int32 index[4] and count[4]; count > 0 = leaf over
+ * triangle slots [index, index + count), count == 0 = inner node `index`,
+ * index < 0 = empty. (On the device each node is stored in 64 bytes with its
+ * child boxes quantised outward to 8 bits per plane.) Call with nodes == NULL to get *n_nodes, then
  * again with room for that many; order[slot] = triangle id; max_depth =
  * depth of the deepest inner node. */
 int aqua_scene_bvh(const aqua_scene* scene, uint32_t* n_nodes, float* nodes, uint32_t* order,
