@@ -140,6 +140,7 @@ _SIGS = {
     "aqua_scene_get_point_lights": (C.c_int, [_P, _P]),
     "aqua_scene_destroy": (None, [_P]),
     "aqua_scene_bvh": (C.c_int, [_P, C.POINTER(C.c_uint32), _P, _P, C.POINTER(C.c_uint32)]),
+    "aqua_scene_bvh_device": (C.c_int, [_P, C.POINTER(C.c_uint32), _P]),
     "aqua_procgen_write": (C.c_int, [C.POINTER(ProcgenParams), C.c_char_p, C.c_char_p, C.c_size_t]),
     "aqua_integrator_default": (None, [C.POINTER(IntegratorCfg)]),
     "aqua_integrator_load": (C.c_int, [C.c_char_p, C.POINTER(IntegratorCfg)]),
@@ -283,8 +284,16 @@ class Scene:
         _check(lib().aqua_scene_get_point_lights(self._h, _ptr(a) if n else None))
         return a
 
+    def bvh_device(self) -> np.ndarray:
+        """The same tree as stored on the device: uint32[n, 16] quantised 64-byte nodes."""
+        n = C.c_uint32(0)
+        _check(lib().aqua_scene_bvh_device(self._h, C.byref(n), None))
+        words = np.zeros((n.value, 16), np.uint32)
+        _check(lib().aqua_scene_bvh_device(self._h, C.byref(n), _ptr(words)))
+        return words
+
     def bvh(self):
-        """(nodes float32[n,32] in the device's 4-wide layout, order uint32[ntri], max_depth)."""
+        """(nodes float32[n,32]: the exact 4-wide tree, order uint32[ntri], max_depth)."""
         n = C.c_uint32(0)
         d = C.c_uint32(0)
         _check(lib().aqua_scene_bvh(self._h, C.byref(n), None, None, C.byref(d)))

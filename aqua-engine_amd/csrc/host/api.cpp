@@ -99,6 +99,25 @@ int aqua_scene_bvh(const aqua_scene* s, uint32_t* n_nodes, float* nodes, uint32_
   })
 }
 
+int aqua_scene_bvh_device(const aqua_scene* s, uint32_t* n_nodes, uint32_t* words) {
+  if (!s || !n_nodes) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  AQUA_GUARD({
+    aqua::Bvh b;
+    aqua::BvhBuildOpts o;
+    aqua::build_bvh(s->verts.data(), uint32_t(s->mat_ids.size()), o, &b);
+    aqua::Bvh4 b4;
+    aqua::collapse_bvh4(b, &b4);
+    if (words) {
+      if (*n_nodes < b4.nodes.size()) return aqua::fail(AQUA_ERR_INVALID_ARG, "node buffer too small");
+      std::vector<uint32_t> q;
+      aqua::quantize_bvh4(b4, &q);
+      memcpy(words, q.data(), q.size() * sizeof(uint32_t));
+    }
+    *n_nodes = uint32_t(b4.nodes.size());
+    return AQUA_OK;
+  })
+}
+
 int aqua_procgen_write(const aqua_procgen_params* p, const char* out_dir, char* scene_path, size_t cap) {
   AQUA_GUARD(return aqua::procgen_write(p, out_dir, scene_path, cap);)
 }

@@ -345,6 +345,14 @@ int aqua_device_count(int* count);
  * depth of the deepest inner node. */
 int aqua_scene_bvh(const aqua_scene* scene, uint32_t* n_nodes, float* nodes, uint32_t* order,
                    uint32_t* max_depth);
+/* Diagnostics: the same tree as the renderer stores it on the device, 16
+ * words (64 bytes) per node: words 0-2 the node origin (float), word 3 the
+ * biased exponents e_x | e_y << 8 | e_z << 16 of the per-axis power-of-two
+ * scales, words 4-9 the child planes lo.x, hi.x, lo.y, hi.y, lo.z, hi.z as
+ * one byte per child (byte k = child k; plane = origin + q * 2^(e - 127),
+ * rounded outward), words 10-13 the child references (index << 3) | count
+ * (0xFFFFFFFF = empty), words 14-15 zero. Same calling pattern as above. */
+int aqua_scene_bvh_device(const aqua_scene* scene, uint32_t* n_nodes, uint32_t* words);
 
 /* Diagnostics: the device's FP32 arithmetic on n inputs, on the current
  * device: out[5i..5i+4] = a/b, sqrt(a), fma(a, b, c), sin(2 pi u), cos(2 pi u),

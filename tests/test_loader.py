@@ -316,3 +316,40 @@ def test_scene_bvh_invariants(box_room, tmp_path):
     _check_bvh4(nodes, order, verts)
     # 4-wide: far fewer nodes than triangles, and the tree stays shallow
     assert len(nodes) < len(verts) / 2 and depth < 20
+
+
+def _decode_device_nodes(words):
+    """Float64 decode of the 64-byte device nodes: (lo[n,4,3], hi[n,4,3], ref[n,4])."""
+    origin = words[:, 0:3].view(np.float32).astype(np.float64)
+    e = np.stack([(words[:, 3] >> (8 * a)) & 0xFF for a in range(3)], 1).astype(np.int64) - 127
+    scale = np.ldexp(1.0, e)                                                     # [n, 3]
+    q = np.stack([(words[:, 4:10] >> (8 * k)) & 0xFF for k in range(4)], 1)     # [n, 4, 6]
+    lo = origin[:, None, :] + q[:, :, 0::2] * scale[:, None, :]
+    hi = origin[:, None, :] + q[:, :, 1::2] * scale[:, None, :]
+    return lo, hi, words[:, 10:14]
+
+
+def test_device_nodes_enclose_the_exact_boxes(box_room, tmp_path):
+    # the quantised device nodes: same children and references, and every
+    # decoded child box encloses the exact one (planes rounded outward), so
+    # culling against them stays conservative
+    p = aqua.procgen(aqua.PROCGEN_INTERIOR, str(tmp_path), seed=4, n_meshes=12, target_tris=20000,
+                     width=8, height=8)
+    for sc in (box_room, aqua.Scene.load(p)):
+        nodes, _, _ = sc.bvh()
+        words = sc.bvh_device()
+        assert words.shape == (len(nodes), 16) and not np.any(words[:, 14:16])
+        lo, hi, ref = _decode_device_nodes(words)
+        idx = nodes[:, 24:28].view(np.int32)
+        cnt = nodes[:, 28:32].view(np.int32)
+        used = idx >= 0
+        np.testing.assert_array_equal(ref == 0xFFFFFFFF, ~used)
+        np.testing.assert_array_equal(ref[used] >> 3, idx[used].astype(np.uint32))
+        np.testing.assert_array_equal(ref[used] & 7, cnt[used].astype(np.uint32))
+        exact_lo = np.stack([nodes[:, 0:4], nodes[:, 8:12], nodes[:, 16:20]], 2).astype(np.float64)
+        exact_hi = np.stack([nodes[:, 4:8], nodes[:, 12:16], nodes[:, 20:24]], 2).astype(np.float64)
+        assert np.all(lo[used] <= exact_lo[used]) and np.all(hi[used] >= exact_hi[used])
+        # and tight: at most one quantisation step (extent / 255) of slack
+        e = np.stack([(words[:, 3] >> (8 * a)) & 0xFF for a in range(3)], 1).astype(np.int64) - 127
+        step = np.broadcast_to(np.ldexp(1.0, e)[:, None, :], lo.shape)
+        assert np.all((exact_lo - lo)[used] <= step[used]) and np.all((hi - exact_hi)[used] <= step[used])
