@@ -238,7 +238,7 @@ __device__ __forceinline__ uint32_t feed_take(LaneFeed& f, bool wants) {
 constexpr uint32_t kRefillBelow = 40;
 
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_intersect(DevBvh bvh, const float4* __restrict__ ro,
+__global__ void __launch_bounds__(kBlock, 6) k_intersect(DevBvh bvh, const float4* __restrict__ ro,
                                                       const float4* __restrict__ rd,
                                                       const uint32_t* count, uint32_t* cursor, uint2* hit,
                                                       unsigned long long* stats) {
@@ -543,7 +543,7 @@ __global__ void k_clear_taux(float4* taux, uint32_t n_slots) {
 }
 
 template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_shadow(DevBvh bvh, Queues q, uint32_t b) {
+__global__ void __launch_bounds__(kBlock, 6) k_shadow(DevBvh bvh, Queues q, uint32_t b) {
   __shared__ uint32_t lds_stack[kLdsStack * kBlock];
   TStack st;
   st_init(st, lds_stack, bvh.spill);
