@@ -238,8 +238,8 @@ __device__ __forceinline__ void encode16(const FwdArgs& a, uint32_t T_, int g, b
       float w = corner_w(c, k);
       float e0, e1;
       load_entry(a.table16, size_t(lv) * T_ + idx, &e0, &e1);
-      s0 += w * e0;
-      s1 += w * e1;
+      s0 = __builtin_fmaf(w, e0, s0);
+      s1 = __builtin_fmaf(w, e1, s1);
     }
     f[2 * q] = valid ? s0 : 0.0f;
     f[2 * q + 1] = valid ? s1 : 0.0f;
