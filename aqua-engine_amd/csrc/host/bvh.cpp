@@ -295,9 +295,43 @@ struct Ref {
   }
 };
 
-void refs_of(const BvhNode& n, std::vector<Ref>* out) {
-  for (int side = 0; side < 2; ++side) {
-    if (n.child[side] < 0) continue;
+// ---- SAH-optimal collapse to 4-wide nodes -----------------------------------
+//
+// An entry is one side of a binary node (a child box with a triangle range or
+// a binary subtree). D(e, k) is the least SAH cost of covering entry e's
+// subtree with at most k child slots of the enclosing 4-wide node:
+//   leaf entry:          D(e, k) = A(e) N(e) c_tri
+//   inner entry, k = 1:  min(A(e) N(e) c_tri   [whole subtree as one leaf, N <= max_leaf],
+//                            A(e) c_node + min_j D(L, j) + D(R, 4 - j)   [a 4-wide node])
+//   inner entry, k >= 2: min(D(e, k - 1), min_j D(L, j) + D(R, k - j))   [e opened]
+// (the dynamic program of Ylitie, Karras & Laine 2017, for width 4). Costs
+// are in units of one triangle test; a 4-wide node visit costs c_node.
+
+constexpr double kNodeCost = 1.0;
+
+struct Collapse {
+  const Bvh& bin;
+  int max_leaf;
+  // per entry (2 per binary node)
+  std::vector<double> d;         // D(e, 1..4)
+  std::vector<int8_t> kj;        // for k = 2..4 (index k - 1): 0 = as for k - 1, else j slots to the left side
+  std::vector<int8_t> node_j;    // the split of the 4-wide node made from e
+  std::vector<uint8_t> as_leaf;  // D(e, 1) covers e with one leaf
+  std::vector<uint32_t> n_tris, first;
+
+  Collapse(const Bvh& b, int ml) : bin(b), max_leaf(ml) {
+    size_t ne = 2 * b.nodes.size();
+    d.assign(4 * ne, 0.0);
+    kj.assign(4 * ne, 0);
+    node_j.assign(ne, 1);
+    as_leaf.assign(ne, 0);
+    n_tris.assign(ne, 0);
+    first.assign(ne, 0);
+  }
+
+  Ref ref_of(uint32_t e) const {
+    const BvhNode& n = bin.nodes[e >> 1];
+    int side = int(e & 1u);
     Ref r;
     r.lo[0] = n.f[side * 4 + 0];
     r.hi[0] = n.f[side * 4 + 1];
@@ -307,60 +341,126 @@ void refs_of(const BvhNode& n, std::vector<Ref>* out) {
     r.hi[2] = n.f[9 + side * 2];
     r.child = n.child[side];
     r.count = n.count[side];
-    out->push_back(r);
+    return r;
   }
-}
 
-uint32_t collapse_rec(const Bvh& bin, std::vector<Ref> refs, uint32_t depth, Bvh4* out) {
-  while (refs.size() < 4) {
-    int best = -1;
-    double best_area = -1.0;
-    for (size_t i = 0; i < refs.size(); ++i)
-      if (refs[i].count == 0 && refs[i].child >= 0 && refs[i].area() > best_area) {
-        best_area = refs[i].area();
-        best = int(i);
-      }
-    if (best < 0) break;
-    std::vector<Ref> sub;
-    refs_of(bin.nodes[size_t(refs[size_t(best)].child)], &sub);
-    refs.erase(refs.begin() + best);
-    if (refs.size() + sub.size() > 4) {
-      // cannot happen (4 - 1 + 2 > 4 only when size was 4)
-      refs.insert(refs.begin() + best, sub.begin(), sub.end());
-      break;
+  // Bottom-up over the binary tree (children before parents).
+  void solve() {
+    std::vector<uint32_t> order;
+    std::vector<uint32_t> stack{0};
+    while (!stack.empty()) {
+      uint32_t m = stack.back();
+      stack.pop_back();
+      order.push_back(m);
+      for (int side = 0; side < 2; ++side)
+        if (bin.nodes[m].child[side] >= 0 && bin.nodes[m].count[side] == 0)
+          stack.push_back(uint32_t(bin.nodes[m].child[side]));
     }
-    refs.insert(refs.begin() + best, sub.begin(), sub.end());
+    for (size_t i = order.size(); i-- > 0;)
+      for (int side = 0; side < 2; ++side) solve_entry(2 * order[i] + uint32_t(side));
   }
-  uint32_t id = uint32_t(out->nodes.size());
-  out->nodes.emplace_back();
-  out->max_inner_depth = std::max(out->max_inner_depth, depth);
-  Bvh4Node node;
-  memset(&node, 0, sizeof node);
-  for (int k = 0; k < 4; ++k) {
-    node.child[k] = -1;
-    node.count[k] = 0;
-  }
-  for (size_t k = 0; k < refs.size(); ++k) {
-    const Ref& r = refs[k];
-    node.lo_x[k] = r.lo[0];
-    node.hi_x[k] = r.hi[0];
-    node.lo_y[k] = r.lo[1];
-    node.hi_y[k] = r.hi[1];
-    node.lo_z[k] = r.lo[2];
-    node.hi_z[k] = r.hi[2];
+
+  void solve_entry(uint32_t e) {
+    const Ref r = ref_of(e);
+    double* D = &d[4 * size_t(e)];
+    if (r.child < 0) return;  // empty side of a single-leaf root: costs nothing
+    const double A = r.area();
     if (r.count > 0) {
-      node.child[k] = r.child;
-      node.count[k] = r.count;
-    } else {
-      std::vector<Ref> sub;
-      refs_of(bin.nodes[size_t(r.child)], &sub);
-      node.child[k] = int32_t(collapse_rec(bin, sub, depth + 1, out));
-      node.count[k] = 0;
+      n_tris[e] = uint32_t(r.count);
+      first[e] = uint32_t(r.child);
+      as_leaf[e] = 1;
+      for (int k = 0; k < 4; ++k) D[k] = A * r.count;
+      return;
+    }
+    const uint32_t L = 2 * uint32_t(r.child), R = L + 1;
+    n_tris[e] = n_tris[L] + n_tris[R];
+    first[e] = first[L];
+    const double* DL = &d[4 * size_t(L)];
+    const double* DR = &d[4 * size_t(R)];
+    double open[5];
+    int8_t oj[5];
+    for (int k = 2; k <= 4; ++k) {
+      open[k] = INFINITY;
+      oj[k] = 1;
+      for (int j = 1; j < k; ++j) {
+        double c = DL[j - 1] + DR[k - j - 1];
+        if (c < open[k]) {
+          open[k] = c;
+          oj[k] = int8_t(j);
+        }
+      }
+    }
+    node_j[e] = oj[4];
+    const double node = A * kNodeCost + open[4];
+    const double leaf = n_tris[e] <= uint32_t(max_leaf) ? A * double(n_tris[e]) : INFINITY;
+    as_leaf[e] = leaf <= node ? 1 : 0;
+    D[0] = std::min(leaf, node);
+    for (int k = 2; k <= 4; ++k) {
+      if (open[k] < D[k - 2]) {
+        D[k - 1] = open[k];
+        kj[4 * size_t(e) + k - 1] = oj[k];
+      } else {
+        D[k - 1] = D[k - 2];
+        kj[4 * size_t(e) + k - 1] = 0;
+      }
     }
   }
-  out->nodes[id] = node;
-  return id;
-}
+
+  // The slots that cover entry e with at most k of them.
+  void distribute(uint32_t e, int k, std::vector<uint32_t>* slots) const {
+    const Ref r = ref_of(e);
+    if (r.child < 0) return;
+    if (r.count > 0 || k == 1) {
+      slots->push_back(e);
+      return;
+    }
+    const int j = kj[4 * size_t(e) + k - 1];
+    if (j == 0) {
+      distribute(e, k - 1, slots);
+      return;
+    }
+    const uint32_t L = 2 * uint32_t(r.child), R = L + 1;
+    distribute(L, j, slots);
+    distribute(R, k - j, slots);
+  }
+
+  // Emits the 4-wide node made from binary node m, its sides sharing the
+  // four slots j : 4 - j; returns its index (preorder).
+  uint32_t emit(uint32_t m, int j, uint32_t depth, Bvh4* out) const {
+    std::vector<uint32_t> slots;
+    if (bin.nodes[m].child[1] < 0) {
+      distribute(2 * m, 4, &slots);
+    } else {
+      distribute(2 * m, j, &slots);
+      distribute(2 * m + 1, 4 - j, &slots);
+    }
+    const uint32_t id = uint32_t(out->nodes.size());
+    out->nodes.emplace_back();
+    out->max_inner_depth = std::max(out->max_inner_depth, depth);
+    Bvh4Node node;
+    memset(&node, 0, sizeof node);
+    for (int k = 0; k < 4; ++k) node.child[k] = -1;
+    for (size_t k = 0; k < slots.size(); ++k) {
+      const uint32_t e = slots[k];
+      const Ref r = ref_of(e);
+      node.lo_x[k] = r.lo[0];
+      node.hi_x[k] = r.hi[0];
+      node.lo_y[k] = r.lo[1];
+      node.hi_y[k] = r.hi[1];
+      node.lo_z[k] = r.lo[2];
+      node.hi_z[k] = r.hi[2];
+      if (as_leaf[e]) {
+        node.child[k] = int32_t(first[e]);
+        node.count[k] = int32_t(n_tris[e]);
+      } else {
+        node.child[k] = int32_t(emit(uint32_t(r.child), node_j[e], depth + 1, out));
+        node.count[k] = 0;
+      }
+    }
+    out->nodes[id] = node;
+    return id;
+  }
+};
 
 }  // namespace
 
@@ -413,12 +513,25 @@ void quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out) {
   }
 }
 
-void collapse_bvh4(const Bvh& bin, Bvh4* out) {
+void collapse_bvh4(const Bvh& bin, Bvh4* out, int max_leaf) {
   *out = Bvh4();
+  if (bin.nodes.empty()) return;
   out->nodes.reserve(bin.nodes.size() / 2 + 1);
-  std::vector<Ref> refs;
-  if (!bin.nodes.empty()) refs_of(bin.nodes[0], &refs);
-  collapse_rec(bin, refs, 0, out);
+  Collapse c(bin, max_leaf);
+  c.solve();
+  // the root is always a 4-wide node: best split of its 4 slots
+  const double* DL = &c.d[0];
+  const double* DR = &c.d[4];
+  int j = 1;
+  double best = INFINITY;
+  for (int a = 1; a < 4; ++a) {
+    double v = DL[a - 1] + DR[4 - a - 1];
+    if (v < best) {
+      best = v;
+      j = a;
+    }
+  }
+  c.emit(0, j, 0, out);
 }
 
 }  // namespace aqua

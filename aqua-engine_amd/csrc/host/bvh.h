@@ -69,9 +69,11 @@ constexpr uint32_t kNodeWords = 16;
 constexpr uint32_t kNoChild = 0xFFFFFFFFu;
 void quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out);
 
-// Collapses a binary tree into 4-wide nodes: each node repeatedly opens its
-// inner child of largest surface area until it has four children (the
-// usual greedy SAH-preserving collapse). Triangle slots are unchanged.
-void collapse_bvh4(const Bvh& bin, Bvh4* out);
+// Collapses a binary tree into 4-wide nodes with the least SAH cost (a
+// dynamic program over the binary tree choosing, for every wide node, which
+// binary descendants become its up to four children and which subtrees of
+// at most max_leaf triangles become single leaves). Triangle slots are
+// unchanged.
+void collapse_bvh4(const Bvh& bin, Bvh4* out, int max_leaf = 4);
 
 }  // namespace aqua
