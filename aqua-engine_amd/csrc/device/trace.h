@@ -202,6 +202,22 @@ AQ_DEV void cswap(uint32_t& a, uint32_t& b) {
   b = hi;
 }
 
+// The hit leaf children of a node as a bit mask over the node's run of leaf
+// slots: the builder gives every node's leaf children one contiguous run (at
+// most 4 x 7 slots), so bit i is slot *base + i.
+AQ_DEV uint32_t leaf_mask(const float tn[4], const uint32_t ref[4], uint32_t* base) {
+  uint32_t b = 0xFFFFFFFFu;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (tn[k] != __builtin_inff() && (ref[k] & 7u) != 0u) b = min(b, ref[k] >> 3);
+  uint32_t m = 0u;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (tn[k] != __builtin_inff() && (ref[k] & 7u) != 0u) m |= ((1u << (ref[k] & 7u)) - 1u) << ((ref[k] >> 3) - b);
+  *base = b;
+  return m;
+}
+
 // ---- resumable traversal: the state of one ray between node visits, so a
 // persistent wave can give a lane a new ray as soon as its last one ends
 // instead of idling until the wave's slowest ray is done.
@@ -230,68 +246,57 @@ AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCoun
   uint32_t ref[4];
   node_test4(c.r, bvh.nodes + 4 * c.node, c.tbest, bvh.delta, tn, ref);
   if (COUNT) cnt->nodes++;
-  // Order hit children near to far: key = entry distance bits with the
-  // child index in the low two bits (entry distances are >= 0).
+  // Hit leaves: every triangle of them, two per round, in one loop (the
+  // result does not depend on their order).
+  uint32_t base;
+  uint32_t m = leaf_mask(tn, ref, &base);
+  while (m) {
+    const uint32_t i0 = __builtin_ctz(m);
+    m &= m - 1u;
+    const bool two = m != 0u;
+    const uint32_t i1 = two ? uint32_t(__builtin_ctz(m)) : i0;
+    m &= m - 1u;
+    const float4* ta = bvh.tris + 3 * (base + i0);
+    const float4* tb = bvh.tris + 3 * (base + i1);
+    const float4 a0 = ta[0], a1 = ta[1], a2 = ta[2];
+    const float4 b0 = tb[0], b1 = tb[1], b2 = tb[2];
+    if (COUNT) cnt->tris += two ? 2u : 1u;
+    float t;
+    if (tri_hit(c.r, xyz(a0), xyz(a1), xyz(a2), &t)) {
+      uint32_t id = __float_as_uint(a0.w);
+      if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
+        c.tbest = t;
+        c.best_id = id;
+        c.best_slot = base + i0;
+      }
+    }
+    if (two && tri_hit(c.r, xyz(b0), xyz(b1), xyz(b2), &t)) {
+      uint32_t id = __float_as_uint(b0.w);
+      if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
+        c.tbest = t;
+        c.best_id = id;
+        c.best_slot = base + i1;
+      }
+    }
+  }
+  // Hit inner children near to far: key = entry distance bits with the
+  // child index in the low two bits (entry distances are >= 0); the nearest
+  // is visited next, the others are pushed far-first.
   uint32_t key[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
-    key[k] = tn[k] != __builtin_inff() ? ((__float_as_uint(tn[k]) & ~3u) | uint32_t(k)) : 0xFFFFFFFFu;
+    key[k] = (tn[k] != __builtin_inff() && (ref[k] & 7u) == 0u) ? ((__float_as_uint(tn[k]) & ~3u) | uint32_t(k))
+                                                                 : 0xFFFFFFFFu;
   cswap(key[0], key[1]);
   cswap(key[2], key[3]);
   cswap(key[0], key[2]);
   cswap(key[1], key[3]);
   cswap(key[1], key[2]);
-  // Leaves are tested in near-to-far order; the nearest inner child is
-  // visited next and the other inner children are pushed far-first.
-  // (Indices stay compile-time so nothing lands in scratch memory.)
-  uint32_t cj[4];
-  bool inner[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const bool valid = key[j] != 0xFFFFFFFFu;
-    const uint32_t p = pick4(ref[0], ref[1], ref[2], ref[3], key[j] & 3u);
-    const uint32_t n = p & 7u;
-    cj[j] = p >> 3;
-    inner[j] = valid && n == 0;
-    if (valid && n > 0) {
-      // two triangles per round, both fetched before either is tested
-      for (uint32_t s = p >> 3, e = (p >> 3) + n; s < e; s += 2) {
-        const bool two = s + 1 < e;
-        const float4* tp = bvh.tris + 3 * s;
-        float4 a0 = tp[0], a1 = tp[1], a2 = tp[2];
-        float4 b0 = a0, b1 = a1, b2 = a2;
-        if (two) {
-          b0 = tp[3];
-          b1 = tp[4];
-          b2 = tp[5];
-        }
-        if (COUNT) cnt->tris += two ? 2u : 1u;
-        float t;
-        if (tri_hit(c.r, xyz(a0), xyz(a1), xyz(a2), &t)) {
-          uint32_t id = __float_as_uint(a0.w);
-          if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
-            c.tbest = t;
-            c.best_id = id;
-            c.best_slot = s;
-          }
-        }
-        if (two && tri_hit(c.r, xyz(b0), xyz(b1), xyz(b2), &t)) {
-          uint32_t id = __float_as_uint(b0.w);
-          if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
-            c.tbest = t;
-            c.best_id = id;
-            c.best_slot = s + 1;
-          }
-        }
-      }
-    }
-  }
-  int first = inner[0] ? 0 : (inner[1] ? 1 : (inner[2] ? 2 : (inner[3] ? 3 : -1)));
 #pragma unroll
   for (int j = 3; j >= 1; --j)
-    if (inner[j] && j != first) st_push(st, cj[j]);
-  if (first >= 0) {
-    c.node = first == 0 ? cj[0] : (first == 1 ? cj[1] : (first == 2 ? cj[2] : cj[3]));
+    if (key[j] != 0xFFFFFFFFu) st_push(st, pick4(ref[0], ref[1], ref[2], ref[3], key[j] & 3u) >> 3);
+  if (key[0] != 0xFFFFFFFFu) {
+    c.node = pick4(ref[0], ref[1], ref[2], ref[3], key[0] & 3u) >> 3;
     return false;
   }
   if (st_empty(st)) return true;
@@ -331,39 +336,37 @@ AQ_DEV int any_step(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
   uint32_t ref[4];
   node_test4(a.r, bvh.nodes + 4 * a.node, a.tmax, bvh.delta, tn, ref);
   if (COUNT) cnt->nodes++;
+  uint32_t base;
+  uint32_t m = leaf_mask(tn, ref, &base);
+  while (m) {
+    const uint32_t i0 = __builtin_ctz(m);
+    m &= m - 1u;
+    const bool two = m != 0u;
+    const uint32_t i1 = two ? uint32_t(__builtin_ctz(m)) : i0;
+    m &= m - 1u;
+    const float4* ta = bvh.tris + 3 * (base + i0);
+    const float4* tb = bvh.tris + 3 * (base + i1);
+    const float4 a0 = ta[0], a1 = ta[1], a2 = ta[2];
+    const float4 b0 = tb[0], b1 = tb[1], b2 = tb[2];
+    if (COUNT) cnt->tris += two ? 2u : 1u;
+    float t;
+    bool hit = tri_hit(a.r, xyz(a0), xyz(a1), xyz(a2), &t) && t > 0.0f && t < a.tmax;
+    if (!hit && two) hit = tri_hit(a.r, xyz(b0), xyz(b1), xyz(b2), &t) && t > 0.0f && t < a.tmax;
+    if (hit) {
+      // drain the stack so the lane's next ray starts empty
+      st.cnt = 0;
+      st.gsp = 0;
+      return 1;
+    }
+  }
   int next = -1;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    if (tn[k] == __builtin_inff()) continue;
-    const uint32_t n = ref[k] & 7u, ch = ref[k] >> 3;
-    if (n > 0) {
-      // two triangles per round, both fetched before either is tested
-      for (uint32_t s = ch; s < ch + n; s += 2) {
-        const bool two = s + 1 < ch + n;
-        const float4* tp = bvh.tris + 3 * s;
-        float4 a0 = tp[0], a1 = tp[1], a2 = tp[2];
-        float4 b0 = a0, b1 = a1, b2 = a2;
-        if (two) {
-          b0 = tp[3];
-          b1 = tp[4];
-          b2 = tp[5];
-        }
-        if (COUNT) cnt->tris += two ? 2u : 1u;
-        float t;
-        bool hit = tri_hit(a.r, xyz(a0), xyz(a1), xyz(a2), &t) && t > 0.0f && t < a.tmax;
-        if (!hit && two) hit = tri_hit(a.r, xyz(b0), xyz(b1), xyz(b2), &t) && t > 0.0f && t < a.tmax;
-        if (hit) {
-          // drain the stack so the lane's next ray starts empty
-          st.cnt = 0;
-          st.gsp = 0;
-          return 1;
-        }
-      }
-    } else if (next < 0) {
-      next = int(ch);
-    } else {
-      st_push(st, ch);
-    }
+    if (tn[k] == __builtin_inff() || (ref[k] & 7u) != 0u) continue;
+    if (next < 0)
+      next = int(ref[k] >> 3);
+    else
+      st_push(st, ref[k] >> 3);
   }
   if (next >= 0) {
     a.node = uint32_t(next);

@@ -91,7 +91,7 @@ int aqua_scene_bvh(const aqua_scene* s, uint32_t* n_nodes, float* nodes, uint32_
     if (nodes) {
       if (*n_nodes < b4.nodes.size()) return aqua::fail(AQUA_ERR_INVALID_ARG, "node buffer too small");
       memcpy(nodes, b4.nodes.data(), b4.nodes.size() * sizeof(aqua::Bvh4Node));
-      if (order) memcpy(order, b.tri_order.data(), b.tri_order.size() * sizeof(uint32_t));
+      if (order) memcpy(order, b4.tri_order.data(), b4.tri_order.size() * sizeof(uint32_t));
     }
     *n_nodes = uint32_t(b4.nodes.size());
     if (max_depth) *max_depth = b4.max_inner_depth;

@@ -440,6 +440,8 @@ struct Collapse {
     Bvh4Node node;
     memset(&node, 0, sizeof node);
     for (int k = 0; k < 4; ++k) node.child[k] = -1;
+    // leaf children first, into one run of slots, before any inner child's
+    // subtree takes slots of its own
     for (size_t k = 0; k < slots.size(); ++k) {
       const uint32_t e = slots[k];
       const Ref r = ref_of(e);
@@ -450,10 +452,15 @@ struct Collapse {
       node.lo_z[k] = r.lo[2];
       node.hi_z[k] = r.hi[2];
       if (as_leaf[e]) {
-        node.child[k] = int32_t(first[e]);
+        node.child[k] = int32_t(out->tri_order.size());
         node.count[k] = int32_t(n_tris[e]);
-      } else {
-        node.child[k] = int32_t(emit(uint32_t(r.child), node_j[e], depth + 1, out));
+        for (uint32_t i = 0; i < n_tris[e]; ++i) out->tri_order.push_back(bin.tri_order[first[e] + i]);
+      }
+    }
+    for (size_t k = 0; k < slots.size(); ++k) {
+      const uint32_t e = slots[k];
+      if (!as_leaf[e]) {
+        node.child[k] = int32_t(emit(uint32_t(ref_of(e).child), node_j[e], depth + 1, out));
         node.count[k] = 0;
       }
     }
@@ -517,6 +524,7 @@ void collapse_bvh4(const Bvh& bin, Bvh4* out, int max_leaf) {
   *out = Bvh4();
   if (bin.nodes.empty()) return;
   out->nodes.reserve(bin.nodes.size() / 2 + 1);
+  out->tri_order.reserve(bin.tri_order.size());
   Collapse c(bin, max_leaf);
   c.solve();
   // the root is always a 4-wide node: best split of its 4 slots

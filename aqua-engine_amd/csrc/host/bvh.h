@@ -52,6 +52,7 @@ static_assert(sizeof(Bvh4Node) == 128, "node layout");
 
 struct Bvh4 {
   std::vector<Bvh4Node> nodes;  // node 0 is the root, preorder
+  std::vector<uint32_t> tri_order;  // triangle slot -> triangle id (load order)
   uint32_t max_inner_depth = 0; // deepest inner node (root = 0)
 };
 
@@ -72,8 +73,10 @@ void quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out);
 // Collapses a binary tree into 4-wide nodes with the least SAH cost (a
 // dynamic program over the binary tree choosing, for every wide node, which
 // binary descendants become its up to four children and which subtrees of
-// at most max_leaf triangles become single leaves). Triangle slots are
-// unchanged.
+// at most max_leaf triangles become single leaves). Triangle slots are laid
+// out anew: the leaf children of every 4-wide node take one contiguous run of
+// slots, in child order, so traversal can test a node's hit leaves as one
+// bit mask over that run.
 void collapse_bvh4(const Bvh& bin, Bvh4* out, int max_leaf = 4);
 
 }  // namespace aqua

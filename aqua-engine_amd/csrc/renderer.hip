@@ -1214,9 +1214,24 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   if (3 * (bvh4.max_inner_depth + 1) > uint32_t(kLdsStack + kSpillStack))
     return aqua::fail(AQUA_ERR_UNSUPPORTED, "BVH depth %u exceeds the traversal stack",
                       bvh4.max_inner_depth);
+  // Traversal tests a node's hit leaves as a bit mask over one run of at
+  // most 4 x 7 slots starting at its first leaf child.
+  if (bvh4.tri_order.size() != n)
+    return aqua::fail(AQUA_ERR_HIP, "BVH4 covers %zu of %u triangles", bvh4.tri_order.size(), n);
+  for (const aqua::Bvh4Node& nd : bvh4.nodes) {
+    uint32_t lo = 0xFFFFFFFFu, hi = 0, sum = 0;
+    for (int k = 0; k < 4; ++k)
+      if (nd.child[k] >= 0 && nd.count[k] > 0) {
+        lo = std::min(lo, uint32_t(nd.child[k]));
+        hi = std::max(hi, uint32_t(nd.child[k] + nd.count[k]));
+        sum += uint32_t(nd.count[k]);
+      }
+    if (sum && (hi - lo != sum || hi > n))
+      return aqua::fail(AQUA_ERR_HIP, "BVH4 leaf children of a node are not one run of slots");
+  }
   std::vector<float4> tris(3 * size_t(n));
   for (uint32_t s = 0; s < n; ++s) {
-    uint32_t id = bvh.tri_order[s];
+    uint32_t id = bvh4.tri_order[s];
     const float* v = &scene->verts[9 * size_t(id)];
     uint32_t m = scene->mat_ids[id];
     float idf, mf;
@@ -1227,7 +1242,7 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
     tris[3 * size_t(s) + 2] = make_float4(v[6], v[7], v[8], 0.0f);
   }
   r->n_tris = n;
-  r->tri_order = bvh.tri_order;
+  r->tri_order = bvh4.tri_order;
   std::vector<uint32_t> qnodes;
   aqua::quantize_bvh4(bvh4, &qnodes);
   std::vector<float4> nodes(qnodes.size() / 4);

@@ -277,13 +277,17 @@ def test_interior_generator(tmp_path):
 
 def _check_bvh4(nodes, order, verts):
     """Every triangle in exactly one leaf; every leaf triangle and every
-    child node inside its parent's box; nodes in preorder."""
+    child node inside its parent's box; nodes in preorder; the leaf children
+    of a node take one run of slots, in child order."""
     ntri = len(verts)
     assert sorted(order.tolist()) == list(range(ntri))
     idx = nodes[:, 24:28].view(np.int32)
     cnt = nodes[:, 28:32].view(np.int32)
     covered = np.zeros(ntri, int)
     for n in range(len(nodes)):
+        leaves = [(idx[n, k], cnt[n, k]) for k in range(4) if idx[n, k] >= 0 and cnt[n, k] > 0]
+        for (s0, c0), (s1, _) in zip(leaves, leaves[1:]):
+            assert s1 == s0 + c0
         for k in range(4):
             if idx[n, k] < 0:
                 continue
