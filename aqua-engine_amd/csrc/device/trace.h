@@ -79,17 +79,14 @@ AQ_DEV bool tri_hit(const RayPre& r, v3 v0, v3 v1, v3 v2, float* t_out) {
   return true;
 }
 
-// Slab test of one box from its six plane distances along the ray; returns
-// the entry distance, or +inf when culled. The t values are finite (|1/d| is
-// clamped), so the hardware min / max (v_min3 / v_max3) apply. Culling only
-// has to be conservative: the far bound is widened by kFarScale and delta,
-// which also covers the rounding of the plane distances.
-AQ_DEV float slab_enter(float t0x, float t1x, float t0y, float t1y, float t0z, float t1z, float tbest,
-                        float delta) {
-  float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(t0x, t1x), __builtin_fminf(t0y, t1y)),
-                             __builtin_fmaxf(__builtin_fminf(t0z, t1z), 0.0f));
-  float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(t0x, t1x), __builtin_fmaxf(t0y, t1y)),
-                             __builtin_fminf(__builtin_fmaxf(t0z, t1z), tbest));
+// Slab test of one box from its near and far plane distances along the ray
+// per axis; returns the entry distance, or +inf when culled. The t values are
+// finite (|1/d| is clamped), so the hardware min / max (v_min3 / v_max3)
+// apply. Culling only has to be conservative: the far bound is widened by
+// kFarScale and delta, which also covers the rounding of the plane distances.
+AQ_DEV float slab_enter(float nx, float fx, float ny, float fy, float nz, float fz, float tbest, float delta) {
+  float tn = __builtin_fmaxf(__builtin_fmaxf(nx, ny), __builtin_fmaxf(nz, 0.0f));
+  float tf = __builtin_fminf(__builtin_fminf(fx, fy), __builtin_fminf(fz, tbest));
   return tn <= __builtin_fmaf(tf, kFarScale, delta) ? tn : __builtin_inff();
 }
 
@@ -186,11 +183,18 @@ AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float del
   ref[1] = q1.w;
   ref[2] = q2.x;
   ref[3] = q2.y;
+  // Near and far planes per axis from the sign of the direction: the plane
+  // distance fma(q, a, b) is monotone in q, and a has the sign of 1/d, so
+  // this picks exactly min / max of the two distances.
+  const bool fx = r.inv.x < 0.0f, fy = r.inv.y < 0.0f, fz = r.inv.z < 0.0f;
+  const uint32_t nx = fx ? q0.y : q0.x, xx = fx ? q0.x : q0.y;
+  const uint32_t ny = fy ? q0.w : q0.z, xy = fy ? q0.z : q0.w;
+  const uint32_t nz = fz ? q1.y : q1.x, xz = fz ? q1.x : q1.y;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    float t = slab_enter(__builtin_fmaf(ubyte(q0.x, k), ax, bx), __builtin_fmaf(ubyte(q0.y, k), ax, bx),
-                         __builtin_fmaf(ubyte(q0.z, k), ay, by), __builtin_fmaf(ubyte(q0.w, k), ay, by),
-                         __builtin_fmaf(ubyte(q1.x, k), az, bz), __builtin_fmaf(ubyte(q1.y, k), az, bz), tbest,
+    float t = slab_enter(__builtin_fmaf(ubyte(nx, k), ax, bx), __builtin_fmaf(ubyte(xx, k), ax, bx),
+                         __builtin_fmaf(ubyte(ny, k), ay, by), __builtin_fmaf(ubyte(xy, k), ay, by),
+                         __builtin_fmaf(ubyte(nz, k), az, bz), __builtin_fmaf(ubyte(xz, k), az, bz), tbest,
                          delta);
     tn[k] = ref[k] != 0xFFFFFFFFu ? t : __builtin_inff();
   }

@@ -542,6 +542,11 @@ __global__ void k_clear_taux(float4* taux, uint32_t n_slots) {
   if (s < n_slots) taux[2 * size_t(s)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// Shadow rays of bounce b: an unoccluded ray adds its contribution to the
+// radiance of its path (the next bounce's slot, or the pass buffer for a path
+// that ended). Each path has at most one shadow ray per bounce and nothing
+// else writes that radiance meanwhile, so the sum is formed when the ray is
+// taken (its loads overlap the traversal) and only stored when it ends.
 template <bool COUNT>
 __global__ void __launch_bounds__(kBlock, 6) k_shadow(DevBvh bvh, Queues q, uint32_t b) {
   __shared__ uint32_t lds_stack[kLdsStack * kBlock];
@@ -553,14 +558,19 @@ __global__ void __launch_bounds__(kBlock, 6) k_shadow(DevBvh bvh, Queues q, uint
   LaneFeed feed = feed_init(n, q.cursors + cursor_set(b, 1));
   AnyRay ar;
   bool active = false;
-  uint32_t ray = 0;
+  float4* Lp = nullptr;
+  float4 lsum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   while (true) {
     if (feed.more) {
       uint32_t r = feed_take(feed, !active);
       if (r != kMiss) {
         float4 o = q.sho[r], d = q.shd[r];
+        float4 cc = q.shc[r];
         any_begin(ar, xyz(o), xyz(d), o.w);
-        ray = r;
+        const uint32_t dst = __float_as_uint(d.w);
+        Lp = (dst & 0x80000000u) ? &q.Lbuf[dst & 0x7FFFFFFFu] : &q.L[nx][dst];
+        const float4 l = *Lp;
+        lsum = make_float4(l.x + cc.x, l.y + cc.y, l.z + cc.z, 0.0f);
         active = true;
       }
     }
@@ -572,13 +582,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_shadow(DevBvh bvh, Queues q, uint
         int res = any_step<COUNT>(bvh, st, ar, &tc);
         if (res != 0) {
           active = false;
-          if (res == 2) {
-            float4 cc = q.shc[ray];
-            uint32_t dst = __float_as_uint(q.shd[ray].w);
-            float4* Lp = (dst & 0x80000000u) ? &q.Lbuf[dst & 0x7FFFFFFFu] : &q.L[nx][dst];
-            float4 l = *Lp;
-            *Lp = make_float4(l.x + cc.x, l.y + cc.y, l.z + cc.z, 0.0f);
-          }
+          if (res == 2) *Lp = lsum;
         }
       }
       live = uint32_t(__popcll(__ballot(active)));
