@@ -113,23 +113,23 @@ struct TraceCount {
 // traversal of a short queue.
 constexpr int kLdsStack = 16;
 constexpr int kSpillStack = 64;
+// Threads per block of every kernel that traverses (the LDS column stride).
+constexpr int kStackThreads = 256;
 
 // Only scalars live in the struct, so they stay in registers.
 struct TStack {
   uint32_t* lds;    // this thread's column
   uint32_t* spill;  // this thread's first spill entry
-  int stride;       // threads per block
   uint32_t sstride; // threads per grid
   int top, cnt, gsp;
 };
 
-// Thread (blockIdx, threadIdx) of a grid of gridDim x blockDim threads.
+// Thread (blockIdx, threadIdx) of a grid of gridDim x kStackThreads threads.
 AQ_DEV void st_init(TStack& s, uint32_t* lds_base, uint32_t* spill_base) {
   const uint32_t tid = threadIdx.x;
   s.lds = lds_base + tid;
-  s.sstride = gridDim.x * blockDim.x;
-  s.spill = spill_base + size_t(blockIdx.x) * blockDim.x + tid;
-  s.stride = int(blockDim.x);
+  s.sstride = gridDim.x * uint32_t(kStackThreads);
+  s.spill = spill_base + size_t(blockIdx.x) * kStackThreads + tid;
   s.top = kLdsStack - 1;
   s.cnt = 0;
   s.gsp = 0;
@@ -137,16 +137,16 @@ AQ_DEV void st_init(TStack& s, uint32_t* lds_base, uint32_t* spill_base) {
 AQ_DEV void st_push(TStack& s, uint32_t v) {
   int slot = s.top + 1 == kLdsStack ? 0 : s.top + 1;
   if (s.cnt == kLdsStack)
-    s.spill[size_t(s.gsp++) * s.sstride] = s.lds[slot * s.stride];
+    s.spill[size_t(s.gsp++) * s.sstride] = s.lds[slot * kStackThreads];
   else
     ++s.cnt;
-  s.lds[slot * s.stride] = v;
+  s.lds[slot * kStackThreads] = v;
   s.top = slot;
 }
 AQ_DEV bool st_empty(const TStack& s) { return s.cnt == 0 && s.gsp == 0; }
 AQ_DEV uint32_t st_pop(TStack& s) {
   if (s.cnt > 0) {
-    uint32_t v = s.lds[s.top * s.stride];
+    uint32_t v = s.lds[s.top * kStackThreads];
     s.top = s.top == 0 ? kLdsStack - 1 : s.top - 1;
     --s.cnt;
     return v;
@@ -167,13 +167,14 @@ AQ_DEV float ubyte(uint32_t w, int k) { return float((w >> (8 * k)) & 0xFFu); }
 // together, straight-line code). A child plane is origin + q * 2^e, so its
 // distance along the ray is q * (2^e / d) + (origin - o) / d: one FMA per
 // plane after six per-node products. Empty children get +inf. ref[k] is
-// the packed child reference (index << 3) | triangle count.
+// the packed child reference (index << 3) | triangle count; leaf[] the
+// node's leaf run (first slot, per-child offset | count bytes).
 AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float delta, float tn[4],
-                       uint32_t ref[4]) {
+                       uint32_t ref[4], uint32_t leaf[2]) {
   const float4 h = np[0];
   const uint4 q0 = *reinterpret_cast<const uint4*>(np + 1);  // lo.x hi.x lo.y hi.y
   const uint4 q1 = *reinterpret_cast<const uint4*>(np + 2);  // lo.z hi.z ref0 ref1
-  const uint4 q2 = *reinterpret_cast<const uint4*>(np + 3);  // ref2 ref3 - -
+  const uint4 q2 = *reinterpret_cast<const uint4*>(np + 3);  // ref2 ref3 leaf run
   const uint32_t e = __float_as_uint(h.w);
   const float sx = __uint_as_float((e & 0xFFu) << 23), sy = __uint_as_float(((e >> 8) & 0xFFu) << 23),
               sz = __uint_as_float(((e >> 16) & 0xFFu) << 23);
@@ -183,6 +184,8 @@ AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float del
   ref[1] = q1.w;
   ref[2] = q2.x;
   ref[3] = q2.y;
+  leaf[0] = q2.z;
+  leaf[1] = q2.w;
   // Near and far planes per axis from the sign of the direction: the plane
   // distance fma(q, a, b) is monotone in q, and a has the sign of 1/d, so
   // this picks exactly min / max of the two distances.
@@ -207,18 +210,16 @@ AQ_DEV void cswap(uint32_t& a, uint32_t& b) {
 }
 
 // The hit leaf children of a node as a bit mask over the node's run of leaf
-// slots: the builder gives every node's leaf children one contiguous run (at
-// most 4 x 7 slots), so bit i is slot *base + i.
-AQ_DEV uint32_t leaf_mask(const float tn[4], const uint32_t ref[4], uint32_t* base) {
-  uint32_t b = 0xFFFFFFFFu;
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (tn[k] != __builtin_inff() && (ref[k] & 7u) != 0u) b = min(b, ref[k] >> 3);
+// slots (bit i = slot leaf[0] + i): leaf[1] holds one byte per child,
+// (offset in the run << 3) | triangle count, 0 for inner and empty children.
+AQ_DEV uint32_t leaf_mask(const float tn[4], uint32_t bits) {
   uint32_t m = 0u;
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (tn[k] != __builtin_inff() && (ref[k] & 7u) != 0u) m |= ((1u << (ref[k] & 7u)) - 1u) << ((ref[k] >> 3) - b);
-  *base = b;
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t b = (bits >> (8 * k)) & 0xFFu;
+    const uint32_t mk = ((1u << (b & 7u)) - 1u) << (b >> 3);
+    m |= tn[k] != __builtin_inff() ? mk : 0u;
+  }
   return m;
 }
 
@@ -248,12 +249,13 @@ template <bool COUNT>
 AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCount* cnt) {
   float tn[4];
   uint32_t ref[4];
-  node_test4(c.r, bvh.nodes + 4 * c.node, c.tbest, bvh.delta, tn, ref);
+  uint32_t leaf[2];
+  node_test4(c.r, bvh.nodes + 4 * c.node, c.tbest, bvh.delta, tn, ref, leaf);
   if (COUNT) cnt->nodes++;
   // Hit leaves: every triangle of them, two per round, in one loop (the
   // result does not depend on their order).
-  uint32_t base;
-  uint32_t m = leaf_mask(tn, ref, &base);
+  const uint32_t base = leaf[0];
+  uint32_t m = leaf_mask(tn, leaf[1]);
   while (m) {
     const uint32_t i0 = __builtin_ctz(m);
     m &= m - 1u;
@@ -338,10 +340,11 @@ template <bool COUNT>
 AQ_DEV int any_step(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
   float tn[4];
   uint32_t ref[4];
-  node_test4(a.r, bvh.nodes + 4 * a.node, a.tmax, bvh.delta, tn, ref);
+  uint32_t leaf[2];
+  node_test4(a.r, bvh.nodes + 4 * a.node, a.tmax, bvh.delta, tn, ref, leaf);
   if (COUNT) cnt->nodes++;
-  uint32_t base;
-  uint32_t m = leaf_mask(tn, ref, &base);
+  const uint32_t base = leaf[0];
+  uint32_t m = leaf_mask(tn, leaf[1]);
   while (m) {
     const uint32_t i0 = __builtin_ctz(m);
     m &= m - 1u;

@@ -110,7 +110,7 @@ int aqua_scene_bvh_device(const aqua_scene* s, uint32_t* n_nodes, uint32_t* word
     if (words) {
       if (*n_nodes < b4.nodes.size()) return aqua::fail(AQUA_ERR_INVALID_ARG, "node buffer too small");
       std::vector<uint32_t> q;
-      aqua::quantize_bvh4(b4, &q);
+      if (!aqua::quantize_bvh4(b4, &q)) return aqua::fail(AQUA_ERR_UNSUPPORTED, "BVH4 leaf runs do not fit the node format");
       memcpy(words, q.data(), q.size() * sizeof(uint32_t));
     }
     *n_nodes = uint32_t(b4.nodes.size());

@@ -471,7 +471,7 @@ struct Collapse {
 
 }  // namespace
 
-void quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out) {
+bool quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out) {
   out->assign(size_t(kNodeWords) * in.nodes.size(), 0u);
   for (size_t i = 0; i < in.nodes.size(); ++i) {
     const Bvh4Node& n = in.nodes[i];
@@ -517,7 +517,22 @@ void quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out) {
       }
       w[10 + k] = (uint32_t(n.child[k]) << 3) | uint32_t(n.count[k]);
     }
+    // the leaf run: collapse_bvh4 gives a node's leaf children consecutive
+    // slots in child order, at most 4 x 7 of them
+    uint32_t base = 0xFFFFFFFFu;
+    for (int k = 0; k < 4; ++k)
+      if (n.child[k] >= 0 && n.count[k] > 0) base = std::min(base, uint32_t(n.child[k]));
+    if (base != 0xFFFFFFFFu) {
+      w[14] = base;
+      for (int k = 0; k < 4; ++k)
+        if (n.child[k] >= 0 && n.count[k] > 0) {
+          const uint32_t off = uint32_t(n.child[k]) - base;
+          if (n.count[k] > 7 || off + uint32_t(n.count[k]) > 32) return false;
+          w[15] |= ((off << 3) | uint32_t(n.count[k])) << (8 * k);
+        }
+    }
   }
+  return true;
 }
 
 void collapse_bvh4(const Bvh& bin, Bvh4* out, int max_leaf) {

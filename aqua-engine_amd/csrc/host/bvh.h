@@ -64,11 +64,15 @@ struct Bvh4 {
 //          lo.z, hi.z, each rounded outward (lo down, hi up)
 //   w10..13 child references (index << 3) | triangle count (count 0 = inner
 //          node, count <= 7); 0xFFFFFFFF = no child
-//   w14..15 zero
+//   w14    first triangle slot of the node's leaf children (one run of
+//          slots; 0 when there are none)
+//   w15    byte k = (offset of child k's triangles in that run << 3) | count
+//          for a leaf child, 0 for an inner or empty child
 // Decoded boxes enclose the original ones, so traversal stays conservative.
 constexpr uint32_t kNodeWords = 16;
 constexpr uint32_t kNoChild = 0xFFFFFFFFu;
-void quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out);
+// false when a node's leaf children do not form one run that fits w15.
+bool quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out);
 
 // Collapses a binary tree into 4-wide nodes with the least SAH cost (a
 // dynamic program over the binary tree choosing, for every wide node, which

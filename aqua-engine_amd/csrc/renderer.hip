@@ -23,6 +23,7 @@ using namespace aqd;
 namespace {
 
 constexpr int kBlock = 256;
+static_assert(kBlock == aqd::kStackThreads, "traversal kernels run kStackThreads threads per block");
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -1248,7 +1249,8 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   r->n_tris = n;
   r->tri_order = bvh4.tri_order;
   std::vector<uint32_t> qnodes;
-  aqua::quantize_bvh4(bvh4, &qnodes);
+  if (!aqua::quantize_bvh4(bvh4, &qnodes))
+    return aqua::fail(AQUA_ERR_UNSUPPORTED, "BVH4 leaf runs do not fit the node format");
   std::vector<float4> nodes(qnodes.size() / 4);
   memcpy(nodes.data(), qnodes.data(), qnodes.size() * sizeof(uint32_t));
   // Culling slack: 2^-16 of the largest coordinate magnitude.

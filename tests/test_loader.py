@@ -342,7 +342,7 @@ def test_device_nodes_enclose_the_exact_boxes(box_room, tmp_path):
     for sc in (box_room, aqua.Scene.load(p)):
         nodes, _, _ = sc.bvh()
         words = sc.bvh_device()
-        assert words.shape == (len(nodes), 16) and not np.any(words[:, 14:16])
+        assert words.shape == (len(nodes), 16)
         lo, hi, ref = _decode_device_nodes(words)
         idx = nodes[:, 24:28].view(np.int32)
         cnt = nodes[:, 28:32].view(np.int32)
@@ -350,6 +350,16 @@ def test_device_nodes_enclose_the_exact_boxes(box_room, tmp_path):
         np.testing.assert_array_equal(ref == 0xFFFFFFFF, ~used)
         np.testing.assert_array_equal(ref[used] >> 3, idx[used].astype(np.uint32))
         np.testing.assert_array_equal(ref[used] & 7, cnt[used].astype(np.uint32))
+        # the leaf run: word 14 = first leaf slot, word 15 = per child
+        # (offset << 3) | count for leaves, 0 otherwise
+        leaf = used & (cnt > 0)
+        run = np.where(leaf, idx, np.iinfo(np.int32).max).min(1)
+        has = leaf.any(1)
+        np.testing.assert_array_equal(words[has, 14], run[has].astype(np.uint32))
+        np.testing.assert_array_equal(words[~has, 14:16], 0)
+        byte = np.stack([(words[:, 15] >> (8 * k)) & 0xFF for k in range(4)], 1)
+        want = np.where(leaf, ((idx - run[:, None]) << 3) | cnt, 0)
+        np.testing.assert_array_equal(byte, want.astype(np.uint32))
         exact_lo = np.stack([nodes[:, 0:4], nodes[:, 8:12], nodes[:, 16:20]], 2).astype(np.float64)
         exact_hi = np.stack([nodes[:, 4:8], nodes[:, 12:16], nodes[:, 20:24]], 2).astype(np.float64)
         assert np.all(lo[used] <= exact_lo[used]) and np.all(hi[used] >= exact_hi[used])
