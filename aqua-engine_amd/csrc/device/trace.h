@@ -223,14 +223,27 @@ AQ_DEV uint32_t leaf_mask(const float tn[4], uint32_t bits) {
   return m;
 }
 
-// ---- resumable traversal: the state of one ray between node visits, so a
+// ---- resumable traversal: the state of one ray between steps, so a
 // persistent wave can give a lane a new ray as soon as its last one ends
-// instead of idling until the wave's slowest ray is done.
+// instead of idling until the wave's slowest ray is done. A step is either
+// one node visit or one round of (up to) two leaf triangles: a node visit
+// only records the hit leaves' triangles as a bit mask, and the wave tests
+// them in rounds of its own once the lanes holding such triangles outnumber
+// a third of the lanes ready to visit a node, so a leaf round serves many
+// lanes instead of the few whose last node had leaves. Neither answer
+// depends on the order of triangle tests and node visits: the closest hit
+// is the least (t, triangle id) over every triangle the watertight test
+// accepts in a box the conservative culling keeps, and deferring a leaf
+// only culls less.
+
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
 
 struct ClosestRay {
   RayPre r;
   float tmax, tbest;
-  uint32_t best_id, best_slot, node;
+  uint32_t best_id, best_slot;
+  uint32_t node;          // next node to visit, kNoNode when none is left
+  uint32_t lbase, lmask;  // leaf triangles still to test: slots lbase + set bits
 };
 
 AQ_DEV void closest_begin(ClosestRay& c, v3 o, v3 d, float tmax) {
@@ -240,54 +253,28 @@ AQ_DEV void closest_begin(ClosestRay& c, v3 o, v3 d, float tmax) {
   c.best_id = 0xFFFFFFFFu;
   c.best_slot = 0xFFFFFFFFu;
   c.node = 0;
+  c.lbase = 0;
+  c.lmask = 0;
 }
 
-// One node visit of a closest-hit ray; true once the ray is finished (its
-// result is (best_slot, tbest); best_slot 0xFFFFFFFF = miss). The stack is
-// empty again when it returns true.
+// The ray is finished: its result is (best_slot, tbest), best_slot
+// 0xFFFFFFFF = miss, and its stack is empty.
+AQ_DEV bool closest_done(const ClosestRay& c) { return c.node == kNoNode && c.lmask == 0u; }
+
+// One node visit (the ray has no leaf triangles pending): records the hit
+// leaves, pushes the other hit inner children far-first and moves to the
+// nearest one (or the top of the stack).
 template <bool COUNT>
-AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCount* cnt) {
+AQ_DEV void closest_node(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCount* cnt) {
   float tn[4];
   uint32_t ref[4];
   uint32_t leaf[2];
   node_test4(c.r, bvh.nodes + 4 * c.node, c.tbest, bvh.delta, tn, ref, leaf);
   if (COUNT) cnt->nodes++;
-  // Hit leaves: every triangle of them, two per round, in one loop (the
-  // result does not depend on their order).
-  const uint32_t base = leaf[0];
-  uint32_t m = leaf_mask(tn, leaf[1]);
-  while (m) {
-    const uint32_t i0 = __builtin_ctz(m);
-    m &= m - 1u;
-    const bool two = m != 0u;
-    const uint32_t i1 = two ? uint32_t(__builtin_ctz(m)) : i0;
-    m &= m - 1u;
-    const float4* ta = bvh.tris + 3 * (base + i0);
-    const float4* tb = bvh.tris + 3 * (base + i1);
-    const float4 a0 = ta[0], a1 = ta[1], a2 = ta[2];
-    const float4 b0 = tb[0], b1 = tb[1], b2 = tb[2];
-    if (COUNT) cnt->tris += two ? 2u : 1u;
-    float t;
-    if (tri_hit(c.r, xyz(a0), xyz(a1), xyz(a2), &t)) {
-      uint32_t id = __float_as_uint(a0.w);
-      if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
-        c.tbest = t;
-        c.best_id = id;
-        c.best_slot = base + i0;
-      }
-    }
-    if (two && tri_hit(c.r, xyz(b0), xyz(b1), xyz(b2), &t)) {
-      uint32_t id = __float_as_uint(b0.w);
-      if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
-        c.tbest = t;
-        c.best_id = id;
-        c.best_slot = base + i1;
-      }
-    }
-  }
-  // Hit inner children near to far: key = entry distance bits with the
-  // child index in the low two bits (entry distances are >= 0); the nearest
-  // is visited next, the others are pushed far-first.
+  c.lbase = leaf[0];
+  c.lmask = leaf_mask(tn, leaf[1]);
+  // key = entry distance bits with the child index in the low two bits
+  // (entry distances are >= 0)
   uint32_t key[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
@@ -301,98 +288,152 @@ AQ_DEV bool closest_step(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCoun
 #pragma unroll
   for (int j = 3; j >= 1; --j)
     if (key[j] != 0xFFFFFFFFu) st_push(st, pick4(ref[0], ref[1], ref[2], ref[3], key[j] & 3u) >> 3);
-  if (key[0] != 0xFFFFFFFFu) {
+  if (key[0] != 0xFFFFFFFFu)
     c.node = pick4(ref[0], ref[1], ref[2], ref[3], key[0] & 3u) >> 3;
-    return false;
-  }
-  if (st_empty(st)) return true;
-  c.node = st_pop(st);
-  return false;
+  else
+    c.node = st_empty(st) ? kNoNode : st_pop(st);
 }
 
-// Closest hit. Returns the triangle slot (0xFFFFFFFF on a miss) and t.
+// The next two of the ray's pending leaf triangles, both fetched before
+// either is tested.
 template <bool COUNT>
-AQ_DEV uint32_t trace_closest(const DevBvh& bvh, TStack& st, v3 o, v3 d, float tmax, float* t_out,
-                              TraceCount* cnt) {
-  ClosestRay c;
-  closest_begin(c, o, d, tmax);
-  while (!closest_step<COUNT>(bvh, st, c, cnt)) {
+AQ_DEV void closest_leaf(const DevBvh& bvh, ClosestRay& c, TraceCount* cnt) {
+  uint32_t m = c.lmask;
+  const uint32_t i0 = __builtin_ctz(m);
+  m &= m - 1u;
+  const bool two = m != 0u;
+  const uint32_t i1 = two ? uint32_t(__builtin_ctz(m)) : i0;
+  c.lmask = m & (m - 1u);
+  const float4* ta = bvh.tris + 3 * (c.lbase + i0);
+  const float4* tb = bvh.tris + 3 * (c.lbase + i1);
+  const float4 a0 = ta[0], a1 = ta[1], a2 = ta[2];
+  const float4 b0 = tb[0], b1 = tb[1], b2 = tb[2];
+  if (COUNT) cnt->tris += two ? 2u : 1u;
+  float t;
+  if (tri_hit(c.r, xyz(a0), xyz(a1), xyz(a2), &t)) {
+    uint32_t id = __float_as_uint(a0.w);
+    if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
+      c.tbest = t;
+      c.best_id = id;
+      c.best_slot = c.lbase + i0;
+    }
   }
-  *t_out = c.tbest;
-  return c.best_slot;
+  if (two && tri_hit(c.r, xyz(b0), xyz(b1), xyz(b2), &t)) {
+    uint32_t id = __float_as_uint(b0.w);
+    if (t > 0.0f && t < c.tmax && (t < c.tbest || (t == c.tbest && id < c.best_id))) {
+      c.tbest = t;
+      c.best_id = id;
+      c.best_slot = c.lbase + i1;
+    }
+  }
+}
+
+// One step of a wave (every lane calls it together; inactive lanes hold no
+// ray): a leaf round once the lanes with leaf triangles pending (which wait
+// meanwhile) number more than a third of those that can visit a node, else
+// a node visit.
+template <bool COUNT>
+AQ_DEV void closest_wave_step(const DevBvh& bvh, TStack& st, ClosestRay& c, bool active, TraceCount* cnt) {
+  const bool wl = active && c.lmask != 0u;
+  const bool wn = active && !wl && c.node != kNoNode;
+  if (3 * __popcll(__ballot(wl)) > __popcll(__ballot(wn))) {
+    if (wl) closest_leaf<COUNT>(bvh, c, cnt);
+  } else if (wn) {
+    closest_node<COUNT>(bvh, st, c, cnt);
+  }
 }
 
 struct AnyRay {
   RayPre r;
   float tmax;
-  uint32_t node;
+  uint32_t node;          // next node to visit, kNoNode when none is left
+  uint32_t lbase, lmask;  // leaf triangles still to test
+  bool occluded;
 };
 
 AQ_DEV void any_begin(AnyRay& a, v3 o, v3 d, float tmax) {
   a.r = ray_pre(o, d);
   a.tmax = tmax;
   a.node = 0;
+  a.lbase = 0;
+  a.lmask = 0;
+  a.occluded = false;
 }
 
-// One node visit of an any-hit ray in (0, tmax): 0 = not finished,
-// 1 = occluded, 2 = unoccluded. The stack is empty once it returns non-zero.
+// Finished: the answer is a.occluded, and the stack is empty.
+AQ_DEV bool any_done(const AnyRay& a) { return a.node == kNoNode && a.lmask == 0u; }
+
 template <bool COUNT>
-AQ_DEV int any_step(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
+AQ_DEV void any_node(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
   float tn[4];
   uint32_t ref[4];
   uint32_t leaf[2];
   node_test4(a.r, bvh.nodes + 4 * a.node, a.tmax, bvh.delta, tn, ref, leaf);
   if (COUNT) cnt->nodes++;
-  const uint32_t base = leaf[0];
-  uint32_t m = leaf_mask(tn, leaf[1]);
-  while (m) {
-    const uint32_t i0 = __builtin_ctz(m);
-    m &= m - 1u;
-    const bool two = m != 0u;
-    const uint32_t i1 = two ? uint32_t(__builtin_ctz(m)) : i0;
-    m &= m - 1u;
-    const float4* ta = bvh.tris + 3 * (base + i0);
-    const float4* tb = bvh.tris + 3 * (base + i1);
-    const float4 a0 = ta[0], a1 = ta[1], a2 = ta[2];
-    const float4 b0 = tb[0], b1 = tb[1], b2 = tb[2];
-    if (COUNT) cnt->tris += two ? 2u : 1u;
-    float t;
-    bool hit = tri_hit(a.r, xyz(a0), xyz(a1), xyz(a2), &t) && t > 0.0f && t < a.tmax;
-    if (!hit && two) hit = tri_hit(a.r, xyz(b0), xyz(b1), xyz(b2), &t) && t > 0.0f && t < a.tmax;
-    if (hit) {
-      // drain the stack so the lane's next ray starts empty
-      st.cnt = 0;
-      st.gsp = 0;
-      return 1;
-    }
-  }
-  int next = -1;
+  a.lbase = leaf[0];
+  a.lmask = leaf_mask(tn, leaf[1]);
+  uint32_t next = kNoNode;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (tn[k] == __builtin_inff() || (ref[k] & 7u) != 0u) continue;
-    if (next < 0)
-      next = int(ref[k] >> 3);
+    if (next == kNoNode)
+      next = ref[k] >> 3;
     else
       st_push(st, ref[k] >> 3);
   }
-  if (next >= 0) {
-    a.node = uint32_t(next);
-    return 0;
-  }
-  if (st_empty(st)) return 2;
-  a.node = st_pop(st);
-  return 0;
+  a.node = next != kNoNode ? next : (st_empty(st) ? kNoNode : st_pop(st));
 }
 
-// Any hit in (0, tmax).
+// Any hit in (0, tmax) among the next two pending leaf triangles ends the
+// ray (and empties its stack).
+template <bool COUNT>
+AQ_DEV void any_leaf(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
+  uint32_t m = a.lmask;
+  const uint32_t i0 = __builtin_ctz(m);
+  m &= m - 1u;
+  const bool two = m != 0u;
+  const uint32_t i1 = two ? uint32_t(__builtin_ctz(m)) : i0;
+  a.lmask = m & (m - 1u);
+  const float4* ta = bvh.tris + 3 * (a.lbase + i0);
+  const float4* tb = bvh.tris + 3 * (a.lbase + i1);
+  const float4 a0 = ta[0], a1 = ta[1], a2 = ta[2];
+  const float4 b0 = tb[0], b1 = tb[1], b2 = tb[2];
+  if (COUNT) cnt->tris += two ? 2u : 1u;
+  float t;
+  bool hit = tri_hit(a.r, xyz(a0), xyz(a1), xyz(a2), &t) && t > 0.0f && t < a.tmax;
+  if (!hit && two) hit = tri_hit(a.r, xyz(b0), xyz(b1), xyz(b2), &t) && t > 0.0f && t < a.tmax;
+  if (hit) {
+    a.occluded = true;
+    a.lmask = 0u;
+    a.node = kNoNode;
+    st.cnt = 0;
+    st.gsp = 0;
+  }
+}
+
+template <bool COUNT>
+AQ_DEV void any_wave_step(const DevBvh& bvh, TStack& st, AnyRay& a, bool active, TraceCount* cnt) {
+  const bool wl = active && a.lmask != 0u;
+  const bool wn = active && !wl && a.node != kNoNode;
+  if (3 * __popcll(__ballot(wl)) > __popcll(__ballot(wn))) {
+    if (wl) any_leaf<COUNT>(bvh, st, a, cnt);
+  } else if (wn) {
+    any_node<COUNT>(bvh, st, a, cnt);
+  }
+}
+
+// Any hit in (0, tmax), one ray per thread.
 template <bool COUNT>
 AQ_DEV bool trace_any(const DevBvh& bvh, TStack& st, v3 o, v3 d, float tmax, TraceCount* cnt) {
   AnyRay a;
   any_begin(a, o, d, tmax);
-  int r;
-  while ((r = any_step<COUNT>(bvh, st, a, cnt)) == 0) {
+  while (!any_done(a)) {
+    if (a.lmask != 0u)
+      any_leaf<COUNT>(bvh, st, a, cnt);
+    else
+      any_node<COUNT>(bvh, st, a, cnt);
   }
-  return r == 1;
+  return a.occluded;
 }
 
 }  // namespace aqd

@@ -266,7 +266,8 @@ __global__ void __launch_bounds__(kBlock, 6) k_intersect(DevBvh bvh, const float
     uint32_t live = uint32_t(__popcll(__ballot(active)));
     if (live == 0) break;
     while (live > keep) {
-      if (active && closest_step<COUNT>(bvh, st, cr, &tc)) {
+      closest_wave_step<COUNT>(bvh, st, cr, active, &tc);
+      if (active && closest_done(cr)) {
         hit[ray] = make_uint2(cr.best_slot, __float_as_uint(cr.tbest));
         active = false;
       }
@@ -579,12 +580,10 @@ __global__ void __launch_bounds__(kBlock, 6) k_shadow(DevBvh bvh, Queues q, uint
     uint32_t live = uint32_t(__popcll(__ballot(active)));
     if (live == 0) break;
     while (live > keep) {
-      if (active) {
-        int res = any_step<COUNT>(bvh, st, ar, &tc);
-        if (res != 0) {
-          active = false;
-          if (res == 2) *Lp = lsum;
-        }
+      any_wave_step<COUNT>(bvh, st, ar, active, &tc);
+      if (active && any_done(ar)) {
+        if (!ar.occluded) *Lp = lsum;
+        active = false;
       }
       live = uint32_t(__popcll(__ballot(active)));
     }
