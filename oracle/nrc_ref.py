@@ -8,8 +8,11 @@ so this restates the published methods the GPU kernels implement:
 * multiresolution hash encoding -- Mueller, Evans, Schied, Keller 2022,
   "Instant Neural Graphics Primitives": L levels of F features, resolution
   N_l = floor(N_min * b^l), trilinear interpolation of the 8 cell corners,
-  dense indexing when (N_l + 1)^3 <= T, otherwise the spatial hash
-  (x * 1) ^ (y * 2654435761) ^ (z * 805459861) mod T;
+  dense indexing when (N_l + 1)^3 <= T, otherwise the spatial hash with
+  its primes applied to 4 x 4 x 2 blocks of vertices, the vertex's place in
+  its block in the low 5 bits:
+  ((((x >> 2) * 1) ^ ((y >> 2) * 2654435761) ^ ((z >> 1) * 805459861)) << 5
+   | (x & 3) | (y & 3) << 2 | (z & 1) << 4) mod T;
 * real spherical harmonics up to degree 4 (16 coefficients, l = 0..3) of the
   direction (the standard graphics table, e.g. Sloan 2008);
 * an MLP of 4 hidden ReLU layers of width 64, no biases (a constant-1 input
@@ -61,7 +64,11 @@ def corner_index(l: int, c: np.ndarray, log2_table=LOG2_TABLE) -> np.ndarray:
         n = level_res(l) + 1
         return (c[..., 0] + n * (c[..., 1] + n * c[..., 2])).astype(np.int64)
     c = c.astype(np.uint64)
-    h = (c[..., 0] * np.uint64(PRIMES[0])) ^ (c[..., 1] * np.uint64(PRIMES[1])) ^ (c[..., 2] * np.uint64(PRIMES[2]))
+    x, y, z = c[..., 0], c[..., 1], c[..., 2]
+    b = ((x >> np.uint64(2)) * np.uint64(PRIMES[0])) ^ ((y >> np.uint64(2)) * np.uint64(PRIMES[1])) ^ \
+        ((z >> np.uint64(1)) * np.uint64(PRIMES[2]))
+    h = ((b & np.uint64(0xFFFFFFFF)) << np.uint64(5)) | (x & np.uint64(3)) | ((y & np.uint64(3)) << np.uint64(2)) | \
+        ((z & np.uint64(1)) << np.uint64(4))
     return (h & np.uint64(0xFFFFFFFF) & np.uint64(T - 1)).astype(np.int64)
 
 

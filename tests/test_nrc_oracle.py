@@ -61,6 +61,22 @@ def test_hash_level_indices_in_range():
         assert idx.min() >= 0 and idx.max() < (1 << 12)
 
 
+def test_hash_levels_keep_vertex_blocks_together():
+    # hashed levels: the 32 vertices of a 4 x 4 x 2 block take one aligned
+    # 32-entry run, in x-fastest order; different blocks hash apart
+    l = next(l for l in range(ref.N_LEVELS) if not ref.level_is_dense(l))
+    rng = np.random.default_rng(3)
+    base = rng.integers(0, ref.level_res(l) // 4, (200, 3)) * np.array([4, 4, 2])
+    off = np.stack(np.meshgrid(np.arange(4), np.arange(4), np.arange(2), indexing="ij"), -1).reshape(-1, 3)
+    idx = ref.corner_index(l, base[:, None, :] + off[None, :, :])
+    assert np.all(idx // 32 == idx[:, :1] // 32)
+    np.testing.assert_array_equal(idx % 32, np.broadcast_to(off[:, 0] + 4 * off[:, 1] + 16 * off[:, 2], idx.shape))
+    # the block part is the spatial hash of the block coordinates
+    bc = (base // np.array([4, 4, 2])).astype(np.uint64)
+    h = (bc[:, 0] * np.uint64(1)) ^ (bc[:, 1] * np.uint64(2654435761)) ^ (bc[:, 2] * np.uint64(805459861))
+    np.testing.assert_array_equal(idx[:, 0] // 32, ((h & np.uint64(0xFFFFFFFF)) % np.uint64(1 << 14)).astype(np.int64))
+
+
 def test_trilinear_reproduces_linear_field():
     # On a dense level, table values that are linear in the vertex position
     # interpolate to the same linear function of x.
