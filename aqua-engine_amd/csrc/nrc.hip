@@ -34,6 +34,10 @@
 namespace {
 
 constexpr int kLevels = 16;
+// Levels kFineLevel0.. accumulate their table gradient as FP16 pairs (one
+// packed atomic per entry instead of two FP32 ones, as tiny-cuda-nn does);
+// the coarser levels, whose entries collect many rows, stay FP32.
+constexpr int kFineLevel0 = 8;
 constexpr int kHidden = 4;
 constexpr int kMlp = 4 * 64 * 64 + 64 * 16;    // 17408 floats
 static_assert(kMlp % 64 == 0, "partial sums run 64 weights per block");
@@ -54,6 +58,7 @@ inline uint32_t max_chunks(uint32_t npad) { return std::max(257u, (npad + 4095u)
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 
 struct Grid {
   float res[kLevels];
@@ -213,7 +218,8 @@ struct FwdArgs {
   const float* ascale;      // FP8: per-layer activation scales (device)
   float* out;               // 3 per row (inference)
   // training
-  float* table_grad;        // [level][T][2] FP32
+  float* table_grad;        // [level][T][2] FP32 (levels < kFineLevel0)
+  uint32_t* table_grad16;   // [level - kFineLevel0][T] two FP16
   uint16_t* HT;             // 5 x [npad][64] layer inputs
   uint16_t* DT;             // 5 x [npad][64] pre-activation grads (last: 16 used)
   float* loss_acc;
@@ -234,7 +240,7 @@ __device__ __forceinline__ float wave_sum(float v) {
 // extras 4g..4g+3).
 __device__ __forceinline__ void encode16(const FwdArgs& a, uint32_t T_, int g, bool valid, float px, float py,
                                          float pz, float4 r0, float4 r1, float4 r2, float4 r3, float f[16]) {
-#pragma unroll 2
+#pragma unroll 1
   for (int q = 0; q < 4; ++q) {
     int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
     Cell c = cell_of(a.G, lv, px, py, pz);
@@ -506,13 +512,14 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
           }
         } else {
           if (d0 == 0.0f && d1 == 0.0f) continue;
+          // levels 8..15: one packed FP16 atomic per corner
+          h2v* gl = reinterpret_cast<h2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_;
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
             uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
             float w = corner_w(c, k);
-            float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
-            atomicAdd(gp, w * d0);
-            atomicAdd(gp + 1, w * d1);
+            h2v v = {(_Float16)(w * d0), (_Float16)(w * d1)};
+            __builtin_amdgcn_global_atomic_fadd_v2f16(gl + idx, v);
           }
         }
       }
@@ -637,12 +644,22 @@ __global__ void __launch_bounds__(256) k_nrc_reduce_partials(const float* partia
   if (threadIdx.x < 64) out[p] = g * gscale;
 }
 
-__global__ void k_nrc_adam_table(float* master, float* grad, float* m, float* v, uint32_t* table16, size_t n_entries,
-                                 AdamArgs A) {
+// Entries below `fine0` take their gradient from the FP32 pairs, the rest
+// from the FP16 pairs of grad16; both are cleared for the next step.
+__global__ void k_nrc_adam_table(float* master, float* grad, uint32_t* grad16, size_t fine0, float* m, float* v,
+                                 uint32_t* table16, size_t n_entries, AdamArgs A) {
   size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n_entries) return;
   float2 p = reinterpret_cast<float2*>(master)[e];
-  float2 gr = reinterpret_cast<float2*>(grad)[e];
+  float2 gr;
+  if (e < fine0) {
+    gr = reinterpret_cast<float2*>(grad)[e];
+    reinterpret_cast<float2*>(grad)[e] = make_float2(0.0f, 0.0f);
+  } else {
+    const h2v h = reinterpret_cast<const h2v*>(grad16)[e - fine0];
+    gr = make_float2(float(h.x), float(h.y));
+    grad16[e - fine0] = 0u;
+  }
   float2 mm = reinterpret_cast<float2*>(m)[e];
   float2 vv = reinterpret_cast<float2*>(v)[e];
   p.x = adam_one(p.x, gr.x * A.gscale, &mm.x, &vv.x, A);
@@ -650,7 +667,6 @@ __global__ void k_nrc_adam_table(float* master, float* grad, float* m, float* v,
   reinterpret_cast<float2*>(master)[e] = p;
   reinterpret_cast<float2*>(m)[e] = mm;
   reinterpret_cast<float2*>(v)[e] = vv;
-  reinterpret_cast<float2*>(grad)[e] = make_float2(0.0f, 0.0f);
   uint16_t lo = __builtin_bit_cast(uint16_t, (_Float16)p.x), hi = __builtin_bit_cast(uint16_t, (_Float16)p.y);
   table16[e] = uint32_t(lo) | (uint32_t(hi) << 16);
 }
@@ -743,6 +759,16 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// The FP16 gradient pairs of the fine levels as FP32 pairs (for reading the
+// gradient back); clears them.
+__global__ void k_grad16_to_f32(uint32_t* g16, size_t n, float* out) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const h2v h = reinterpret_cast<const h2v*>(g16)[i];
+  reinterpret_cast<float2*>(out)[i] = make_float2(float(h.x), float(h.y));
+  g16[i] = 0u;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -798,9 +824,11 @@ struct aqua_nrc {
   aqua_nrc_cfg cfg{};
   Grid grid{};
   size_t entries = 0;  // levels * T
+  size_t fine_entries() const { return entries / kLevels * (kLevels - kFineLevel0); }
   uint32_t step = 0;
   int cus = 256;
   Buf<float> tmaster, tgrad, tm, tv;
+  Buf<uint32_t> tgrad16;  // gradient of levels >= kFineLevel0, FP16 pairs
   Buf<uint32_t> t16;
   Buf<float> mmaster, mm, mv, partial;
   Buf<uint16_t> pack_fwd, pack_bwd;  // FP16 or BF16 bits, per training precision
@@ -876,6 +904,7 @@ struct aqua_nrc {
     a.npad = (n + 31u) & ~31u;
     a.table16 = t16.p;
     a.table_grad = tgrad.p;
+    a.table_grad16 = tgrad16.p;
     a.HT = HT.p;
     a.DT = DT.p;
     a.loss_acc = loss.p;
@@ -977,8 +1006,9 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
                                                         nrc->partial.p);
     HIPCHK(hipEventRecord(nrc->ev[3], s));
     k_nrc_adam_mlp<<<kMlp / 64, 256, 0, s>>>(nrc->partial.p, nchunks, nrc->mmaster.p, nrc->mm.p, nrc->mv.p, A);
-    k_nrc_adam_table<<<blocks_for(nrc->entries), kBlock, 0, s>>>(nrc->tmaster.p, nrc->tgrad.p, nrc->tm.p,
-                                                                 nrc->tv.p, nrc->t16.p, nrc->entries, A);
+    k_nrc_adam_table<<<blocks_for(nrc->entries), kBlock, 0, s>>>(
+        nrc->tmaster.p, nrc->tgrad.p, nrc->tgrad16.p, nrc->entries - nrc->fine_entries(), nrc->tm.p, nrc->tv.p,
+        nrc->t16.p, nrc->entries, A);
     if (int e = nrc->repack(s)) return e;
     HIPCHK(hipEventRecord(nrc->ev[4], s));
     HIPCHK(hipGetLastError());
@@ -1100,6 +1130,7 @@ int aqua_nrc_create(int device, const aqua_nrc_cfg* cfg, aqua_nrc** out) {
   hipStream_t s = r->stream;
   if (int e = r->tmaster.alloc(2 * r->entries)) return e;
   if (int e = r->tgrad.alloc(2 * r->entries)) return e;
+  if (int e = r->tgrad16.alloc(r->fine_entries())) return e;
   if (int e = r->tm.alloc(2 * r->entries)) return e;
   if (int e = r->tv.alloc(2 * r->entries)) return e;
   if (int e = r->t16.alloc(r->entries)) return e;
@@ -1117,6 +1148,7 @@ int aqua_nrc_create(int device, const aqua_nrc_cfg* cfg, aqua_nrc** out) {
   if (int e = r->ensure_rows(std::max(32u, cfg->max_batch))) return e;
   k_nrc_init_table<<<blocks_for(r->entries), kBlock, 0, s>>>(r->tmaster.p, r->t16.p, r->entries, cfg->seed);
   k_fill_f32<<<blocks_for(2 * r->entries), kBlock, 0, s>>>(r->tgrad.p, 2 * r->entries, 0.0f);
+  k_fill_u32<<<blocks_for(r->fine_entries()), kBlock, 0, s>>>(r->tgrad16.p, r->fine_entries(), 0u);
   k_fill_f32<<<blocks_for(2 * r->entries), kBlock, 0, s>>>(r->tm.p, 2 * r->entries, 0.0f);
   k_fill_f32<<<blocks_for(2 * r->entries), kBlock, 0, s>>>(r->tv.p, 2 * r->entries, 0.0f);
   k_fill_f32<<<blocks_for(kMlp), kBlock, 0, s>>>(r->mm.p, kMlp, 0.0f);
@@ -1176,6 +1208,7 @@ int aqua_nrc_set_params(aqua_nrc* nrc, const float* table, const float* mlp) {
     HIPCHK(hipStreamSynchronize(s));
   }
   k_fill_f32<<<blocks_for(2 * nrc->entries), kBlock, 0, s>>>(nrc->tgrad.p, 2 * nrc->entries, 0.0f);
+  k_fill_u32<<<blocks_for(nrc->fine_entries()), kBlock, 0, s>>>(nrc->tgrad16.p, nrc->fine_entries(), 0u);
   k_fill_f32<<<blocks_for(2 * nrc->entries), kBlock, 0, s>>>(nrc->tm.p, 2 * nrc->entries, 0.0f);
   k_fill_f32<<<blocks_for(2 * nrc->entries), kBlock, 0, s>>>(nrc->tv.p, 2 * nrc->entries, 0.0f);
   k_fill_f32<<<blocks_for(kMlp), kBlock, 0, s>>>(nrc->mm.p, kMlp, 0.0f);
@@ -1264,6 +1297,8 @@ int aqua_nrc_gradients(aqua_nrc* nrc, uint32_t n, const float* records, const fl
   float lsum = 0.0f;
   HIPCHK(hipMemcpyAsync(&lsum, nrc->loss.p, sizeof lsum, hipMemcpyDeviceToHost, s));
   if (g_mlp) HIPCHK(hipMemcpyAsync(g_mlp, gm.p, kMlp * sizeof(float), hipMemcpyDeviceToHost, s));
+  const size_t fine = nrc->fine_entries(), fine0 = nrc->entries - fine;
+  k_grad16_to_f32<<<blocks_for(fine), kBlock, 0, s>>>(nrc->tgrad16.p, fine, nrc->tgrad.p + 2 * fine0);
   if (g_table)
     HIPCHK(hipMemcpyAsync(g_table, nrc->tgrad.p, 2 * nrc->entries * sizeof(float), hipMemcpyDeviceToHost, s));
   k_fill_f32<<<blocks_for(2 * nrc->entries), kBlock, 0, s>>>(nrc->tgrad.p, 2 * nrc->entries, 0.0f);
