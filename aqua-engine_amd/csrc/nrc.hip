@@ -34,10 +34,11 @@
 namespace {
 
 constexpr int kLevels = 16;
-// Levels kFineLevel0.. accumulate their table gradient as FP16 pairs (one
-// packed atomic per entry instead of two FP32 ones, as tiny-cuda-nn does);
-// the coarser levels, whose entries collect many rows, stay FP32.
-constexpr int kFineLevel0 = 8;
+// Levels kFineLevel0.. (hashed with the default 2^19 table) accumulate their
+// table gradient as FP16 pairs (one packed atomic per entry instead of two
+// FP32 ones, as tiny-cuda-nn does); levels 0-3, whose few entries collect
+// many rows each, stay FP32.
+constexpr int kFineLevel0 = 4;
 constexpr int kHidden = 4;
 constexpr int kMlp = 4 * 64 * 64 + 64 * 16;    // 17408 floats
 static_assert(kMlp % 64 == 0, "partial sums run 64 weights per block");
@@ -486,7 +487,7 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
           // Levels 0..7: the 16 rows of a tile (spatially sorted by the
           // caller) often share an entry. Runs of equal entries across the
           // lane group's 16 lanes are summed first (segmented suffix sum),
-          // and only the first lane of a run issues the atomics.
+          // and only the first lane of a run issues the atomic(s).
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
             uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
@@ -505,14 +506,20 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
               }
             }
             if (head && (v0 != 0.0f || v1 != 0.0f)) {
-              float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
-              atomicAdd(gp, v0);
-              atomicAdd(gp + 1, v1);
+              if (lv >= kFineLevel0) {
+                h2v hv = {(_Float16)v0, (_Float16)v1};
+                __builtin_amdgcn_global_atomic_fadd_v2f16(
+                    reinterpret_cast<h2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_ + idx, hv);
+              } else {
+                float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
+                atomicAdd(gp, v0);
+                atomicAdd(gp + 1, v1);
+              }
             }
           }
         } else {
           if (d0 == 0.0f && d1 == 0.0f) continue;
-          // levels 8..15: one packed FP16 atomic per corner
+          // levels 8..15: one packed FP16 atomic per corner, unmerged
           h2v* gl = reinterpret_cast<h2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_;
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
