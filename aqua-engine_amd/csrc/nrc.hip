@@ -35,9 +35,11 @@ namespace {
 
 constexpr int kLevels = 16;
 // Levels kFineLevel0.. (hashed with the default 2^19 table) accumulate their
-// table gradient as FP16 pairs (one packed atomic per entry instead of two
-// FP32 ones, as tiny-cuda-nn does); levels 0-3, whose few entries collect
-// many rows each, stay FP32.
+// table gradient as BF16 pairs: one packed atomic per entry instead of two
+// FP32 ones. BF16 rather than FP16 because the relative L2 gradient
+// 2 (y - t) / (y^2 + 0.01) reaches 200 t where the prediction is near zero,
+// and radiance targets reach 10^3 and more: FP16 sums overflow there. Levels
+// 0-3, whose few entries collect many rows each, stay FP32.
 constexpr int kFineLevel0 = 4;
 constexpr int kHidden = 4;
 constexpr int kMlp = 4 * 64 * 64 + 64 * 16;    // 17408 floats
@@ -59,7 +61,7 @@ inline uint32_t max_chunks(uint32_t npad) { return std::max(257u, (npad + 4095u)
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
-typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef __bf16 g2v __attribute__((ext_vector_type(2)));  // BF16 gradient pair
 
 struct Grid {
   float res[kLevels];
@@ -182,6 +184,19 @@ struct Op<AQUA_NRC_BF16> {
   static __device__ __forceinline__ uint16_t bits(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
 };
 
+// FP16 training: backward values are clamped to the FP16 range before they
+// become 16-bit operands or stored gradients, so an extreme sample (a
+// prediction near zero against a bright target makes the relative L2
+// gradient about 200 x the target) saturates instead of turning into inf and
+// then NaN. BF16 has the FP32 range and needs no clamp.
+template <int P>
+__device__ __forceinline__ float clamp16(float v) {
+  if constexpr (P == AQUA_NRC_F16)
+    return __builtin_amdgcn_fmed3f(v, -65504.0f, 65504.0f);
+  else
+    return v;
+}
+
 __device__ __forceinline__ float sat448(float v) { return v > 448.0f ? 448.0f : (v < -448.0f ? -448.0f : v); }
 
 template <>
@@ -220,7 +235,7 @@ struct FwdArgs {
   float* out;               // 3 per row (inference)
   // training
   float* table_grad;        // [level][T][2] FP32 (levels < kFineLevel0)
-  uint32_t* table_grad16;   // [level - kFineLevel0][T] two FP16
+  uint32_t* table_grad16;   // [level - kFineLevel0][T] two BF16
   uint16_t* HT;             // 5 x [npad][64] layer inputs
   uint16_t* DT;             // 5 x [npad][64] pre-activation grads (last: 16 used)
   float* loss_acc;
@@ -438,7 +453,7 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
           float den = y[j] * y[j] + 0.01f;
           float d = y[j] - tv[j];
           loss_lane += d * d / den;
-          dy[j] = 2.0f * d / den;
+          dy[j] = clamp16<P>(2.0f * d / den);
         }
       }
       store4<O>(a.DT + (size_t(kHidden) * a.npad + row) * 64 + 4 * g, dy);
@@ -454,7 +469,7 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
           f4v acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
           acc2 = O::mma(O::unpack(pb[kHidden * 8 * 64 + mt * 64 + lane]), bd, acc2);
 #pragma unroll
-          for (int j = 0; j < 4; ++j) dz[mt][j] = ((mask[kHidden] >> (4 * mt + j)) & 1u) ? acc2[j] : 0.0f;
+          for (int j = 0; j < 4; ++j) dz[mt][j] = ((mask[kHidden] >> (4 * mt + j)) & 1u) ? clamp16<P>(acc2[j]) : 0.0f;
         }
       }
 #pragma unroll
@@ -471,7 +486,7 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
           acc2 = O::mma(O::unpack(pb[((LL * 4 + mt) * 2 + 1) * 64 + lane]), bd1, acc2);
 #pragma unroll
           for (int j = 0; j < 4; ++j)
-            dz[mt][j] = (LL == 0 || ((mask[LL] >> (4 * mt + j)) & 1u)) ? acc2[j] : 0.0f;
+            dz[mt][j] = (LL == 0 || ((mask[LL] >> (4 * mt + j)) & 1u)) ? clamp16<P>(acc2[j]) : 0.0f;
         }
       }
       // ---- hash-grid gradient scatter: dz now holds d loss / d input
@@ -507,9 +522,9 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
             }
             if (head && (v0 != 0.0f || v1 != 0.0f)) {
               if (lv >= kFineLevel0) {
-                h2v hv = {(_Float16)v0, (_Float16)v1};
-                __builtin_amdgcn_global_atomic_fadd_v2f16(
-                    reinterpret_cast<h2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_ + idx, hv);
+                g2v hv = {(__bf16)v0, (__bf16)v1};
+                __builtin_amdgcn_global_atomic_fadd_v2bf16(
+                    reinterpret_cast<g2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_ + idx, hv);
               } else {
                 float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
                 atomicAdd(gp, v0);
@@ -519,14 +534,14 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
           }
         } else {
           if (d0 == 0.0f && d1 == 0.0f) continue;
-          // levels 8..15: one packed FP16 atomic per corner, unmerged
-          h2v* gl = reinterpret_cast<h2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_;
+          // levels 8..15: one packed BF16 atomic per corner, unmerged
+          g2v* gl = reinterpret_cast<g2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_;
 #pragma unroll
           for (int k = 0; k < 8; ++k) {
             uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
             float w = corner_w(c, k);
-            h2v v = {(_Float16)(w * d0), (_Float16)(w * d1)};
-            __builtin_amdgcn_global_atomic_fadd_v2f16(gl + idx, v);
+            g2v v = {(__bf16)(w * d0), (__bf16)(w * d1)};
+            __builtin_amdgcn_global_atomic_fadd_v2bf16(gl + idx, v);
           }
         }
       }
@@ -652,7 +667,7 @@ __global__ void __launch_bounds__(256) k_nrc_reduce_partials(const float* partia
 }
 
 // Entries below `fine0` take their gradient from the FP32 pairs, the rest
-// from the FP16 pairs of grad16; both are cleared for the next step.
+// from the BF16 pairs of grad16; both are cleared for the next step.
 __global__ void k_nrc_adam_table(float* master, float* grad, uint32_t* grad16, size_t fine0, float* m, float* v,
                                  uint32_t* table16, size_t n_entries, AdamArgs A) {
   size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -663,7 +678,7 @@ __global__ void k_nrc_adam_table(float* master, float* grad, uint32_t* grad16, s
     gr = reinterpret_cast<float2*>(grad)[e];
     reinterpret_cast<float2*>(grad)[e] = make_float2(0.0f, 0.0f);
   } else {
-    const h2v h = reinterpret_cast<const h2v*>(grad16)[e - fine0];
+    const g2v h = reinterpret_cast<const g2v*>(grad16)[e - fine0];
     gr = make_float2(float(h.x), float(h.y));
     grad16[e - fine0] = 0u;
   }
@@ -766,12 +781,12 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
-// The FP16 gradient pairs of the fine levels as FP32 pairs (for reading the
+// The BF16 gradient pairs of the fine levels as FP32 pairs (for reading the
 // gradient back); clears them.
 __global__ void k_grad16_to_f32(uint32_t* g16, size_t n, float* out) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const h2v h = reinterpret_cast<const h2v*>(g16)[i];
+  const g2v h = reinterpret_cast<const g2v*>(g16)[i];
   reinterpret_cast<float2*>(out)[i] = make_float2(float(h.x), float(h.y));
   g16[i] = 0u;
 }
@@ -835,7 +850,7 @@ struct aqua_nrc {
   uint32_t step = 0;
   int cus = 256;
   Buf<float> tmaster, tgrad, tm, tv;
-  Buf<uint32_t> tgrad16;  // gradient of levels >= kFineLevel0, FP16 pairs
+  Buf<uint32_t> tgrad16;  // gradient of levels >= kFineLevel0, BF16 pairs
   Buf<uint32_t> t16;
   Buf<float> mmaster, mm, mv, partial;
   Buf<uint16_t> pack_fwd, pack_bwd;  // FP16 or BF16 bits, per training precision

@@ -157,3 +157,21 @@ def test_fp8_inference_after_calibration(gpu):
     want = ref.infer(rec, t.reshape(16, 1 << LOG2, 2), m)
     got = nrc.infer(rec, aqua.NRC_FP8)
     assert rel_l2(got, want) < TOL[aqua.NRC_FP8], rel_l2(got, want)
+
+
+@pytest.mark.parametrize("prec", [aqua.NRC_F16, aqua.NRC_BF16])
+def test_bright_targets_keep_training_finite(gpu, prec):
+    # A prediction near zero against a bright target (an emitter seen through
+    # a suffix path) makes the relative L2 gradient 2 (y - t) / (y^2 + 0.01)
+    # about 200 t = 4e6, far past 65504: FP16 backward operands must saturate
+    # and the 16-bit table-gradient pairs must not overflow to inf.
+    nrc, _, _ = make(prec, lr=1e-3)
+    rec = records(8192, 9)
+    tgt = np.full((8192, 3), 2e4, np.float32)
+    _, gt, gm = nrc.gradients(rec, tgt)
+    assert np.all(np.isfinite(gt)) and np.all(np.isfinite(gm))
+    assert np.abs(gt).max() > 0 and np.abs(gm).max() > 0
+    st = nrc.train(rec, tgt, steps=5)
+    assert np.isfinite(st.loss)
+    t, m = nrc.params()
+    assert np.all(np.isfinite(t)) and np.all(np.isfinite(m))
