@@ -379,6 +379,8 @@ def main():
     avg_launch_ms = ms_isect / max(1, launches)
     bytes_per_launch = bytes_per_ray * isect_rays / max(1, launches)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    nodes_per_launch = cst.nodes_visited / closest_rays * isect_rays / max(1, launches)
+    node_rate = nodes_per_launch / (avg_launch_ms * 1e-3) if avg_launch_ms > 0 else 0.0
 
     out = {
         "metric": METRIC,
@@ -421,6 +423,13 @@ def main():
             "tris_per_ray": round(cst.tris_tested / closest_rays, 2),
             "avg_launch_ms": round(avg_launch_ms, 4),
             "launches": launches,
+            # node fetches per second against the rate HBM alone could feed
+            # (8 TB/s / 64-byte nodes = 125 G nodes/s)
+            "node_fetch_rate": {
+                "nodes_per_s": round(node_rate),
+                "peak_nodes_per_s": round(HBM_PEAK_GBS * 1e9 / NODE_BYTES),
+                "frac": round(node_rate / (HBM_PEAK_GBS * 1e9 / NODE_BYTES), 4),
+            },
         },
     }
     if not args.no_cpu_baseline:
