@@ -90,8 +90,15 @@ AQ_DEV float slab_enter(float nx, float fx, float ny, float fy, float nz, float 
   return tn <= __builtin_fmaf(tf, kFarScale, delta) ? tn : __builtin_inff();
 }
 
+// The first kTopNodes nodes (the builder puts the top three levels there)
+// are staged in LDS by every traversal block: every ray starts at the root,
+// so these are the most-fetched nodes.
+constexpr uint32_t kTopNodes = 85;
+
 struct DevBvh {
   const float4* nodes;  // 4 per 4-wide node (64-byte quantised nodes)
+  const float4* top;    // nodes [0, n_top) in LDS (set by the kernel)
+  uint32_t n_top;       // min(kTopNodes, node count)
   const float4* tris;   // 3 per triangle slot: (v0, id), (v1, material), (v2, 0)
   float delta;
   uint32_t* spill;      // traversal spill stacks: kSpillStack x (grid threads) words
@@ -123,6 +130,14 @@ struct TStack {
   uint32_t sstride; // threads per grid
   int top, cnt, gsp;
 };
+
+// Every thread of the block calls it: copies nodes [0, bvh.n_top) to `lds`
+// and points bvh.top at it.
+AQ_DEV void stage_top(DevBvh& bvh, float4* lds) {
+  for (uint32_t i = threadIdx.x; i < 4 * bvh.n_top; i += blockDim.x) lds[i] = bvh.nodes[i];
+  __syncthreads();
+  bvh.top = lds;
+}
 
 // Thread (blockIdx, threadIdx) of a grid of gridDim x kStackThreads threads.
 AQ_DEV void st_init(TStack& s, uint32_t* lds_base, uint32_t* spill_base) {
@@ -269,7 +284,7 @@ AQ_DEV void closest_node(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCoun
   float tn[4];
   uint32_t ref[4];
   uint32_t leaf[2];
-  node_test4(c.r, bvh.nodes + 4 * c.node, c.tbest, bvh.delta, tn, ref, leaf);
+  node_test4(c.r, (c.node < bvh.n_top ? bvh.top : bvh.nodes) + 4 * c.node, c.tbest, bvh.delta, tn, ref, leaf);
   if (COUNT) cnt->nodes++;
   c.lbase = leaf[0];
   c.lmask = leaf_mask(tn, leaf[1]);
@@ -368,7 +383,7 @@ AQ_DEV void any_node(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) 
   float tn[4];
   uint32_t ref[4];
   uint32_t leaf[2];
-  node_test4(a.r, bvh.nodes + 4 * a.node, a.tmax, bvh.delta, tn, ref, leaf);
+  node_test4(a.r, (a.node < bvh.n_top ? bvh.top : bvh.nodes) + 4 * a.node, a.tmax, bvh.delta, tn, ref, leaf);
   if (COUNT) cnt->nodes++;
   a.lbase = leaf[0];
   a.lmask = leaf_mask(tn, leaf[1]);

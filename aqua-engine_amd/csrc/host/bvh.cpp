@@ -555,6 +555,37 @@ void collapse_bvh4(const Bvh& bin, Bvh4* out, int max_leaf) {
     }
   }
   c.emit(0, j, 0, out);
+  order_top_levels(out, kTopDepth);
+}
+
+void order_top_levels(Bvh4* b, int depth) {
+  const size_t nn = b->nodes.size();
+  if (nn == 0) return;
+  std::vector<int> d(nn, -1);
+  std::vector<uint32_t> bfs{0};
+  d[0] = 0;
+  for (size_t i = 0; i < bfs.size(); ++i) {
+    const Bvh4Node& nd = b->nodes[bfs[i]];
+    if (d[bfs[i]] >= depth) continue;
+    for (int k = 0; k < 4; ++k)
+      if (nd.child[k] >= 0 && nd.count[k] == 0) {
+        d[size_t(nd.child[k])] = d[bfs[i]] + 1;
+        bfs.push_back(uint32_t(nd.child[k]));
+      }
+  }
+  std::vector<uint32_t> id(nn, 0xFFFFFFFFu);
+  uint32_t next = 0;
+  for (uint32_t v : bfs) id[v] = next++;
+  for (size_t v = 0; v < nn; ++v)
+    if (id[v] == 0xFFFFFFFFu) id[v] = next++;
+  std::vector<Bvh4Node> out(nn);
+  for (size_t v = 0; v < nn; ++v) {
+    Bvh4Node nd = b->nodes[v];
+    for (int k = 0; k < 4; ++k)
+      if (nd.child[k] >= 0 && nd.count[k] == 0) nd.child[k] = int32_t(id[size_t(nd.child[k])]);
+    out[id[v]] = nd;
+  }
+  b->nodes.swap(out);
 }
 
 }  // namespace aqua

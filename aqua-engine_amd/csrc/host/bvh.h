@@ -51,7 +51,7 @@ struct Bvh4Node {
 static_assert(sizeof(Bvh4Node) == 128, "node layout");
 
 struct Bvh4 {
-  std::vector<Bvh4Node> nodes;  // node 0 is the root, preorder
+  std::vector<Bvh4Node> nodes;  // node 0 is the root; top levels first, then preorder
   std::vector<uint32_t> tri_order;  // triangle slot -> triangle id (load order)
   uint32_t max_inner_depth = 0; // deepest inner node (root = 0)
 };
@@ -82,5 +82,12 @@ bool quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out);
 // slots, in child order, so traversal can test a node's hit leaves as one
 // bit mask over that run.
 void collapse_bvh4(const Bvh& bin, Bvh4* out, int max_leaf = 4);
+
+// Node order: the nodes of depth <= kTopDepth (at most 1 + 4 + 16 + 64 = 85)
+// come first, breadth-first, so the traversal kernels can stage a prefix of
+// the node array in LDS; the rest keep their preorder. A child's index is
+// always larger than its parent's. collapse_bvh4 applies it.
+constexpr int kTopDepth = 3;
+void order_top_levels(Bvh4* b, int depth);
 
 }  // namespace aqua

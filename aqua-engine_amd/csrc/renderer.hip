@@ -243,6 +243,8 @@ __global__ void __launch_bounds__(kBlock, 6) k_intersect(DevBvh bvh, const float
                                                       const float4* __restrict__ rd,
                                                       const uint32_t* count, uint32_t* cursor, uint2* hit,
                                                       unsigned long long* stats) {
+  __shared__ float4 lds_top[4 * kTopNodes];
+  stage_top(bvh, lds_top);
   __shared__ uint32_t lds_stack[kLdsStack * kBlock];
   TStack st;
   st_init(st, lds_stack, bvh.spill);
@@ -551,6 +553,8 @@ __global__ void k_clear_taux(float4* taux, uint32_t n_slots) {
 // taken (its loads overlap the traversal) and only stored when it ends.
 template <bool COUNT>
 __global__ void __launch_bounds__(kBlock, 6) k_shadow(DevBvh bvh, Queues q, uint32_t b) {
+  __shared__ float4 lds_top[4 * kTopNodes];
+  stage_top(bvh, lds_top);
   __shared__ uint32_t lds_stack[kLdsStack * kBlock];
   TStack st;
   st_init(st, lds_stack, bvh.spill);
@@ -659,6 +663,8 @@ __global__ void k_hits_out(const uint2* hit, const float4* tris, uint32_t n, uin
 
 __global__ void __launch_bounds__(kBlock) k_occluded(DevBvh bvh, const float4* ro, const float4* rd,
                                                      uint32_t n, uint8_t* out) {
+  __shared__ float4 lds_top[4 * kTopNodes];
+  stage_top(bvh, lds_top);
   __shared__ uint32_t lds_stack[kLdsStack * kBlock];
   TStack st;
   st_init(st, lds_stack, bvh.spill);
@@ -840,6 +846,8 @@ struct aqua_renderer {
   DevScene dev_scene() const {
     DevScene s;
     s.bvh.nodes = nodes.p;
+    s.bvh.top = nullptr;
+    s.bvh.n_top = uint32_t(std::min<size_t>(kTopNodes, nodes.n / 4));  // 4 float4 per node
     s.bvh.tris = tris.p;
     s.bvh.delta = delta;
     s.bvh.spill = spill.p;

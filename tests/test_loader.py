@@ -306,6 +306,19 @@ def _check_bvh4(nodes, order, verts):
                 chi = np.stack([nodes[child, 4:8], nodes[child, 12:16], nodes[child, 20:24]], 1)[used]
                 assert np.all(clo >= lo) and np.all(chi <= hi)
     assert np.all(covered == 1)
+    # the top three levels come first, breadth-first (the kernels stage a
+    # prefix of the node array in LDS)
+    depth = {0: 0}
+    order = [0]
+    for n in order:
+        for k in range(4):
+            if idx[n, k] >= 0 and cnt[n, k] == 0:
+                depth[int(idx[n, k])] = depth[n] + 1
+                order.append(int(idx[n, k]))
+    top = sorted(v for v, d in depth.items() if d <= 3)
+    assert top == list(range(len(top)))
+    by_bfs = [v for v in order if depth[v] <= 3]
+    assert by_bfs == sorted(by_bfs)
 
 
 def test_scene_bvh_invariants(box_room, tmp_path):
