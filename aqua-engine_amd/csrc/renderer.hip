@@ -296,6 +296,9 @@ __device__ __forceinline__ uint32_t pick_light(const float* cdf, uint32_t n, flo
 
 // One path vertex per thread: emission (MIS-weighted), then, below the
 // depth limit, next-event estimation, BSDF sampling and Russian roulette.
+// NRC: the cache is on (pp.nrc): query records and training slots; the plain
+// tracer's instance carries none of their registers.
+template <bool NRC>
 __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, uint32_t b, Queues q) {
   const uint32_t c = b & 1u, nx = c ^ 1u;
   const uint32_t n = q.counters[cnt_rays(b)];
@@ -339,7 +342,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
         }
         // ---- neural radiance cache: query (and end the path) or record a
         // training sample of the reflected radiance at this vertex
-        if (pp.nrc) {
+        if (NRC) {
           bool is_train = (path_id % pp.train_stride) == pp.train_offset && path_id / pp.train_stride < pp.n_train;
           uint32_t tslot = 0xFFFFFFFFu;
           if (!is_train && depth == pp.query_depth)
@@ -482,7 +485,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
       q.shd[k] = s_d;
       q.shc[k] = s_c;
     }
-    if (pp.nrc) {
+    if (NRC) {
       uint32_t qi = block_append(query, &q.counters[kCntQuery]);
       if (query) {
         q.qrec[4 * size_t(qi) + 0] = qr0;
@@ -990,7 +993,10 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     else
       k_intersect<false><<<gp, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, cur, q.hit, q.stats);
     if (timing) HIPCHK(hipEventRecord(ev[1], s));
-    k_shade<<<g, kBlock, 0, s>>>(sc, pp, b, q);
+    if (pp.nrc)
+      k_shade<true><<<g, kBlock, 0, s>>>(sc, pp, b, q);
+    else
+      k_shade<false><<<g, kBlock, 0, s>>>(sc, pp, b, q);
     if (timing) HIPCHK(hipEventRecord(ev[2], s));
     if (b < pp.max_depth) {
       if (count)
