@@ -10,6 +10,7 @@
 #pragma once
 
 #include "bsdf.h"
+#include "morton.h"
 #include "trace.h"
 
 namespace aqd {
@@ -95,6 +96,8 @@ struct Queues {
   // cache queries (compacted) and training slots (train index x depth)
   float4* qrec;    // 4 per query: (pos, roughness), (dir, -), (normal, -), (albedo, -)
   float4* qaux;    // (throughput, path id bits)
+  uint32_t* qkey;  // Morton key of each query's position (sort input)
+  uint32_t* qidx;  // its slot (sort input)
   float4* trec;    // 4 per training slot, as qrec
   float4* taux;    // 2 per training slot: (throughput, valid), (radiance after emission, -)
 };

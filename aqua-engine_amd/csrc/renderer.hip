@@ -493,6 +493,8 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
         q.qrec[4 * size_t(qi) + 2] = qr2;
         q.qrec[4 * size_t(qi) + 3] = qr3;
         q.qaux[qi] = qa;
+        q.qkey[qi] = morton30(qr0.x, qr0.y, qr0.z);
+        q.qidx[qi] = qi;
       }
     }
   }
@@ -882,6 +884,8 @@ struct aqua_renderer {
     q.stats = stats.p;
     q.qrec = qrec.p;
     q.qaux = qaux.p;
+    q.qkey = skey.p;
+    q.qidx = sidx.p;
     q.trec = trec.p;
     q.taux = taux.p;
     return q;
@@ -1023,7 +1027,8 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     if (2 * size_t(n) > r->skey.n || 2 * size_t(n) > r->sidx.n)
       return aqua::fail(AQUA_ERR_HIP, "query sort buffers too small for %u queries", n);
     if (int e = r->stemp.ensure(need)) return e;  // rocPRIM's need is not monotone in n
-    HIPCHK(aqua::query_sort(r->stemp.p, r->stemp.n, r->qrec.p, q.counters + kCntQuery, n, r->skey.p, r->sidx.p,
+    // k_shade wrote the queries' keys; the sort fills the padding slots
+    HIPCHK(aqua::query_sort(r->stemp.p, r->stemp.n, nullptr, q.counters + kCntQuery, n, r->skey.p, r->sidx.p,
                             s, &perm));
     if (int e = aqua::nrc_infer_device(r->nrc, n, r->qrec.p, r->qout.p, prec, s, q.counters + kCntQuery, perm))
       return e;

@@ -17,26 +17,14 @@
 
 #include <cstdint>
 
+#include "device/morton.h"
+
 namespace aqua {
 
 constexpr unsigned kSortLowBit = 6;
 constexpr unsigned kSortHighBit = 30;
 
 namespace {
-
-__device__ __forceinline__ uint32_t spread3(uint32_t v) {
-  v &= 0x3FFu;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
-__device__ __forceinline__ uint32_t cell10(float x) {
-  float c = fminf(fmaxf(x, 0.0f), 1.0f) * 1024.0f;
-  return min(uint32_t(c), 1023u);
-}
 
 // rec: 4 float4 per row, row 0's xyz = position in [0,1]^3.
 __global__ void k_query_keys(const float4* rec, const uint32_t* count, uint32_t n, uint32_t* key,
@@ -46,9 +34,19 @@ __global__ void k_query_keys(const float4* rec, const uint32_t* count, uint32_t 
     uint32_t k = 0xFFFFFFFFu;
     if (i < m) {
       float4 p = rec[4 * size_t(i)];
-      k = spread3(cell10(p.x)) | (spread3(cell10(p.y)) << 1) | (spread3(cell10(p.z)) << 2);
+      k = aqd::morton30(p.x, p.y, p.z);
     }
     key[i] = k;
+    idx[i] = i;
+  }
+}
+
+// Slots [*count, n) of keys the producer already wrote for [0, *count):
+// the all-ones key (sorts last) and their own index.
+__global__ void k_query_tail(const uint32_t* count, uint32_t n, uint32_t* key, uint32_t* idx) {
+  const uint32_t m = *count;
+  for (uint32_t i = m + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    key[i] = 0xFFFFFFFFu;
     idx[i] = i;
   }
 }
@@ -67,13 +65,21 @@ hipError_t query_sort_storage(uint32_t n, size_t* bytes) {
 // each (double buffers); *perm receives the sorted slot order (n entries,
 // the real queries first: LSD radix sort is stable, so they precede the
 // padding even where a key saturates). temp holds temp_bytes of at least
-// query_sort_storage(n). Stream-ordered, no host sync.
+// query_sort_storage(n). With rec == nullptr the producer has already
+// written keys[i] = Morton key and idx[i] = i for the real queries
+// [0, *d_count) (d_count required); only the padding slots are filled here.
+// Stream-ordered, no host sync.
 hipError_t query_sort(void* temp, size_t temp_bytes, const float4* rec, const uint32_t* d_count, uint32_t n,
                       uint32_t* keys, uint32_t* idx, hipStream_t s, const uint32_t** perm) {
   size_t blocks = (size_t(n) + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   if (blocks < 1) blocks = 1;
-  k_query_keys<<<int(blocks), 256, 0, s>>>(rec, d_count, n, keys, idx);
+  if (rec)
+    k_query_keys<<<int(blocks), 256, 0, s>>>(rec, d_count, n, keys, idx);
+  else if (d_count)
+    k_query_tail<<<int(blocks), 256, 0, s>>>(d_count, n, keys, idx);
+  else
+    return hipErrorInvalidValue;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   rocprim::double_buffer<uint32_t> kb(keys, keys + n), vb(idx, idx + n);
