@@ -41,6 +41,7 @@ constexpr int kLevels = 16;
 // and radiance targets reach 10^3 and more: FP16 sums overflow there. Levels
 // 0-3, whose few entries collect many rows each, stay FP32.
 constexpr int kFineLevel0 = 4;
+static_assert(kFineLevel0 <= 8, "levels 8..15 (the unmerged scatter) must accumulate in the BF16 pairs");
 constexpr int kHidden = 4;
 constexpr int kMlp = 4 * 64 * 64 + 64 * 16;    // 17408 floats
 static_assert(kMlp % 64 == 0, "partial sums run 64 weights per block");

@@ -1240,7 +1240,7 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   // Traversal tests a node's hit leaves as a bit mask over one run of at
   // most 4 x 7 slots starting at its first leaf child.
   if (bvh4.tri_order.size() != n)
-    return aqua::fail(AQUA_ERR_HIP, "BVH4 covers %zu of %u triangles", bvh4.tri_order.size(), n);
+    return aqua::fail(AQUA_ERR_UNSUPPORTED, "BVH4 covers %zu of %u triangles", bvh4.tri_order.size(), n);
   for (const aqua::Bvh4Node& nd : bvh4.nodes) {
     uint32_t lo = 0xFFFFFFFFu, hi = 0, sum = 0;
     for (int k = 0; k < 4; ++k)
@@ -1250,7 +1250,7 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
         sum += uint32_t(nd.count[k]);
       }
     if (sum && (hi - lo != sum || hi > n))
-      return aqua::fail(AQUA_ERR_HIP, "BVH4 leaf children of a node are not one run of slots");
+      return aqua::fail(AQUA_ERR_UNSUPPORTED, "BVH4 leaf children of a node are not one run of slots");
   }
   std::vector<float4> tris(3 * size_t(n));
   for (uint32_t s = 0; s < n; ++s) {
