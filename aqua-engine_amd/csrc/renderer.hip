@@ -791,6 +791,11 @@ struct DevBuf {
 struct aqua_renderer {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Shadow rays of bounce b run on stream2, overlapped with the closest-hit
+  // rays of bounce b + 1 (independent work that fills each other's tail);
+  // the fork / join events order them against the shading kernels.
+  hipStream_t stream2 = nullptr;
+  std::vector<hipEvent_t> fork_ev, join_ev;
   aqua_integrator_cfg cfg{};
   aqua_camera camera{};
   aqua_renderer_opts opts{};
@@ -799,7 +804,8 @@ struct aqua_renderer {
   int persistent_blocks = 2048;  // resident blocks of the traversal kernels
   std::vector<uint32_t> tri_order;
   DevBuf<float4> tris, nodes, mats, lights;
-  DevBuf<uint32_t> spill;  // traversal spill stacks (kSpillStack words per grid thread)
+  DevBuf<uint32_t> spill;   // traversal spill stacks (kSpillStack words per grid thread)
+  DevBuf<uint32_t> spill2;  // the same for the shadow kernels, which may run concurrently
   DevBuf<float> cdf, pmf;
   // path queues
   size_t cap = 0;
@@ -832,6 +838,10 @@ struct aqua_renderer {
       (void)hipSetDevice(device);
       (void)hipStreamSynchronize(stream);
     }
+    if (stream2) (void)hipStreamSynchronize(stream2);
+    for (hipEvent_t e : fork_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (nrc) aqua_nrc_destroy(nrc);
     for (hipEvent_t e : kev) (void)hipEventDestroy(e);
     for (hipEvent_t e : nev)
@@ -986,6 +996,13 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   if (n_slots) k_clear_taux<<<int((n_slots + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.taux, n_slots);
   k_camera<<<g, kBlock, 0, s>>>(pp, q);
   bool timing = r->opts.timing != 0;
+  // Without per-kernel timing, the shadow rays of bounce b (stream2) overlap
+  // the closest-hit rays of bounce b + 1; shading b + 1 waits for them (they
+  // add to the radiance it moves, and it rewrites the shadow queue).
+  const bool overlap = !timing;
+  DevBvh shadow_bvh = sc.bvh;
+  shadow_bvh.spill = r->spill2.p;
+  hipStream_t s2 = r->stream2;
   for (uint32_t b = 0; b <= pp.max_depth; ++b) {
     uint32_t c = b & 1u;
     const uint32_t* cnt = q.counters + cnt_rays(b);
@@ -997,19 +1014,28 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     else
       k_intersect<false><<<gp, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, cur, q.hit, q.stats);
     if (timing) HIPCHK(hipEventRecord(ev[1], s));
+    if (overlap && b > 0) HIPCHK(hipStreamWaitEvent(s, r->join_ev[b - 1], 0));
     if (pp.nrc)
       k_shade<true><<<g, kBlock, 0, s>>>(sc, pp, b, q);
     else
       k_shade<false><<<g, kBlock, 0, s>>>(sc, pp, b, q);
     if (timing) HIPCHK(hipEventRecord(ev[2], s));
     if (b < pp.max_depth) {
+      hipStream_t ss = s;
+      if (overlap) {
+        HIPCHK(hipEventRecord(r->fork_ev[b], s));
+        HIPCHK(hipStreamWaitEvent(s2, r->fork_ev[b], 0));
+        ss = s2;
+      }
       if (count)
-        k_shadow<true><<<gp, kBlock, 0, s>>>(sc.bvh, q, b);
+        k_shadow<true><<<gp, kBlock, 0, ss>>>(shadow_bvh, q, b);
       else
-        k_shadow<false><<<gp, kBlock, 0, s>>>(sc.bvh, q, b);
+        k_shadow<false><<<gp, kBlock, 0, ss>>>(shadow_bvh, q, b);
+      if (overlap) HIPCHK(hipEventRecord(r->join_ev[b], s2));
     }
     if (timing) HIPCHK(hipEventRecord(ev[3], s));
   }
+  if (overlap && pp.max_depth > 0) HIPCHK(hipStreamWaitEvent(s, r->join_ev[pp.max_depth - 1], 0));
   HIPCHK(hipGetLastError());
   float ms_nrc = 0.0f;
   bool trained = false;
@@ -1217,6 +1243,11 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   r->camera = scene->camera;
   r->opts.timing = 1;
   HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&r->stream2, hipStreamNonBlocking));
+  r->fork_ev.resize(kMaxBounces + 1);
+  r->join_ev.resize(kMaxBounces + 1);
+  for (hipEvent_t& e : r->fork_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (hipEvent_t& e : r->join_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   HIPCHK(hipEventCreate(&r->ev0));
   HIPCHK(hipEventCreate(&r->ev1));
   r->kev.resize(4 * (kMaxBounces + 1));
@@ -1331,6 +1362,7 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   if (int e = r->stats.ensure(kStWords)) return e;
   // traversal spill stacks of every thread of the (capped) traversal grids
   if (int e = r->spill.ensure(size_t(kSpillStack) * r->persistent_blocks * kBlock)) return e;
+  if (int e = r->spill2.ensure(size_t(kSpillStack) * r->persistent_blocks * kBlock)) return e;
   if (cfg->type == AQUA_INTEGRATOR_NRC) {
     if (cfg->nrc_precision > AQUA_NRC_FP8)
       return aqua::fail(AQUA_ERR_INVALID_ARG, "unknown nrc_precision %u", cfg->nrc_precision);

@@ -145,6 +145,39 @@ def test_render_interior_ggx_parity(gpu, tmp_path):
     np.testing.assert_array_equal(g, c)
 
 
+def test_overlapped_shadow_rays_do_not_change_the_film(gpu, box_room, box_room_oracle):
+    # without per-kernel timing the shadow rays of bounce b run on a second
+    # stream beside the closest-hit rays of bounce b + 1; the film must be the
+    # one of the serialized order, bit for bit (and the oracle's), also with
+    # the cache on
+    cam = small_camera(box_room.camera(), 61, 35)
+    r = aqua.Renderer(box_room, cfg(spp=6))
+    r.set_opts(timing=True)
+    r.render(6, frame=3, camera=cam)
+    serial = r.film(cam)
+    r.set_opts(timing=False)
+    for _ in range(3):
+        r.render(6, frame=3, camera=cam)
+        np.testing.assert_array_equal(r.film(cam), serial)
+    img_c, _ = box_room_oracle.render(aqua.camera_to_array(cam), 6, 5, frame=3)
+    np.testing.assert_array_equal(serial, img_c)
+    # large enough that both traversal grids are full and really run together
+    big = small_camera(box_room.camera(), 512, 384)
+    r2 = aqua.Renderer(box_room, cfg(spp=4, max_depth=8))
+    r2.render(4, frame=7, camera=big)
+    serial = r2.film(big)
+    r2.set_opts(timing=False)
+    r2.render(4, frame=7, camera=big)
+    np.testing.assert_array_equal(r2.film(big), serial)
+    c = cfg(spp=4)
+    c.type = aqua.INTEGRATOR_NRC
+    c.nrc_train_paths = 256
+    n = aqua.Renderer(box_room, c)
+    n.set_opts(timing=False)
+    st = n.render(4, frame=1, camera=cam)
+    assert st.nrc_queries > 0 and np.all(np.isfinite(n.film(cam)))
+
+
 def test_pass_split_does_not_change_the_film(gpu, box_room, box_room_oracle):
     # sample s of pixel p draws from a fixed position of stream p, so the film
     # is the same however the samples are split into passes (here 3 + 3 + 1
