@@ -173,6 +173,7 @@ def nrc_eval_leg(aqua, frames=tuple(range(15, 240, 30)), n_frames=240, oracle_sp
     pt = aqua.integrator_default()
     pt.max_depth = 8
     ref = aqua.Renderer(scene, pt)
+    ref.set_opts(timing=False)  # ms/frame from the frame's own events; kernels overlapped
     caches = {}
     for qv in (2, 1):
         c = aqua.integrator_default()
@@ -183,6 +184,7 @@ def nrc_eval_leg(aqua, frames=tuple(range(15, 240, 30)), n_frames=240, oracle_sp
         c.nrc_train_paths = 1 << 16
         c.nrc_query_vertex = qv
         caches[qv] = aqua.Renderer(scene, c)
+        caches[qv].set_opts(timing=False)
     out = []
     next_frame = 0
     for fe in frames:
