@@ -788,14 +788,78 @@ struct DevBuf {
   }
 };
 
+// One set of path queues with its counters, work cursors, traversal spill
+// stacks and streams. A pass runs on one lane; without per-kernel timing,
+// consecutive passes of a frame alternate between two lanes and run
+// concurrently (only their film accumulations are ordered), so one pass's
+// memory-bound shading overlaps the other's traversal.
+struct Lane {
+  hipStream_t s = nullptr;   // camera, closest hit, shading, film
+  // Shadow rays of bounce b run on s2, overlapped with the closest-hit rays
+  // of bounce b + 1 (independent work that fills each other's tail); the
+  // fork / join events order them against the shading kernels.
+  hipStream_t s2 = nullptr;
+  std::vector<hipEvent_t> fork_ev, join_ev;
+  hipEvent_t film_done = nullptr, done = nullptr;
+  size_t cap = 0;
+  DevBuf<float4> ro[2], rd[2], tp[2], L[2], sho, shd, shc, Lbuf;
+  DevBuf<uint4> rng[2];
+  DevBuf<uint2> hit;
+  DevBuf<uint32_t> counters, cursors;
+  DevBuf<uint32_t> spill;   // traversal spill stacks (kSpillStack words per grid thread)
+  DevBuf<uint32_t> spill2;  // the same for the shadow kernels, which may run concurrently
+
+  int init(int persistent_blocks, int max_bounces) {
+    if (s) return AQUA_OK;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+    fork_ev.resize(size_t(max_bounces) + 1);
+    join_ev.resize(size_t(max_bounces) + 1);
+    for (hipEvent_t& e : fork_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (hipEvent_t& e : join_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&film_done, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    if (int e = counters.ensure(kCntWords)) return e;
+    if (int e = cursors.ensure(kCursorWords)) return e;
+    if (int e = spill.ensure(size_t(kSpillStack) * persistent_blocks * kBlock)) return e;
+    if (int e = spill2.ensure(size_t(kSpillStack) * persistent_blocks * kBlock)) return e;
+    return AQUA_OK;
+  }
+
+  int ensure(size_t n) {
+    if (n <= cap) return AQUA_OK;
+    for (int k = 0; k < 2; ++k) {
+      if (int e = ro[k].ensure(n)) return e;
+      if (int e = rd[k].ensure(n)) return e;
+      if (int e = tp[k].ensure(n)) return e;
+      if (int e = L[k].ensure(n)) return e;
+      if (int e = rng[k].ensure(n)) return e;
+    }
+    if (int e = hit.ensure(n)) return e;
+    if (int e = sho.ensure(n)) return e;
+    if (int e = shd.ensure(n)) return e;
+    if (int e = shc.ensure(n)) return e;
+    if (int e = Lbuf.ensure(n)) return e;
+    cap = n;
+    return AQUA_OK;
+  }
+
+  ~Lane() {
+    if (s) (void)hipStreamSynchronize(s);
+    if (s2) (void)hipStreamSynchronize(s2);
+    for (hipEvent_t e : fork_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
+    if (film_done) (void)hipEventDestroy(film_done);
+    if (done) (void)hipEventDestroy(done);
+    if (s2) (void)hipStreamDestroy(s2);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
 struct aqua_renderer {
   int device = 0;
-  hipStream_t stream = nullptr;
-  // Shadow rays of bounce b run on stream2, overlapped with the closest-hit
-  // rays of bounce b + 1 (independent work that fills each other's tail);
-  // the fork / join events order them against the shading kernels.
-  hipStream_t stream2 = nullptr;
-  std::vector<hipEvent_t> fork_ev, join_ev;
+  hipStream_t stream = nullptr;  // lane[0].s
+  Lane lane[2];
   aqua_integrator_cfg cfg{};
   aqua_camera camera{};
   aqua_renderer_opts opts{};
@@ -804,16 +868,12 @@ struct aqua_renderer {
   int persistent_blocks = 2048;  // resident blocks of the traversal kernels
   std::vector<uint32_t> tri_order;
   DevBuf<float4> tris, nodes, mats, lights;
-  DevBuf<uint32_t> spill;   // traversal spill stacks (kSpillStack words per grid thread)
-  DevBuf<uint32_t> spill2;  // the same for the shadow kernels, which may run concurrently
   DevBuf<float> cdf, pmf;
-  // path queues
-  size_t cap = 0;
-  DevBuf<float4> ro[2], rd[2], tp[2], L[2], sho, shd, shc, Lbuf;
-  DevBuf<uint4> rng[2];
-  DevBuf<uint2> hit;
-  DevBuf<uint32_t> counters, cursors;
+  size_t nrc_cap = 0;  // query buffers sized for this many paths
   DevBuf<unsigned long long> stats;
+  // pinned host copies of each pass's counters (read after the frame)
+  uint32_t* pass_counters = nullptr;
+  size_t pass_counters_cap = 0;
   // films
   uint32_t film_w = 0, film_h = 0, film_spp = 0, oracle_spp = 0;
   DevBuf<float4> film;
@@ -834,21 +894,16 @@ struct aqua_renderer {
   uint32_t batch_n = 0;
 
   ~aqua_renderer() {
-    if (stream) {
-      (void)hipSetDevice(device);
-      (void)hipStreamSynchronize(stream);
-    }
-    if (stream2) (void)hipStreamSynchronize(stream2);
-    for (hipEvent_t e : fork_ev) (void)hipEventDestroy(e);
-    for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
-    if (stream2) (void)hipStreamDestroy(stream2);
+    (void)hipSetDevice(device);
+    if (lane[1].s) (void)hipStreamSynchronize(lane[1].s);
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (pass_counters) (void)hipHostFree(pass_counters);
     if (nrc) aqua_nrc_destroy(nrc);
     for (hipEvent_t e : kev) (void)hipEventDestroy(e);
     for (hipEvent_t e : nev)
       if (e) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
   }
 
   // Blocks of a persistent traversal launch over n rays: a multiple of 8
@@ -865,7 +920,7 @@ struct aqua_renderer {
     s.bvh.n_top = uint32_t(std::min<size_t>(kTopNodes, nodes.n / 4));  // 4 float4 per node
     s.bvh.tris = tris.p;
     s.bvh.delta = delta;
-    s.bvh.spill = spill.p;
+    s.bvh.spill = lane[0].spill.p;
     s.mats = mats.p;
     s.lights = lights.p;
     s.light_cdf = cdf.p;
@@ -875,22 +930,23 @@ struct aqua_renderer {
     return s;
   }
 
-  Queues queues() {
+  Queues queues(int li = 0) {
+    const Lane& l = lane[li];
     Queues q;
     for (int k = 0; k < 2; ++k) {
-      q.ro[k] = ro[k].p;
-      q.rd[k] = rd[k].p;
-      q.tp[k] = tp[k].p;
-      q.L[k] = L[k].p;
-      q.rng[k] = rng[k].p;
+      q.ro[k] = l.ro[k].p;
+      q.rd[k] = l.rd[k].p;
+      q.tp[k] = l.tp[k].p;
+      q.L[k] = l.L[k].p;
+      q.rng[k] = l.rng[k].p;
     }
-    q.hit = hit.p;
-    q.sho = sho.p;
-    q.shd = shd.p;
-    q.shc = shc.p;
-    q.Lbuf = Lbuf.p;
-    q.counters = counters.p;
-    q.cursors = cursors.p;
+    q.hit = l.hit.p;
+    q.sho = l.sho.p;
+    q.shd = l.shd.p;
+    q.shc = l.shc.p;
+    q.Lbuf = l.Lbuf.p;
+    q.counters = l.counters.p;
+    q.cursors = l.cursors.p;
     q.stats = stats.p;
     q.qrec = qrec.p;
     q.qaux = qaux.p;
@@ -901,28 +957,17 @@ struct aqua_renderer {
     return q;
   }
 
+  // Lane 0's queues (and the cache's query buffers) for n paths.
   int ensure_queues(size_t n) {
-    if (n <= cap) return AQUA_OK;
-    for (int k = 0; k < 2; ++k) {
-      if (int e = ro[k].ensure(n)) return e;
-      if (int e = rd[k].ensure(n)) return e;
-      if (int e = tp[k].ensure(n)) return e;
-      if (int e = L[k].ensure(n)) return e;
-      if (int e = rng[k].ensure(n)) return e;
-    }
-    if (int e = hit.ensure(n)) return e;
-    if (int e = sho.ensure(n)) return e;
-    if (int e = shd.ensure(n)) return e;
-    if (int e = shc.ensure(n)) return e;
-    if (int e = Lbuf.ensure(n)) return e;
-    if (nrc) {
+    if (int e = lane[0].ensure(n)) return e;
+    if (nrc && n > nrc_cap) {
       if (int e = qrec.ensure(4 * n)) return e;
       if (int e = qaux.ensure(n)) return e;
       if (int e = qout.ensure(3 * n)) return e;
       if (int e = skey.ensure(2 * n)) return e;
       if (int e = sidx.ensure(2 * n)) return e;
+      nrc_cap = n;
     }
-    cap = n;
     return AQUA_OK;
   }
 };
@@ -983,12 +1028,27 @@ int select_device(int device, int* cu_count = nullptr) {
   return AQUA_OK;
 }
 
-// One pass: S samples of every pixel through all bounces, then the film.
-int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats* st, bool count) {
-  hipStream_t s = r->stream;
+// Ray counts of one pass from its counters.
+void add_pass_counts(aqua_frame_stats* st, const uint32_t* hc, const PassParams& pp) {
+  st->camera_rays += hc[cnt_rays(0)];
+  for (uint32_t b = 1; b <= pp.max_depth; ++b) st->extension_rays += hc[cnt_rays(b)];
+  for (uint32_t b = 0; b < pp.max_depth; ++b) st->shadow_rays += hc[cnt_shadow(b)];
+  st->n_intersect_launches += pp.max_depth + 1;
+}
+
+// One pass on lane li: S samples of every pixel through all bounces, then
+// the film (after film_wait, if given: the previous pass's film on the other
+// lane). With counters_out the pass's counters are copied there
+// asynchronously and the caller adds them to the stats after the frame;
+// otherwise the pass is waited for and its counts are added here.
+int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_frame_stats* st, bool count,
+             hipEvent_t film_wait = nullptr, uint32_t* counters_out = nullptr) {
+  Lane& lane = r->lane[li];
+  hipStream_t s = lane.s;
   uint32_t n = pp.npix * pp.spp_in_pass;
   DevScene sc = r->dev_scene();
-  Queues q = r->queues();
+  sc.bvh.spill = lane.spill.p;
+  Queues q = r->queues(li);
   int g = grid_for(n);
   int gp = r->trav_grid(n);
   const uint32_t n_slots = pp.nrc ? pp.n_train * (pp.max_depth + 1u) : 0u;
@@ -1001,8 +1061,8 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
   // add to the radiance it moves, and it rewrites the shadow queue).
   const bool overlap = !timing;
   DevBvh shadow_bvh = sc.bvh;
-  shadow_bvh.spill = r->spill2.p;
-  hipStream_t s2 = r->stream2;
+  shadow_bvh.spill = lane.spill2.p;
+  hipStream_t s2 = lane.s2;
   for (uint32_t b = 0; b <= pp.max_depth; ++b) {
     uint32_t c = b & 1u;
     const uint32_t* cnt = q.counters + cnt_rays(b);
@@ -1014,7 +1074,7 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     else
       k_intersect<false><<<gp, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, cur, q.hit, q.stats);
     if (timing) HIPCHK(hipEventRecord(ev[1], s));
-    if (overlap && b > 0) HIPCHK(hipStreamWaitEvent(s, r->join_ev[b - 1], 0));
+    if (overlap && b > 0) HIPCHK(hipStreamWaitEvent(s, lane.join_ev[b - 1], 0));
     if (pp.nrc)
       k_shade<true><<<g, kBlock, 0, s>>>(sc, pp, b, q);
     else
@@ -1023,19 +1083,19 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     if (b < pp.max_depth) {
       hipStream_t ss = s;
       if (overlap) {
-        HIPCHK(hipEventRecord(r->fork_ev[b], s));
-        HIPCHK(hipStreamWaitEvent(s2, r->fork_ev[b], 0));
+        HIPCHK(hipEventRecord(lane.fork_ev[b], s));
+        HIPCHK(hipStreamWaitEvent(s2, lane.fork_ev[b], 0));
         ss = s2;
       }
       if (count)
         k_shadow<true><<<gp, kBlock, 0, ss>>>(shadow_bvh, q, b);
       else
         k_shadow<false><<<gp, kBlock, 0, ss>>>(shadow_bvh, q, b);
-      if (overlap) HIPCHK(hipEventRecord(r->join_ev[b], s2));
+      if (overlap) HIPCHK(hipEventRecord(lane.join_ev[b], s2));
     }
     if (timing) HIPCHK(hipEventRecord(ev[3], s));
   }
-  if (overlap && pp.max_depth > 0) HIPCHK(hipStreamWaitEvent(s, r->join_ev[pp.max_depth - 1], 0));
+  if (overlap && pp.max_depth > 0) HIPCHK(hipStreamWaitEvent(s, lane.join_ev[pp.max_depth - 1], 0));
   HIPCHK(hipGetLastError());
   float ms_nrc = 0.0f;
   bool trained = false;
@@ -1062,10 +1122,12 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     if (timing) HIPCHK(hipEventRecord(r->nev[1], s));
   }
   int gf = int((pp.npix + kBlock - 1) / kBlock);
+  if (film_wait) HIPCHK(hipStreamWaitEvent(s, film_wait, 0));
   if (fp64)
     k_film64<<<gf, kBlock, 0, s>>>(pp, q.Lbuf, r->film64.p, q.stats);
   else
     k_film32<<<gf, kBlock, 0, s>>>(pp, q.Lbuf, r->film.p, q.stats);
+  HIPCHK(hipEventRecord(lane.film_done, s));
   if (pp.nrc && n_slots) {
     // training: targets of this pass's suffix paths, then Adam steps, all on
     // the renderer's stream; the sample count is read back to size them
@@ -1109,6 +1171,11 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     }
   }
   HIPCHK(hipGetLastError());
+  if (counters_out) {
+    HIPCHK(hipMemcpyAsync(counters_out, q.counters, kCntWords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(lane.done, s));
+    return AQUA_OK;
+  }
   // Per-pass counters (ray counts) are read back once the pass is queued;
   // the copy waits on the stream, so passes stay ordered.
   uint32_t hc[kCntWords];
@@ -1131,10 +1198,7 @@ int run_pass(aqua_renderer* r, const PassParams& pp, bool fp64, aqua_frame_stats
     st->nrc_train_records += n_train_rec;
     if (pp.nrc && n_slots) st->nrc_loss = nrc_loss;
     st->ms_nrc += ms_nrc;
-    st->camera_rays += hc[cnt_rays(0)];
-    for (uint32_t b = 1; b <= pp.max_depth; ++b) st->extension_rays += hc[cnt_rays(b)];
-    for (uint32_t b = 0; b < pp.max_depth; ++b) st->shadow_rays += hc[cnt_shadow(b)];
-    st->n_intersect_launches += pp.max_depth + 1;
+    add_pass_counts(st, hc, pp);
     if (timing) {
       for (uint32_t b = 0; b <= pp.max_depth; ++b) {
         const hipEvent_t* ev = &r->kev[4 * b];
@@ -1242,12 +1306,10 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   r->cfg = *cfg;
   r->camera = scene->camera;
   r->opts.timing = 1;
-  HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&r->stream2, hipStreamNonBlocking));
-  r->fork_ev.resize(kMaxBounces + 1);
-  r->join_ev.resize(kMaxBounces + 1);
-  for (hipEvent_t& e : r->fork_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  for (hipEvent_t& e : r->join_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // lane 0: streams, counters, cursors and the traversal spill stacks of
+  // every thread of the (capped) traversal grids; lane 1 on first use
+  if (int e = r->lane[0].init(r->persistent_blocks, kMaxBounces)) return e;
+  r->stream = r->lane[0].s;
   HIPCHK(hipEventCreate(&r->ev0));
   HIPCHK(hipEventCreate(&r->ev1));
   r->kev.resize(4 * (kMaxBounces + 1));
@@ -1357,12 +1419,7 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   if (int e = r->lights.upload(lights.data(), lights.size(), s)) return e;
   if (int e = r->cdf.upload(cdf.data(), cdf.size(), s)) return e;
   if (int e = r->pmf.upload(pmf.data(), pmf.size(), s)) return e;
-  if (int e = r->counters.ensure(kCntWords)) return e;
-  if (int e = r->cursors.ensure(kCursorWords)) return e;
   if (int e = r->stats.ensure(kStWords)) return e;
-  // traversal spill stacks of every thread of the (capped) traversal grids
-  if (int e = r->spill.ensure(size_t(kSpillStack) * r->persistent_blocks * kBlock)) return e;
-  if (int e = r->spill2.ensure(size_t(kSpillStack) * r->persistent_blocks * kBlock)) return e;
   if (cfg->type == AQUA_INTEGRATOR_NRC) {
     if (cfg->nrc_precision > AQUA_NRC_FP8)
       return aqua::fail(AQUA_ERR_INVALID_ARG, "unknown nrc_precision %u", cfg->nrc_precision);
@@ -1422,18 +1479,48 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
   bool count = r->opts.count_traversal != 0;
   k_zero_stats<<<1, 64, 0, r->stream>>>(r->stats.p);
   HIPCHK(hipEventRecord(r->ev0, r->stream));
-  for (uint32_t s0 = 0; s0 < spp; s0 += S) {
+  // Two lanes: consecutive passes run concurrently on their own queues and
+  // streams, their films chained in pass order. Only without per-kernel
+  // timing, traversal counters or the cache (whose passes read counts back).
+  const uint32_t n_pass = (spp + S - 1) / S;
+  const bool two = n_pass > 1 && !r->opts.timing && !count && !(r->nrc && r->nrc_enabled);
+  if (two) {
+    if (int e = r->lane[1].init(r->persistent_blocks, kMaxBounces)) return e;
+    if (int e = r->lane[1].ensure(size_t(npix) * S)) return e;
+    if (n_pass * size_t(kCntWords) > r->pass_counters_cap) {
+      if (r->pass_counters) HIPCHK(hipHostFree(r->pass_counters));
+      r->pass_counters = nullptr;
+      r->pass_counters_cap = 0;
+      HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&r->pass_counters), n_pass * size_t(kCntWords) * 4,
+                           hipHostMallocDefault));
+      r->pass_counters_cap = n_pass * size_t(kCntWords);
+    }
+    // lane 1 starts after the frame's set-up on lane 0 (film and stats cleared)
+    HIPCHK(hipEventRecord(r->lane[0].done, r->stream));
+    HIPCHK(hipStreamWaitEvent(r->lane[1].s, r->lane[0].done, 0));
+  }
+  std::vector<PassParams> passes;
+  for (uint32_t k = 0, s0 = 0; s0 < spp; s0 += S, ++k) {
     pp.sample0 = spp0 + s0;
     pp.spp_in_pass = std::min(S, spp - s0);
     pp.frame = frame;
     pp.nrc = 0;
     if (!fp64 && r->nrc && r->nrc_enabled)
       if (int e = nrc_pass_params(r, pp.npix * pp.spp_in_pass, frame, pp.sample0, &pp)) return e;
-    if (int e = run_pass(r, pp, fp64, &st, count)) return e;
+    if (two) {
+      const int li = int(k & 1u);
+      hipEvent_t wait = k > 0 ? r->lane[li ^ 1].film_done : nullptr;
+      if (int e = run_pass(r, li, pp, fp64, &st, count, wait, r->pass_counters + size_t(k) * kCntWords)) return e;
+      passes.push_back(pp);
+    } else if (int e = run_pass(r, 0, pp, fp64, &st, count)) {
+      return e;
+    }
     ++st.n_passes;
   }
+  if (two) HIPCHK(hipStreamWaitEvent(r->stream, r->lane[1].done, 0));
   HIPCHK(hipEventRecord(r->ev1, r->stream));
   HIPCHK(hipEventSynchronize(r->ev1));
+  for (size_t k = 0; k < passes.size(); ++k) add_pass_counts(&st, r->pass_counters + k * kCntWords, passes[k]);
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
   st.ms_frame = ms;
@@ -1568,8 +1655,8 @@ static int upload_rays(aqua_renderer* r, uint32_t n, const float* rays) {
     d[i] = make_float4(x[4], x[5], x[6], 0.0f);
   }
   if (int e = r->ensure_queues(n)) return e;
-  HIPCHK(hipMemcpyAsync(r->ro[0].p, o.data(), n * sizeof(float4), hipMemcpyHostToDevice, r->stream));
-  HIPCHK(hipMemcpyAsync(r->rd[0].p, d.data(), n * sizeof(float4), hipMemcpyHostToDevice, r->stream));
+  HIPCHK(hipMemcpyAsync(r->lane[0].ro[0].p, o.data(), n * sizeof(float4), hipMemcpyHostToDevice, r->stream));
+  HIPCHK(hipMemcpyAsync(r->lane[0].rd[0].p, d.data(), n * sizeof(float4), hipMemcpyHostToDevice, r->stream));
   HIPCHK(hipStreamSynchronize(r->stream));
   return AQUA_OK;
 }
@@ -1606,7 +1693,7 @@ int aqua_trace_occluded(aqua_renderer* r, uint32_t n, const float* rays, uint8_t
   DevBuf<uint8_t> out;
   if (int e = out.ensure(n)) return e;
   hipStream_t s = r->stream;
-  k_occluded<<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, r->ro[0].p, r->rd[0].p, n, out.p);
+  k_occluded<<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, r->lane[0].ro[0].p, r->lane[0].rd[0].p, n, out.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(occluded, out.p, n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

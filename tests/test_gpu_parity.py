@@ -145,11 +145,12 @@ def test_render_interior_ggx_parity(gpu, tmp_path):
     np.testing.assert_array_equal(g, c)
 
 
-def test_overlapped_shadow_rays_do_not_change_the_film(gpu, box_room, box_room_oracle):
+def test_overlapped_kernels_and_passes_do_not_change_the_film(gpu, box_room, box_room_oracle):
     # without per-kernel timing the shadow rays of bounce b run on a second
-    # stream beside the closest-hit rays of bounce b + 1; the film must be the
-    # one of the serialized order, bit for bit (and the oracle's), also with
-    # the cache on
+    # stream beside the closest-hit rays of bounce b + 1, and consecutive
+    # passes run concurrently on two sets of queues; the film must be the one
+    # of the serialized order, bit for bit (and the oracle's), for the FP32
+    # and the FP64 film, and the cache (single lane) must still work
     cam = small_camera(box_room.camera(), 61, 35)
     r = aqua.Renderer(box_room, cfg(spp=6))
     r.set_opts(timing=True)
@@ -159,6 +160,13 @@ def test_overlapped_shadow_rays_do_not_change_the_film(gpu, box_room, box_room_o
     for _ in range(3):
         r.render(6, frame=3, camera=cam)
         np.testing.assert_array_equal(r.film(cam), serial)
+    # several passes: consecutive passes run concurrently on two sets of
+    # queues and streams, their films added in pass order
+    for spp_pass in (1, 2, 4):
+        r.set_opts(samples_per_pass=spp_pass, timing=False)
+        st = r.render(6, frame=3, camera=cam)
+        assert st.n_passes == (6 + spp_pass - 1) // spp_pass
+        np.testing.assert_array_equal(r.film(cam), serial)
     img_c, _ = box_room_oracle.render(aqua.camera_to_array(cam), 6, 5, frame=3)
     np.testing.assert_array_equal(serial, img_c)
     # large enough that both traversal grids are full and really run together
@@ -166,9 +174,21 @@ def test_overlapped_shadow_rays_do_not_change_the_film(gpu, box_room, box_room_o
     r2 = aqua.Renderer(box_room, cfg(spp=4, max_depth=8))
     r2.render(4, frame=7, camera=big)
     serial = r2.film(big)
-    r2.set_opts(timing=False)
-    r2.render(4, frame=7, camera=big)
+    st_serial = r2.render(4, frame=7, camera=big)
+    r2.set_opts(samples_per_pass=1, timing=False)
+    st = r2.render(4, frame=7, camera=big)
     np.testing.assert_array_equal(r2.film(big), serial)
+    assert (st.camera_rays, st.extension_rays, st.shadow_rays) == \
+        (st_serial.camera_rays, st_serial.extension_rays, st_serial.shadow_rays)
+    # FP64 film, resumed across calls, 2-sample passes
+    r3 = aqua.Renderer(box_room, cfg(spp=4))
+    r3.render_oracle(0, 5, frame=2, camera=cam)
+    r3.render_oracle(5, 4, frame=2, camera=cam)
+    serial64 = r3.oracle_film(cam)
+    r3.set_opts(samples_per_pass=2, timing=False)
+    r3.render_oracle(0, 5, frame=2, camera=cam)
+    r3.render_oracle(5, 4, frame=2, camera=cam)
+    np.testing.assert_array_equal(r3.oracle_film(cam), serial64)
     c = cfg(spp=4)
     c.type = aqua.INTEGRATOR_NRC
     c.nrc_train_paths = 256
