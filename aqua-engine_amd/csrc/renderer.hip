@@ -808,6 +808,24 @@ struct Lane {
   DevBuf<uint32_t> counters, cursors;
   DevBuf<uint32_t> spill;   // traversal spill stacks (kSpillStack words per grid thread)
   DevBuf<uint32_t> spill2;  // the same for the shadow kernels, which may run concurrently
+  // cache queries of this lane's pass: records, (throughput, path), outputs,
+  // Morton sort buffers
+  size_t nrc_cap = 0;
+  DevBuf<float4> qrec, qaux;
+  DevBuf<float> qout;
+  DevBuf<uint32_t> skey, sidx;  // Morton keys, slot order (double buffers)
+  DevBuf<uint8_t> stemp;
+
+  int ensure_nrc(size_t n) {
+    if (n <= nrc_cap) return AQUA_OK;
+    if (int e = qrec.ensure(4 * n)) return e;
+    if (int e = qaux.ensure(n)) return e;
+    if (int e = qout.ensure(3 * n)) return e;
+    if (int e = skey.ensure(2 * n)) return e;
+    if (int e = sidx.ensure(2 * n)) return e;
+    nrc_cap = n;
+    return AQUA_OK;
+  }
 
   int init(int persistent_blocks, int max_bounces) {
     if (s) return AQUA_OK;
@@ -869,7 +887,6 @@ struct aqua_renderer {
   std::vector<uint32_t> tri_order;
   DevBuf<float4> tris, nodes, mats, lights;
   DevBuf<float> cdf, pmf;
-  size_t nrc_cap = 0;  // query buffers sized for this many paths
   DevBuf<unsigned long long> stats;
   // pinned host copies of each pass's counters (read after the frame)
   uint32_t* pass_counters = nullptr;
@@ -887,17 +904,17 @@ struct aqua_renderer {
   bool nrc_enabled = true;
   uint64_t nrc_steps = 0;  // Adam steps the renderer has taken
   float bmin[3] = {0, 0, 0}, bscale[3] = {1, 1, 1};
-  DevBuf<float4> qrec, qaux, trec, taux, brec, btgt;
-  DevBuf<float> qout;
-  DevBuf<uint32_t> skey, sidx;  // query sort: Morton keys, slot order (double buffers)
-  DevBuf<uint8_t> stemp;
+  DevBuf<float4> trec, taux, brec, btgt;  // training slots of a frame's first pass, batch
   uint32_t batch_n = 0;
+  uint32_t train_nt = 0;           // training samples of the frame's first pass
+  hipEvent_t pass0_done = nullptr;  // two lanes: the frame's first pass is done
 
   ~aqua_renderer() {
     (void)hipSetDevice(device);
     if (lane[1].s) (void)hipStreamSynchronize(lane[1].s);
     if (stream) (void)hipStreamSynchronize(stream);
     if (pass_counters) (void)hipHostFree(pass_counters);
+    if (pass0_done) (void)hipEventDestroy(pass0_done);
     if (nrc) aqua_nrc_destroy(nrc);
     for (hipEvent_t e : kev) (void)hipEventDestroy(e);
     for (hipEvent_t e : nev)
@@ -948,26 +965,20 @@ struct aqua_renderer {
     q.counters = l.counters.p;
     q.cursors = l.cursors.p;
     q.stats = stats.p;
-    q.qrec = qrec.p;
-    q.qaux = qaux.p;
-    q.qkey = skey.p;
-    q.qidx = sidx.p;
+    q.qrec = l.qrec.p;
+    q.qaux = l.qaux.p;
+    q.qkey = l.skey.p;
+    q.qidx = l.sidx.p;
     q.trec = trec.p;
     q.taux = taux.p;
     return q;
   }
 
-  // Lane 0's queues (and the cache's query buffers) for n paths.
-  int ensure_queues(size_t n) {
-    if (int e = lane[0].ensure(n)) return e;
-    if (nrc && n > nrc_cap) {
-      if (int e = qrec.ensure(4 * n)) return e;
-      if (int e = qaux.ensure(n)) return e;
-      if (int e = qout.ensure(3 * n)) return e;
-      if (int e = skey.ensure(2 * n)) return e;
-      if (int e = sidx.ensure(2 * n)) return e;
-      nrc_cap = n;
-    }
+  // Lane li's queues (and the cache's query buffers) for n paths.
+  int ensure_queues(size_t n, int li = 0) {
+    if (int e = lane[li].ensure(n)) return e;
+    if (nrc)
+      if (int e = lane[li].ensure_nrc(n)) return e;
     return AQUA_OK;
   }
 };
@@ -1030,6 +1041,7 @@ int select_device(int device, int* cu_count = nullptr) {
 
 // Ray counts of one pass from its counters.
 void add_pass_counts(aqua_frame_stats* st, const uint32_t* hc, const PassParams& pp) {
+  if (pp.nrc) st->nrc_queries += hc[kCntQuery];
   st->camera_rays += hc[cnt_rays(0)];
   for (uint32_t b = 1; b <= pp.max_depth; ++b) st->extension_rays += hc[cnt_rays(b)];
   for (uint32_t b = 0; b < pp.max_depth; ++b) st->shadow_rays += hc[cnt_shadow(b)];
@@ -1098,9 +1110,7 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
   if (overlap && pp.max_depth > 0) HIPCHK(hipStreamWaitEvent(s, lane.join_ev[pp.max_depth - 1], 0));
   HIPCHK(hipGetLastError());
   float ms_nrc = 0.0f;
-  bool trained = false;
-  uint32_t n_query = 0, n_train_rec = 0;
-  float nrc_loss = 0.0f;
+  uint32_t n_query = 0;
   if (pp.nrc) {
     // cache queries: sort by position, infer, then add throughput x
     // prediction to each path; the query count stays on the device (no host
@@ -1110,15 +1120,15 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
     const uint32_t* perm = nullptr;
     size_t need = 0;
     HIPCHK(aqua::query_sort_storage(n, &need));
-    if (2 * size_t(n) > r->skey.n || 2 * size_t(n) > r->sidx.n)
+    if (2 * size_t(n) > lane.skey.n || 2 * size_t(n) > lane.sidx.n)
       return aqua::fail(AQUA_ERR_HIP, "query sort buffers too small for %u queries", n);
-    if (int e = r->stemp.ensure(need)) return e;  // rocPRIM's need is not monotone in n
+    if (int e = lane.stemp.ensure(need)) return e;  // rocPRIM's need is not monotone in n
     // k_shade wrote the queries' keys; the sort fills the padding slots
-    HIPCHK(aqua::query_sort(r->stemp.p, r->stemp.n, nullptr, q.counters + kCntQuery, n, r->skey.p, r->sidx.p,
-                            s, &perm));
-    if (int e = aqua::nrc_infer_device(r->nrc, n, r->qrec.p, r->qout.p, prec, s, q.counters + kCntQuery, perm))
+    HIPCHK(aqua::query_sort(lane.stemp.p, lane.stemp.n, nullptr, q.counters + kCntQuery, n, lane.skey.p,
+                            lane.sidx.p, s, &perm));
+    if (int e = aqua::nrc_infer_device(r->nrc, n, lane.qrec.p, lane.qout.p, prec, s, q.counters + kCntQuery, perm))
       return e;
-    k_nrc_apply<<<grid_for(n), kBlock, 0, s>>>(r->qaux.p, r->qout.p, q.counters + kCntQuery, q.Lbuf);
+    k_nrc_apply<<<grid_for(n), kBlock, 0, s>>>(lane.qaux.p, lane.qout.p, q.counters + kCntQuery, q.Lbuf);
     if (timing) HIPCHK(hipEventRecord(r->nev[1], s));
   }
   int gf = int((pp.npix + kBlock - 1) / kBlock);
@@ -1129,46 +1139,11 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
     k_film32<<<gf, kBlock, 0, s>>>(pp, q.Lbuf, r->film.p, q.stats);
   HIPCHK(hipEventRecord(lane.film_done, s));
   if (pp.nrc && n_slots) {
-    // training: targets of this pass's suffix paths, then Adam steps, all on
-    // the renderer's stream; the sample count is read back to size them
-    if (timing) HIPCHK(hipEventRecord(r->nev[2], s));
+    // training targets of this pass's suffix paths (into the batch); the
+    // Adam steps run once the frame's passes are done (nrc_train_frame)
     k_nrc_targets<<<std::min(grid_for(n_slots), 4096), kBlock, 0, s>>>(
         q.trec, q.taux, n_slots, pp.max_depth + 1u, pp.train_stride, pp.train_offset, q.Lbuf, r->brec.p,
         r->btgt.p, q.counters + kCntTrain);
-    HIPCHK(hipGetLastError());
-    uint32_t nt = 0;
-    HIPCHK(hipMemcpyAsync(&nt, q.counters + kCntTrain, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    n_train_rec = nt;
-    r->batch_n = nt;
-    // Adam steps: while the cache is young (its first 128 steps) four per
-    // frame, each on a quarter of the samples, so it converges within the
-    // first frames; after that one per frame on all of them (measured: the
-    // same image error as four, at a third of the cost). Only the last step
-    // waits, for its loss.
-    const uint32_t steps = r->nrc_steps < 128 ? 4u : 1u;
-    uint32_t per = (nt + steps - 1) / steps;
-    uint32_t last = steps;
-    for (uint32_t k = 0; k < steps && per > 0; ++k) {
-      uint32_t lo = k * per, cnt = std::min(per, nt - std::min(nt, lo));
-      if (cnt >= 32) last = k;
-    }
-    for (uint32_t k = 0; k <= last && last < steps; ++k) {
-      uint32_t lo = k * per, cnt = std::min(per, nt - std::min(nt, lo));
-      if (cnt < 32) continue;
-      aqua_nrc_stats ns;
-      if (int e = aqua::nrc_train_device(r->nrc, cnt, r->brec.p + 4 * size_t(lo), r->btgt.p + lo, 1,
-                                         k == last ? &ns : nullptr, s))
-        return e;
-      if (k == last) nrc_loss = ns.loss;
-      ++r->nrc_steps;
-    }
-    if (timing) HIPCHK(hipEventRecord(r->nev[3], s));
-    trained = true;
-    if (!std::isfinite(nrc_loss)) {
-      // a diverged cache is not used again: frames fall back to plain tracing
-      r->nrc_enabled = false;
-    }
   }
   HIPCHK(hipGetLastError());
   if (counters_out) {
@@ -1183,20 +1158,15 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
   HIPCHK(hipStreamSynchronize(s));
   if (pp.nrc) {
     n_query = hc[kCntQuery];
+    if (n_slots) r->train_nt = hc[kCntTrain];
     if (timing) {
       float ms = 0.0f;
       HIPCHK(hipEventElapsedTime(&ms, r->nev[0], r->nev[1]));
       ms_nrc += ms;
-      if (trained) {
-        HIPCHK(hipEventElapsedTime(&ms, r->nev[2], r->nev[3]));
-        ms_nrc += ms;
-      }
     }
   }
   if (st) {
-    st->nrc_queries += n_query;
-    st->nrc_train_records += n_train_rec;
-    if (pp.nrc && n_slots) st->nrc_loss = nrc_loss;
+    (void)n_query;
     st->ms_nrc += ms_nrc;
     add_pass_counts(st, hc, pp);
     if (timing) {
@@ -1212,6 +1182,50 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
       }
     }
   }
+  return AQUA_OK;
+}
+
+// The frame's Adam steps on the batch its first pass collected (nt samples),
+// on the renderer's stream once every pass's queries are answered (the steps
+// rewrite the weights those queries read). While the cache is young (its
+// first 128 steps) four steps per frame, each on a quarter of the samples,
+// so it converges within the first frames; after that one per frame on all
+// of them (measured: the same image error as four, at a third of the cost).
+// Only the last step waits, for its loss. A non-finite loss switches the
+// cache off: later frames render as the plain tracer.
+int nrc_train_frame(aqua_renderer* r, uint32_t nt, aqua_frame_stats* st) {
+  hipStream_t s = r->stream;
+  const bool timing = r->opts.timing != 0;
+  r->batch_n = nt;
+  if (timing) HIPCHK(hipEventRecord(r->nev[2], s));
+  const uint32_t steps = r->nrc_steps < 128 ? 4u : 1u;
+  uint32_t per = (nt + steps - 1) / steps;
+  uint32_t last = steps;
+  for (uint32_t k = 0; k < steps && per > 0; ++k) {
+    uint32_t lo = k * per, cnt = std::min(per, nt - std::min(nt, lo));
+    if (cnt >= 32) last = k;
+  }
+  float loss = 0.0f;
+  for (uint32_t k = 0; k <= last && last < steps; ++k) {
+    uint32_t lo = k * per, cnt = std::min(per, nt - std::min(nt, lo));
+    if (cnt < 32) continue;
+    aqua_nrc_stats ns;
+    if (int e = aqua::nrc_train_device(r->nrc, cnt, r->brec.p + 4 * size_t(lo), r->btgt.p + lo, 1,
+                                       k == last ? &ns : nullptr, s))
+      return e;
+    if (k == last) loss = ns.loss;
+    ++r->nrc_steps;
+  }
+  if (timing) {
+    HIPCHK(hipEventRecord(r->nev[3], s));
+    HIPCHK(hipEventSynchronize(r->nev[3]));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, r->nev[2], r->nev[3]));
+    st->ms_nrc += ms;
+  }
+  st->nrc_train_records += nt;
+  st->nrc_loss = loss;
+  if (!std::isfinite(loss)) r->nrc_enabled = false;  // a diverged cache is not used again
   return AQUA_OK;
 }
 
@@ -1480,13 +1494,16 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
   k_zero_stats<<<1, 64, 0, r->stream>>>(r->stats.p);
   HIPCHK(hipEventRecord(r->ev0, r->stream));
   // Two lanes: consecutive passes run concurrently on their own queues and
-  // streams, their films chained in pass order. Only without per-kernel
-  // timing, traversal counters or the cache (whose passes read counts back).
+  // streams (cache queries included), their films chained in pass order;
+  // the pass counters are read once the frame is done. Not with per-kernel
+  // timing or traversal counters, which want the passes one at a time.
   const uint32_t n_pass = (spp + S - 1) / S;
-  const bool two = n_pass > 1 && !r->opts.timing && !count && !(r->nrc && r->nrc_enabled);
+  const bool two = n_pass > 1 && !r->opts.timing && !count;
+  const bool cache = !fp64 && r->nrc && r->nrc_enabled;
   if (two) {
     if (int e = r->lane[1].init(r->persistent_blocks, kMaxBounces)) return e;
-    if (int e = r->lane[1].ensure(size_t(npix) * S)) return e;
+    if (int e = r->ensure_queues(size_t(npix) * S, 1)) return e;
+    if (!r->pass0_done) HIPCHK(hipEventCreateWithFlags(&r->pass0_done, hipEventDisableTiming));
     if (n_pass * size_t(kCntWords) > r->pass_counters_cap) {
       if (r->pass_counters) HIPCHK(hipHostFree(r->pass_counters));
       r->pass_counters = nullptr;
@@ -1500,17 +1517,21 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
     HIPCHK(hipStreamWaitEvent(r->lane[1].s, r->lane[0].done, 0));
   }
   std::vector<PassParams> passes;
+  bool train = false;  // a pass collected training samples
   for (uint32_t k = 0, s0 = 0; s0 < spp; s0 += S, ++k) {
     pp.sample0 = spp0 + s0;
     pp.spp_in_pass = std::min(S, spp - s0);
     pp.frame = frame;
     pp.nrc = 0;
-    if (!fp64 && r->nrc && r->nrc_enabled)
+    if (cache) {
       if (int e = nrc_pass_params(r, pp.npix * pp.spp_in_pass, frame, pp.sample0, &pp)) return e;
+      train = train || pp.n_train > 0;
+    }
     if (two) {
       const int li = int(k & 1u);
       hipEvent_t wait = k > 0 ? r->lane[li ^ 1].film_done : nullptr;
       if (int e = run_pass(r, li, pp, fp64, &st, count, wait, r->pass_counters + size_t(k) * kCntWords)) return e;
+      if (k == 0) HIPCHK(hipEventRecord(r->pass0_done, r->lane[0].s));
       passes.push_back(pp);
     } else if (int e = run_pass(r, 0, pp, fp64, &st, count)) {
       return e;
@@ -1518,6 +1539,15 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
     ++st.n_passes;
   }
   if (two) HIPCHK(hipStreamWaitEvent(r->stream, r->lane[1].done, 0));
+  if (train) {
+    // the first pass's training-sample count sizes the frame's Adam steps
+    uint32_t nt = r->train_nt;
+    if (two) {
+      HIPCHK(hipEventSynchronize(r->pass0_done));
+      nt = r->pass_counters[kCntTrain];
+    }
+    if (int e = nrc_train_frame(r, nt, &st)) return e;
+  }
   HIPCHK(hipEventRecord(r->ev1, r->stream));
   HIPCHK(hipEventSynchronize(r->ev1));
   for (size_t k = 0; k < passes.size(); ++k) add_pass_counts(&st, r->pass_counters + k * kCntWords, passes[k]);
