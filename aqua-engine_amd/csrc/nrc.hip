@@ -114,6 +114,12 @@ __device__ __forceinline__ float corner_w(const Cell& c, int k) {
   return (wx * wy) * wz;
 }
 
+typedef uint32_t u2a __attribute__((ext_vector_type(2), aligned(4)));
+__device__ __forceinline__ void unpack_entry(uint32_t u, float* a, float* b) {
+  *a = float(__builtin_bit_cast(_Float16, uint16_t(u & 0xFFFFu)));
+  *b = float(__builtin_bit_cast(_Float16, uint16_t(u >> 16)));
+}
+
 // Two FP16 features of one table entry.
 __device__ __forceinline__ void load_entry(const uint32_t* table16, size_t at, float* a, float* b) {
   uint32_t u = table16[at];
@@ -262,12 +268,25 @@ __device__ __forceinline__ void encode16(const FwdArgs& a, uint32_t T_, int g, b
     int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
     Cell c = cell_of(a.G, lv, px, py, pz);
     float s0 = 0.0f, s1 = 0.0f;
+    const uint32_t* tl = a.table16 + size_t(lv) * T_;
+    // Corners x and x + 1 are adjacent words (dense levels always; hashed
+    // levels unless x + 1 starts the next block): one 8-byte load fetches
+    // both, so a gather instruction touches half as many lines. Otherwise
+    // the second word is fetched on its own (the table has 2 spare words,
+    // so the wide load stays in bounds).
+    const bool pair = ((a.G.dense_mask >> lv) & 1u) || (c.c[0] & 3) != 3;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
-      float w = corner_w(c, k);
+    for (int k = 0; k < 8; k += 2) {
+      const int cy = c.c[1] + ((k >> 1) & 1), cz = c.c[2] + (k >> 2);
+      u2a u = *reinterpret_cast<const u2a*>(tl + grid_index(a.G, lv, c.c[0], cy, cz));
+      if (!pair) u.y = tl[grid_index(a.G, lv, c.c[0] + 1, cy, cz)];
       float e0, e1;
-      load_entry(a.table16, size_t(lv) * T_ + idx, &e0, &e1);
+      unpack_entry(u.x, &e0, &e1);
+      float w = corner_w(c, k);
+      s0 = __builtin_fmaf(w, e0, s0);
+      s1 = __builtin_fmaf(w, e1, s1);
+      unpack_entry(u.y, &e0, &e1);
+      w = corner_w(c, k + 1);
       s0 = __builtin_fmaf(w, e0, s0);
       s1 = __builtin_fmaf(w, e1, s1);
     }
@@ -1156,7 +1175,7 @@ int aqua_nrc_create(int device, const aqua_nrc_cfg* cfg, aqua_nrc** out) {
   if (int e = r->tgrad16.alloc(r->fine_entries())) return e;
   if (int e = r->tm.alloc(2 * r->entries)) return e;
   if (int e = r->tv.alloc(2 * r->entries)) return e;
-  if (int e = r->t16.alloc(r->entries)) return e;
+  if (int e = r->t16.alloc(r->entries + 2)) return e;  // + 2 spare words: see encode16
   if (int e = r->mmaster.alloc(kMlp)) return e;
   if (int e = r->mm.alloc(kMlp)) return e;
   if (int e = r->mv.alloc(kMlp)) return e;
@@ -1170,6 +1189,7 @@ int aqua_nrc_create(int device, const aqua_nrc_cfg* cfg, aqua_nrc** out) {
   if (int e = r->act_amax.alloc(8)) return e;
   if (int e = r->ensure_rows(std::max(32u, cfg->max_batch))) return e;
   k_nrc_init_table<<<blocks_for(r->entries), kBlock, 0, s>>>(r->tmaster.p, r->t16.p, r->entries, cfg->seed);
+  k_fill_u32<<<1, kBlock, 0, s>>>(r->t16.p + r->entries, 2, 0u);
   k_fill_f32<<<blocks_for(2 * r->entries), kBlock, 0, s>>>(r->tgrad.p, 2 * r->entries, 0.0f);
   k_fill_u32<<<blocks_for(r->fine_entries()), kBlock, 0, s>>>(r->tgrad16.p, r->fine_entries(), 0u);
   k_fill_f32<<<blocks_for(2 * r->entries), kBlock, 0, s>>>(r->tm.p, 2 * r->entries, 0.0f);

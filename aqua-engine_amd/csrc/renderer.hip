@@ -142,10 +142,13 @@ __global__ void k_camera(PassParams pp, Queues q) {
 // 64-aligned ranges; range x belongs to the blocks b with b % 8 == x, which
 // the hardware deals to XCD x, so a range's rays share that XCD's L2. A wave
 // first takes its static 64-ray chunk of its own range, then further chunks
-// from its range's cursor, then helps the other ranges through theirs. A
-// cursor is read before it is incremented, so exhausted ranges cost no
-// atomics and a short queue costs none at all; each cursor has its own
-// 128-byte line. The grid is a multiple of 8 blocks.
+// from its range's cursor, then helps the other ranges through theirs. The
+// own range's cursor is incremented without a look first: every claim is a
+// round trip to a line that all of the XCD's waves hit, and a read before
+// each increment had doubled that cost (it was most of the time of a queue
+// of short rays). Another range's cursor is read before it is incremented,
+// so exhausted ranges cost no atomics; each cursor has its own 128-byte line.
+// The grid is a multiple of 8 blocks.
 struct WaveWork {
   uint32_t n, per, waves_x, wave_x, x;
   uint32_t* cur;  // 8 cursors, kCursorStride words apart
@@ -178,8 +181,9 @@ __device__ __forceinline__ uint32_t ww_next(WaveWork& w) {
     uint32_t c = kMiss;
     if (__lane_id() == 0) {
       uint32_t* p = w.cur + r * kCursorStride;
-      uint32_t seen = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (lo + 64ull * (uint64_t(w.waves_x) + seen) < hi) c = atomicAdd(p, 1u);
+      if (w.phase == 1u ||
+          lo + 64ull * (uint64_t(w.waves_x) + __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < hi)
+        c = atomicAdd(p, 1u);
     }
     c = __shfl(c, 0);
     if (c != kMiss) {
