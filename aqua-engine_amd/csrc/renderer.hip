@@ -239,8 +239,9 @@ __device__ __forceinline__ uint32_t feed_take(LaneFeed& f, bool wants) {
 }
 
 // A wave keeps traversing while more than this many of its lanes hold a ray,
-// then refills the idle ones (until the queue is drained).
-constexpr uint32_t kRefillBelow = 40;
+// then refills the idle ones (until the queue is drained). With chunk claims
+// a single atomic each, refilling once 16 lanes are idle beats waiting for 24.
+constexpr uint32_t kRefillBelow = 48;
 
 template <bool COUNT>
 __global__ void __launch_bounds__(kBlock, 6) k_intersect(DevBvh bvh, const float4* __restrict__ ro,
