@@ -806,6 +806,8 @@ struct Lane {
   hipStream_t s2 = nullptr;
   std::vector<hipEvent_t> fork_ev, join_ev;
   hipEvent_t film_done = nullptr, done = nullptr;
+  // cache queries on s2 beside the training paths' last bounces
+  hipEvent_t q_fork = nullptr, q_done = nullptr;
   size_t cap = 0;
   DevBuf<float4> ro[2], rd[2], tp[2], L[2], sho, shd, shc, Lbuf;
   DevBuf<uint4> rng[2];
@@ -842,6 +844,8 @@ struct Lane {
     for (hipEvent_t& e : join_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&film_done, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&q_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&q_done, hipEventDisableTiming));
     if (int e = counters.ensure(kCntWords)) return e;
     if (int e = cursors.ensure(kCursorWords)) return e;
     if (int e = spill.ensure(size_t(kSpillStack) * persistent_blocks * kBlock)) return e;
@@ -874,6 +878,8 @@ struct Lane {
     for (hipEvent_t e : join_ev) (void)hipEventDestroy(e);
     if (film_done) (void)hipEventDestroy(film_done);
     if (done) (void)hipEventDestroy(done);
+    if (q_fork) (void)hipEventDestroy(q_fork);
+    if (q_done) (void)hipEventDestroy(q_done);
     if (s2) (void)hipStreamDestroy(s2);
     if (s) (void)hipStreamDestroy(s);
   }
@@ -1080,6 +1086,31 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
   DevBvh shadow_bvh = sc.bvh;
   shadow_bvh.spill = lane.spill2.p;
   hipStream_t s2 = lane.s2;
+  // Cache queries: sort by position, infer, then add throughput x prediction
+  // to each path; the query count stays on the device (no host round trip).
+  auto cache_queries = [&](hipStream_t qs) -> int {
+    const uint32_t* perm = nullptr;
+    size_t need = 0;
+    HIPCHK(aqua::query_sort_storage(n, &need));
+    if (2 * size_t(n) > lane.skey.n || 2 * size_t(n) > lane.sidx.n)
+      return aqua::fail(AQUA_ERR_HIP, "query sort buffers too small for %u queries", n);
+    if (int e = lane.stemp.ensure(need)) return e;  // rocPRIM's need is not monotone in n
+    // k_shade wrote the queries' keys; the sort fills the padding slots
+    HIPCHK(aqua::query_sort(lane.stemp.p, lane.stemp.n, nullptr, q.counters + kCntQuery, n, lane.skey.p,
+                            lane.sidx.p, qs, &perm));
+    if (int e = aqua::nrc_infer_device(r->nrc, n, lane.qrec.p, lane.qout.p, r->cfg.nrc_precision, qs,
+                                       q.counters + kCntQuery, perm))
+      return e;
+    k_nrc_apply<<<grid_for(n), kBlock, 0, qs>>>(lane.qaux.p, lane.qout.p, q.counters + kCntQuery, q.Lbuf);
+    return AQUA_OK;
+  };
+  // Without timing, the queries are final once the query vertex is shaded:
+  // they run on s2 while s traces the training paths' remaining bounces
+  // (whose shadow rays then stay on s). The apply touches only query paths'
+  // radiance, the training paths only their own.
+  const bool qfork = overlap && pp.nrc && pp.query_depth <= pp.max_depth;
+  auto shadow_on_s2 = [&](uint32_t b) { return overlap && !(qfork && b >= pp.query_depth); };
+  bool queries_forked = false;
   for (uint32_t b = 0; b <= pp.max_depth; ++b) {
     uint32_t c = b & 1u;
     const uint32_t* cnt = q.counters + cnt_rays(b);
@@ -1091,15 +1122,22 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
     else
       k_intersect<false><<<gp, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, cur, q.hit, q.stats);
     if (timing) HIPCHK(hipEventRecord(ev[1], s));
-    if (overlap && b > 0) HIPCHK(hipStreamWaitEvent(s, lane.join_ev[b - 1], 0));
+    if (b > 0 && shadow_on_s2(b - 1)) HIPCHK(hipStreamWaitEvent(s, lane.join_ev[b - 1], 0));
     if (pp.nrc)
       k_shade<true><<<g, kBlock, 0, s>>>(sc, pp, b, q);
     else
       k_shade<false><<<g, kBlock, 0, s>>>(sc, pp, b, q);
     if (timing) HIPCHK(hipEventRecord(ev[2], s));
+    if (qfork && b == pp.query_depth) {
+      HIPCHK(hipEventRecord(lane.q_fork, s));
+      HIPCHK(hipStreamWaitEvent(s2, lane.q_fork, 0));
+      if (int e = cache_queries(s2)) return e;
+      HIPCHK(hipEventRecord(lane.q_done, s2));
+      queries_forked = true;
+    }
     if (b < pp.max_depth) {
       hipStream_t ss = s;
-      if (overlap) {
+      if (shadow_on_s2(b)) {
         HIPCHK(hipEventRecord(lane.fork_ev[b], s));
         HIPCHK(hipStreamWaitEvent(s2, lane.fork_ev[b], 0));
         ss = s2;
@@ -1108,32 +1146,20 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
         k_shadow<true><<<gp, kBlock, 0, ss>>>(shadow_bvh, q, b);
       else
         k_shadow<false><<<gp, kBlock, 0, ss>>>(shadow_bvh, q, b);
-      if (overlap) HIPCHK(hipEventRecord(lane.join_ev[b], s2));
+      if (shadow_on_s2(b)) HIPCHK(hipEventRecord(lane.join_ev[b], s2));
     }
     if (timing) HIPCHK(hipEventRecord(ev[3], s));
   }
-  if (overlap && pp.max_depth > 0) HIPCHK(hipStreamWaitEvent(s, lane.join_ev[pp.max_depth - 1], 0));
+  if (pp.max_depth > 0 && shadow_on_s2(pp.max_depth - 1))
+    HIPCHK(hipStreamWaitEvent(s, lane.join_ev[pp.max_depth - 1], 0));
   HIPCHK(hipGetLastError());
   float ms_nrc = 0.0f;
   uint32_t n_query = 0;
-  if (pp.nrc) {
-    // cache queries: sort by position, infer, then add throughput x
-    // prediction to each path; the query count stays on the device (no host
-    // round trip)
-    uint32_t prec = r->cfg.nrc_precision;
+  if (queries_forked) {
+    HIPCHK(hipStreamWaitEvent(s, lane.q_done, 0));
+  } else if (pp.nrc) {
     if (timing) HIPCHK(hipEventRecord(r->nev[0], s));
-    const uint32_t* perm = nullptr;
-    size_t need = 0;
-    HIPCHK(aqua::query_sort_storage(n, &need));
-    if (2 * size_t(n) > lane.skey.n || 2 * size_t(n) > lane.sidx.n)
-      return aqua::fail(AQUA_ERR_HIP, "query sort buffers too small for %u queries", n);
-    if (int e = lane.stemp.ensure(need)) return e;  // rocPRIM's need is not monotone in n
-    // k_shade wrote the queries' keys; the sort fills the padding slots
-    HIPCHK(aqua::query_sort(lane.stemp.p, lane.stemp.n, nullptr, q.counters + kCntQuery, n, lane.skey.p,
-                            lane.sidx.p, s, &perm));
-    if (int e = aqua::nrc_infer_device(r->nrc, n, lane.qrec.p, lane.qout.p, prec, s, q.counters + kCntQuery, perm))
-      return e;
-    k_nrc_apply<<<grid_for(n), kBlock, 0, s>>>(lane.qaux.p, lane.qout.p, q.counters + kCntQuery, q.Lbuf);
+    if (int e = cache_queries(s)) return e;
     if (timing) HIPCHK(hipEventRecord(r->nev[1], s));
   }
   int gf = int((pp.npix + kBlock - 1) / kBlock);
