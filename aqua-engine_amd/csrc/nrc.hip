@@ -120,14 +120,6 @@ __device__ __forceinline__ void unpack_entry(uint32_t u, float* a, float* b) {
   *b = float(__builtin_bit_cast(_Float16, uint16_t(u >> 16)));
 }
 
-// Two FP16 features of one table entry.
-__device__ __forceinline__ void load_entry(const uint32_t* table16, size_t at, float* a, float* b) {
-  uint32_t u = table16[at];
-  _Float16 lo = __builtin_bit_cast(_Float16, uint16_t(u & 0xFFFFu));
-  _Float16 hi = __builtin_bit_cast(_Float16, uint16_t(u >> 16));
-  *a = float(lo);
-  *b = float(hi);
-}
 
 // SH basis l = 0..3 (16 coefficients) of a unit direction.
 __device__ __forceinline__ void sh16(float x, float y, float z, float* o) {
