@@ -189,13 +189,26 @@ def test_overlapped_kernels_and_passes_do_not_change_the_film(gpu, box_room, box
     r3.render_oracle(0, 5, frame=2, camera=cam)
     r3.render_oracle(5, 4, frame=2, camera=cam)
     np.testing.assert_array_equal(r3.oracle_film(cam), serial64)
-    c = cfg(spp=4)
+    # the cache: without timing, the queries' sort, inference and apply run on
+    # the second stream beside the training paths' last bounces, and passes
+    # alternate lanes. A fresh renderer's first frame depends only on the
+    # initial weights (training follows the film), so it must match the
+    # serialized frame bit for bit, in one pass and in several.
+    c = cfg(spp=4, max_depth=8)
     c.type = aqua.INTEGRATOR_NRC
     c.nrc_train_paths = 256
-    n = aqua.Renderer(box_room, c)
-    n.set_opts(timing=False)
-    st = n.render(4, frame=1, camera=cam)
-    assert st.nrc_queries > 0 and np.all(np.isfinite(n.film(cam)))
+    for view in (cam, big):
+        for spp_pass in (0, 1):
+            ref = aqua.Renderer(box_room, c)
+            ref.set_opts(samples_per_pass=spp_pass, timing=True)
+            st_ref = ref.render(4, frame=1, camera=view)
+            n = aqua.Renderer(box_room, c)
+            n.set_opts(samples_per_pass=spp_pass, timing=False)
+            st = n.render(4, frame=1, camera=view)
+            assert st.nrc_queries > 0 and st.nrc_queries == st_ref.nrc_queries
+            img = n.film(view)
+            assert np.all(np.isfinite(img))
+            np.testing.assert_array_equal(img, ref.film(view))
 
 
 def test_pass_split_does_not_change_the_film(gpu, box_room, box_room_oracle):
