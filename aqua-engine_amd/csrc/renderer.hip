@@ -1530,10 +1530,15 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
   HIPCHK(hipEventRecord(r->ev0, r->stream));
   // Two lanes: consecutive passes run concurrently on their own queues and
   // streams (cache queries included), their films chained in pass order;
-  // the pass counters are read once the frame is done. Not with per-kernel
-  // timing or traversal counters, which want the passes one at a time.
+  // the pass counters are read once the frame is done. Only for passes of
+  // fewer than kLanePaths paths, whose launches leave the GPU room for a
+  // second pass (1080p, 4 spp per pass: +12 %); passes of the default size
+  // fill it alone, and a second one only competes for it (16 spp per pass:
+  // -4 %). Not with per-kernel timing or traversal counters, which want the
+  // passes one at a time.
+  constexpr uint64_t kLanePaths = 16ull << 20;
   const uint32_t n_pass = (spp + S - 1) / S;
-  const bool two = n_pass > 1 && !r->opts.timing && !count;
+  const bool two = n_pass > 1 && uint64_t(npix) * S < kLanePaths && !r->opts.timing && !count;
   const bool cache = !fp64 && r->nrc && r->nrc_enabled;
   if (two) {
     if (int e = r->lane[1].init(r->persistent_blocks, kMaxBounces)) return e;
