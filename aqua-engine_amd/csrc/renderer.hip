@@ -1268,11 +1268,13 @@ int prepare_frame(aqua_renderer* r, const aqua_camera* cam_in, uint32_t spp, aqu
   uint64_t npix = uint64_t(cam->width) * cam->height;
   uint32_t s_pass = r->opts.samples_per_pass;
   if (s_pass == 0) {
-    // About 32 M paths per pass (16 spp at 1080p): bounce launches are then
-    // long enough that their ramp-up and tails stop mattering (1080p, 64
-    // spp: 8 M paths per pass ran 10 % slower). A pass holds ~340 bytes per
-    // path per lane in queues, ~22 GB for two lanes at this size.
-    uint64_t target = 32ull << 20;
+    // About 128 M paths per pass (64 spp at 1080p): bounce launches are then
+    // long enough that their ramp-up and tails stop mattering, and a frame
+    // has few passes, each with its own tail of short bounces (1080p, 64
+    // spp: 8 M paths per pass ran 16 % slower, 32 M 5 %). A pass holds ~230
+    // bytes per path in queues, ~340 with cache queries: 30-44 GB at this
+    // size, a sixth of the GPU's 288 GB.
+    uint64_t target = 128ull << 20;
     s_pass = uint32_t(std::max<uint64_t>(1, target / npix));
   }
   s_pass = std::min(s_pass, spp);

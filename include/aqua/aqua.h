@@ -202,7 +202,7 @@ typedef struct aqua_frame_stats {
 
 /* 32 bytes. */
 typedef struct aqua_renderer_opts {
-  uint32_t samples_per_pass;   /* 0 = choose (about 32 M paths in flight)     */
+  uint32_t samples_per_pass;   /* 0 = choose (about 128 M paths in flight)    */
   uint32_t count_traversal;    /* 1 = count nodes / triangles (slower)        */
   uint32_t timing;             /* 1 = per-kernel HIP-event timing             */
   uint32_t reserved[5];
