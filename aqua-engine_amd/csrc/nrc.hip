@@ -992,8 +992,11 @@ int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* 
 // `steps` Adam steps on n samples already on the device, stream-ordered on
 // `s`. With stats, the call waits for the steps and reports their timing and
 // the last loss; without, it returns at once (the loss stays on the device).
+// With before_update, the first weight update also waits for that event (the
+// weights may be read on another stream until then; the forward and
+// backward passes only read them).
 int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
-                     aqua_nrc_stats* stats, hipStream_t s) {
+                     aqua_nrc_stats* stats, hipStream_t s, hipEvent_t before_update) {
   if (n == 0 || steps == 0) return AQUA_OK;
   if (int e = nrc->ensure_rows(n)) return e;
   FwdArgs a = nrc->args(n, d_rec, d_tgt);
@@ -1039,6 +1042,8 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
       k_nrc_wgrad<AQUA_NRC_F16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),
                                                         nrc->partial.p);
     HIPCHK(hipEventRecord(nrc->ev[3], s));
+    // the weights may still be read on another stream until before_update
+    if (before_update && it == 0) HIPCHK(hipStreamWaitEvent(s, before_update, 0));
     k_nrc_adam_mlp<<<kMlp / 64, 256, 0, s>>>(nrc->partial.p, nchunks, nrc->mmaster.p, nrc->mm.p, nrc->mv.p, A);
     k_nrc_adam_table<<<blocks_for(nrc->entries), kBlock, 0, s>>>(
         nrc->tmaster.p, nrc->tgrad.p, nrc->tgrad16.p, nrc->entries - nrc->fine_entries(), nrc->tm.p, nrc->tv.p,
@@ -1295,7 +1300,8 @@ int aqua_nrc_train(aqua_nrc* nrc, uint32_t n, const float* records, const float*
   HIPCHK(hipSetDevice(nrc->device));
   if (int e = upload_records(nrc, n, records, targets)) return e;
   aqua_nrc_stats st;
-  if (int e = aqua::nrc_train_device(nrc, n, nrc->rec_buf.p, nrc->tgt_buf.p, steps, &st, nrc->stream)) return e;
+  if (int e = aqua::nrc_train_device(nrc, n, nrc->rec_buf.p, nrc->tgt_buf.p, steps, &st, nrc->stream, nullptr))
+    return e;
   if (stats) *stats = st;
   return AQUA_OK;
 }
@@ -1349,7 +1355,7 @@ int aqua_nrc_train_device(aqua_nrc* nrc, uint32_t n, const void* d_records, cons
   if (!nrc || (n && (!d_records || !d_targets))) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
   HIPCHK(hipSetDevice(nrc->device));
   if (int e = aqua::nrc_train_device(nrc, n, static_cast<const float4*>(d_records),
-                                     static_cast<const float4*>(d_targets), steps, stats, nrc->stream))
+                                     static_cast<const float4*>(d_targets), steps, stats, nrc->stream, nullptr))
     return e;
   HIPCHK(hipStreamSynchronize(nrc->stream));
   return AQUA_OK;

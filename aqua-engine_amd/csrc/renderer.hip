@@ -806,8 +806,9 @@ struct Lane {
   hipStream_t s2 = nullptr;
   std::vector<hipEvent_t> fork_ev, join_ev;
   hipEvent_t film_done = nullptr, done = nullptr;
-  // cache queries on s2 beside the training paths' last bounces
-  hipEvent_t q_fork = nullptr, q_done = nullptr;
+  // cache queries on s2 beside the training paths' last bounces; tail_done:
+  // those bounces are traced (the film, then on s2, waits for it)
+  hipEvent_t q_fork = nullptr, q_done = nullptr, tail_done = nullptr;
   size_t cap = 0;
   DevBuf<float4> ro[2], rd[2], tp[2], L[2], sho, shd, shc, Lbuf;
   DevBuf<uint4> rng[2];
@@ -846,6 +847,7 @@ struct Lane {
     HIPCHK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&q_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&q_done, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&tail_done, hipEventDisableTiming));
     if (int e = counters.ensure(kCntWords)) return e;
     if (int e = cursors.ensure(kCursorWords)) return e;
     if (int e = spill.ensure(size_t(kSpillStack) * persistent_blocks * kBlock)) return e;
@@ -880,6 +882,7 @@ struct Lane {
     if (done) (void)hipEventDestroy(done);
     if (q_fork) (void)hipEventDestroy(q_fork);
     if (q_done) (void)hipEventDestroy(q_done);
+    if (tail_done) (void)hipEventDestroy(tail_done);
     if (s2) (void)hipStreamDestroy(s2);
     if (s) (void)hipStreamDestroy(s);
   }
@@ -998,7 +1001,7 @@ namespace aqua {
 int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* d_out, uint32_t precision,
                      hipStream_t s, const uint32_t* d_count, const uint32_t* d_perm);
 int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
-                     aqua_nrc_stats* stats, hipStream_t s);
+                     aqua_nrc_stats* stats, hipStream_t s, hipEvent_t before_update = nullptr);
 }  // namespace aqua
 
 namespace {
@@ -1075,6 +1078,8 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
   int g = grid_for(n);
   int gp = r->trav_grid(n);
   const uint32_t n_slots = pp.nrc ? pp.n_train * (pp.max_depth + 1u) : 0u;
+  // the lane's previous film may still be reading its queues on s2
+  HIPCHK(hipStreamWaitEvent(s, lane.film_done, 0));
   k_reset<<<64, kBlock, 0, s>>>(q.counters, n, q.cursors, cursor_set(pp.max_depth + 1, 0));
   if (n_slots) k_clear_taux<<<int((n_slots + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.taux, n_slots);
   k_camera<<<g, kBlock, 0, s>>>(pp, q);
@@ -1155,20 +1160,27 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
   HIPCHK(hipGetLastError());
   float ms_nrc = 0.0f;
   uint32_t n_query = 0;
+  hipStream_t fs = s;  // the film's stream
   if (queries_forked) {
-    HIPCHK(hipStreamWaitEvent(s, lane.q_done, 0));
+    // the film follows the apply on s2 once the training paths are traced;
+    // s goes on to their targets, and the frame's training step can start
+    // while the queries are still inferred (it only reads the weights; its
+    // update waits for the film)
+    HIPCHK(hipEventRecord(lane.tail_done, s));
+    HIPCHK(hipStreamWaitEvent(s2, lane.tail_done, 0));
+    fs = s2;
   } else if (pp.nrc) {
     if (timing) HIPCHK(hipEventRecord(r->nev[0], s));
     if (int e = cache_queries(s)) return e;
     if (timing) HIPCHK(hipEventRecord(r->nev[1], s));
   }
   int gf = int((pp.npix + kBlock - 1) / kBlock);
-  if (film_wait) HIPCHK(hipStreamWaitEvent(s, film_wait, 0));
+  if (film_wait) HIPCHK(hipStreamWaitEvent(fs, film_wait, 0));
   if (fp64)
-    k_film64<<<gf, kBlock, 0, s>>>(pp, q.Lbuf, r->film64.p, q.stats);
+    k_film64<<<gf, kBlock, 0, fs>>>(pp, q.Lbuf, r->film64.p, q.stats);
   else
-    k_film32<<<gf, kBlock, 0, s>>>(pp, q.Lbuf, r->film.p, q.stats);
-  HIPCHK(hipEventRecord(lane.film_done, s));
+    k_film32<<<gf, kBlock, 0, fs>>>(pp, q.Lbuf, r->film.p, q.stats);
+  HIPCHK(hipEventRecord(lane.film_done, fs));
   if (pp.nrc && n_slots) {
     // training targets of this pass's suffix paths (into the batch); the
     // Adam steps run once the frame's passes are done (nrc_train_frame)
@@ -1224,7 +1236,7 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
 // of them (measured: the same image error as four, at a third of the cost).
 // Only the last step waits, for its loss. A non-finite loss switches the
 // cache off: later frames render as the plain tracer.
-int nrc_train_frame(aqua_renderer* r, uint32_t nt, aqua_frame_stats* st) {
+int nrc_train_frame(aqua_renderer* r, uint32_t nt, aqua_frame_stats* st, hipEvent_t before_update) {
   hipStream_t s = r->stream;
   const bool timing = r->opts.timing != 0;
   r->batch_n = nt;
@@ -1242,8 +1254,9 @@ int nrc_train_frame(aqua_renderer* r, uint32_t nt, aqua_frame_stats* st) {
     if (cnt < 32) continue;
     aqua_nrc_stats ns;
     if (int e = aqua::nrc_train_device(r->nrc, cnt, r->brec.p + 4 * size_t(lo), r->btgt.p + lo, 1,
-                                       k == last ? &ns : nullptr, s))
+                                       k == last ? &ns : nullptr, s, before_update))
       return e;
+    before_update = nullptr;  // later steps follow the first update on s
     if (k == last) loss = ns.loss;
     ++r->nrc_steps;
   }
@@ -1581,6 +1594,9 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
     ++st.n_passes;
   }
   if (two) HIPCHK(hipStreamWaitEvent(r->stream, r->lane[1].done, 0));
+  // the last pass's film (possibly still running on its lane's s2) follows
+  // every earlier film and every inference of the frame
+  const hipEvent_t last_film = r->lane[two ? (n_pass - 1u) & 1u : 0u].film_done;
   if (train) {
     // the first pass's training-sample count sizes the frame's Adam steps
     uint32_t nt = r->train_nt;
@@ -1588,8 +1604,9 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
       HIPCHK(hipEventSynchronize(r->pass0_done));
       nt = r->pass_counters[kCntTrain];
     }
-    if (int e = nrc_train_frame(r, nt, &st)) return e;
+    if (int e = nrc_train_frame(r, nt, &st, last_film)) return e;
   }
+  HIPCHK(hipStreamWaitEvent(r->stream, last_film, 0));
   HIPCHK(hipEventRecord(r->ev1, r->stream));
   HIPCHK(hipEventSynchronize(r->ev1));
   for (size_t k = 0; k < passes.size(); ++k) add_pass_counts(&st, r->pass_counters + k * kCntWords, passes[k]);
