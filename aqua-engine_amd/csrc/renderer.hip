@@ -795,11 +795,11 @@ struct DevBuf {
 
 // One set of path queues with its counters, work cursors, traversal spill
 // stacks and streams. A pass runs on one lane; without per-kernel timing,
-// consecutive passes of a frame alternate between two lanes and run
-// concurrently (only their film accumulations are ordered), so one pass's
+// consecutive passes of fewer than 16 M paths alternate between two lanes and
+// run concurrently (only their film accumulations are ordered), so one pass's
 // memory-bound shading overlaps the other's traversal.
 struct Lane {
-  hipStream_t s = nullptr;   // camera, closest hit, shading, film
+  hipStream_t s = nullptr;   // camera, closest hit, shading (and the film, unless on s2)
   // Shadow rays of bounce b run on s2, overlapped with the closest-hit rays
   // of bounce b + 1 (independent work that fills each other's tail); the
   // fork / join events order them against the shading kernels.
@@ -1229,8 +1229,10 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
 }
 
 // The frame's Adam steps on the batch its first pass collected (nt samples),
-// on the renderer's stream once every pass's queries are answered (the steps
-// rewrite the weights those queries read). While the cache is young (its
+// on the renderer's stream. The first step's forward and backward passes may
+// run while queries are still being inferred (they only read the weights);
+// its update waits for before_update, the frame's last film, which follows
+// every inference (the update rewrites the weights). While the cache is young (its
 // first 128 steps) four steps per frame, each on a quarter of the samples,
 // so it converges within the first frames; after that one per frame on all
 // of them (measured: the same image error as four, at a third of the cost).
