@@ -16,7 +16,7 @@ and the CPU restatement's rate on a bounded sample of the same frame. Unless
 --no-nrc, two cache legs follow: one training step on 2^18 samples
 ("nrc_train") and the ms/frame-to-relMSE-1e-2 comparison of the cache against
 the plain tracer on the box-room fly-through ("nrc_eval", whose "nrc_over_pt"
-is the north-star ratio).
+is the cache's time over the tracer's, querying at the second path vertex).
 """
 import argparse
 import importlib.util
@@ -315,7 +315,7 @@ def main():
         args.nrc_train = args.nrc_eval = True
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path tracer shards by pixel tiles across GPUs; see DESIGN.md)", file=sys.stderr)
+              "(the path tracer would shard by interleaved pixel tiles, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     if args.calibrate_copy:
