@@ -180,6 +180,16 @@ def test_overlapped_kernels_and_passes_do_not_change_the_film(gpu, box_room, box
     np.testing.assert_array_equal(r2.film(big), serial)
     assert (st.camera_rays, st.extension_rays, st.shadow_rays) == \
         (st_serial.camera_rays, st_serial.extension_rays, st_serial.shadow_rays)
+    # the default pass size puts a 64-spp frame of this size in one pass of
+    # 12.6 M paths; 16-spp passes (3.1 M paths) alternate lanes: same film
+    r2.set_opts(samples_per_pass=0, timing=False)
+    st = r2.render(64, frame=9, camera=big)
+    assert st.n_passes == 1
+    one_pass = r2.film(big)
+    r2.set_opts(samples_per_pass=16, timing=False)
+    st = r2.render(64, frame=9, camera=big)
+    assert st.n_passes == 4
+    np.testing.assert_array_equal(r2.film(big), one_pass)
     # FP64 film, resumed across calls, 2-sample passes
     r3 = aqua.Renderer(box_room, cfg(spp=4))
     r3.render_oracle(0, 5, frame=2, camera=cam)
