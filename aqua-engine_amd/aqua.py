@@ -163,6 +163,7 @@ _SIGS = {
     "aqua_nrc_param_counts": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
     "aqua_nrc_get_params": (C.c_int, [_P, _P, _P]),
     "aqua_nrc_set_params": (C.c_int, [_P, _P, _P]),
+    "aqua_nrc_fp8_scales": (C.c_int, [_P, _P, _P]),
     "aqua_nrc_infer": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P]),
     "aqua_nrc_train": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(NrcStats)]),
     "aqua_nrc_train_device": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(NrcStats)]),
@@ -469,6 +470,13 @@ class Nrc:
         table = np.ascontiguousarray(table, np.float32)
         mlp = np.ascontiguousarray(mlp, np.float32)
         _check(lib().aqua_nrc_set_params(self._h, _ptr(table), _ptr(mlp)))
+
+    def fp8_scales(self):
+        """(wscale[5], ascale[5]) of the FP8 query path."""
+        w = np.zeros(5, np.float32)
+        a = np.zeros(5, np.float32)
+        _check(lib().aqua_nrc_fp8_scales(self._h, _ptr(w), _ptr(a)))
+        return w, a
 
     def infer(self, records, precision=NRC_F16) -> np.ndarray:
         rec = np.ascontiguousarray(records, np.float32).reshape(-1, 16)

@@ -1223,6 +1223,15 @@ int aqua_nrc_get_params(aqua_nrc* nrc, float* table, float* mlp) {
   return AQUA_OK;
 }
 
+int aqua_nrc_fp8_scales(aqua_nrc* nrc, float* wscale, float* ascale) {
+  if (!nrc || !wscale || !ascale) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  HIPCHK(hipSetDevice(nrc->device));
+  HIPCHK(hipMemcpyAsync(wscale, nrc->wscale.p, 5 * sizeof(float), hipMemcpyDeviceToHost, nrc->stream));
+  HIPCHK(hipMemcpyAsync(ascale, nrc->ascale.p, 5 * sizeof(float), hipMemcpyDeviceToHost, nrc->stream));
+  HIPCHK(hipStreamSynchronize(nrc->stream));
+  return AQUA_OK;
+}
+
 int aqua_nrc_set_params(aqua_nrc* nrc, const float* table, const float* mlp) {
   if (!nrc || !table || !mlp) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
   HIPCHK(hipSetDevice(nrc->device));

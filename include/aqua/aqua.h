@@ -296,6 +296,13 @@ int aqua_nrc_param_counts(const aqua_nrc* nrc, uint64_t* n_table, uint32_t* n_ml
 int aqua_nrc_get_params(aqua_nrc* nrc, float* table, float* mlp);
 /* Replaces the parameters and clears the Adam state. */
 int aqua_nrc_set_params(aqua_nrc* nrc, const float* table, const float* mlp);
+/* The FP8 query path's per-layer scales (5 each): weights are quantised as
+ * e4m3(sat448(w * wscale[L])), layer inputs as e4m3(sat448(x * ascale[L])),
+ * and each layer's FP32 accumulator is multiplied by 1 / (ascale[L] *
+ * wscale[L]). wscale = 448 / amax |W_L| of the current weights; ascale =
+ * 448 / amax |layer input| seen by the last training forward pass (1 before
+ * any training). Diagnostics for checking the FP8 path. */
+int aqua_nrc_fp8_scales(aqua_nrc* nrc, float* wscale, float* ascale);
 /* Cache query: rgb[3i..3i+2] for each record (host arrays). */
 int aqua_nrc_infer(aqua_nrc* nrc, uint32_t n, const float* records, uint32_t precision, float* rgb);
 /* `steps` Adam steps on one batch (host arrays; targets 3 floats per row). */

@@ -211,6 +211,36 @@ def forward_torch(X, Ws, quant=None):
     return br(fr(h) @ Ws[N_HIDDEN])[:, :3]
 
 
+E4M3_MAX = 448.0
+
+
+def e4m3(x):
+    """Round float32 values to OCP FP8 e4m3 (round to nearest even, subnormals
+    kept; inputs are pre-saturated to +-448, where e4m3 has no infinity) and
+    back to float64."""
+    x32 = torch.as_tensor(x).to(torch.float32)
+    x32 = torch.clamp(x32, -E4M3_MAX, E4M3_MAX)
+    return x32.to(torch.float8_e4m3fn).to(torch.float64)
+
+
+def forward_fp8(X, Ws, wscale, ascale):
+    """The FP8 query path: per layer L, inputs e4m3(sat(x * ascale[L])) and
+    weights e4m3(sat(w * wscale[L])) multiplied exactly and accumulated (here
+    in float64; FP32 on the GPU), the sum scaled by float32 1 / (ascale[L] *
+    wscale[L]); ReLU between the hidden layers. Every product and scale is
+    formed in float32 as the kernels form it."""
+    h = X
+    for L, W in enumerate(Ws):
+        sa = np.float32(ascale[L])
+        sw = np.float32(wscale[L])
+        deq = float(np.float32(1.0) / np.float32(sa * sw))
+        hq = e4m3(h.to(torch.float32) * torch.tensor(sa, dtype=torch.float32))
+        Wq = e4m3(W.to(torch.float32) * torch.tensor(sw, dtype=torch.float32))
+        out = (hq @ Wq) * deq
+        h = torch.relu(out).to(torch.float32).to(torch.float64) if L < N_HIDDEN else out
+    return h[:, :3]
+
+
 def relative_l2(y, t):
     return (((y - t) ** 2) / (y.detach() ** 2 + 0.01)).mean()
 
@@ -218,15 +248,21 @@ def relative_l2(y, t):
 def _table_values(table_np, quant):
     t = np.asarray(table_np, np.float64)
     # the kernels gather from an FP16 copy of the table in every mode
+    # (the FP8 path included)
     return t.astype(np.float16).astype(np.float64) if quant is not None else t
 
 
-def infer(rec, table_np, mlp_np, quant=None):
-    """float64 forward pass -> (n, 3)."""
+def infer(rec, table_np, mlp_np, quant=None, fp8_scales=None):
+    """float64 forward pass -> (n, 3). quant "fp8" needs fp8_scales =
+    (wscale[5], ascale[5]), the scales the query kernel was given."""
     table = torch.from_numpy(_table_values(table_np, quant))
     Ws = [torch.from_numpy(W.astype(np.float64)) for W in unpack_mlp(mlp_np)]
     with torch.no_grad():
-        return forward_torch(features_torch(rec, table), Ws, quant).numpy()
+        X = features_torch(rec, table)
+        if quant == "fp8":
+            wscale, ascale = fp8_scales
+            return forward_fp8(X, Ws, wscale, ascale).numpy()
+        return forward_torch(X, Ws, quant).numpy()
 
 
 def loss_and_grads(rec, targets, table_np, mlp_np, dtype=torch.float64 if torch else None, quant=None):

@@ -148,15 +148,34 @@ def test_training_fits_a_smooth_field(gpu, prec):
     assert rel_l2(y, tgt) < 0.1
 
 
-def test_fp8_inference_after_calibration(gpu):
+# FP8 against the restatement that quantises to e4m3 with the kernel's own
+# per-layer scales: what remains is FP32 vs float64 accumulation (and the
+# FP32 features), which can move a value across an e4m3 rounding boundary
+# now and then -- a one-ulp flip of a 3-bit mantissa is 6 %, so the bound is
+# on the relative L2 over all outputs, not per element.
+FP8_TIGHT = 1e-2
+
+
+@pytest.mark.parametrize("seed", [8, 9])
+def test_fp8_inference_after_calibration(gpu, seed):
     # per-layer activation amax comes from the last training forward pass
-    nrc, table, mlp = make(aqua.NRC_BF16, lr=1e-9)
-    rec = records(2048, 8)
+    nrc, table, mlp = make(aqua.NRC_BF16, lr=1e-9, seed=seed)
+    rec = records(2048, seed)
     nrc.train(rec, np.zeros((2048, 3), np.float32), steps=1)
     t, m = nrc.params()
-    want = ref.infer(rec, t.reshape(16, 1 << LOG2, 2), m)
+    wscale, ascale = nrc.fp8_scales()
+    # the weight scales are 448 / amax |W| of the weights the kernel holds
+    Ws = ref.unpack_mlp(m)
+    for L in range(5):
+        assert wscale[L] == np.float32(448.0) / np.float32(np.abs(Ws[L]).max()), L
+        assert np.isfinite(ascale[L]) and ascale[L] > 0
     got = nrc.infer(rec, aqua.NRC_FP8)
-    assert rel_l2(got, want) < TOL[aqua.NRC_FP8], rel_l2(got, want)
+    emu = ref.infer(rec, t.reshape(16, 1 << LOG2, 2), m, quant="fp8", fp8_scales=(wscale, ascale))
+    exact = ref.infer(rec, t.reshape(16, 1 << LOG2, 2), m)
+    assert rel_l2(got, emu) < FP8_TIGHT, rel_l2(got, emu)
+    assert rel_l2(got, exact) < TOL[aqua.NRC_FP8], rel_l2(got, exact)
+    # the emulation is not the exact forward pass: it carries the e4m3 error
+    assert rel_l2(emu, exact) > 3 * rel_l2(got, emu)
 
 
 @pytest.mark.parametrize("prec", [aqua.NRC_F16, aqua.NRC_BF16])

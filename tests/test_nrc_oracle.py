@@ -163,3 +163,72 @@ def test_adam_matches_torch():
         tp.grad = torch.tensor(g)
         opt.step()
         np.testing.assert_allclose(p, tp.detach().numpy(), rtol=1e-12, atol=1e-14)
+
+
+def _e4m3_values():
+    """Every finite OCP e4m3 value from its bit fields: sign, 4-bit exponent
+    (bias 7), 3-bit mantissa; exponent 0 is subnormal (m / 8 * 2^-6); the
+    code S.1111.111 is NaN, so the largest finite value is 1.75 * 2^8 = 448."""
+    vals = []
+    for code in range(256):
+        s, e, m = code >> 7, (code >> 3) & 15, code & 7
+        if e == 15 and m == 7:
+            continue
+        mag = (m / 8.0) * 2.0 ** -6 if e == 0 else (1.0 + m / 8.0) * 2.0 ** (e - 7)
+        vals.append(-mag if s else mag)
+    return np.array(sorted(set(vals)))
+
+
+def test_e4m3_rounding_is_nearest_even_over_every_code():
+    grid = _e4m3_values()
+    assert len(grid) == 253 and grid.max() == 448.0 and grid[grid > 0].min() == 2.0 ** -9
+    # representable values round to themselves
+    np.testing.assert_array_equal(ref.e4m3(torch.tensor(grid)).numpy(), grid)
+    # random values: the nearest grid value, ties to the even mantissa
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-448, 448, 20000), rng.uniform(-1, 1, 20000) * 2.0 ** -5,
+                        (grid[:-1] + grid[1:]) / 2])  # exact midpoints
+    x = x.astype(np.float32).astype(np.float64)
+    got = ref.e4m3(torch.tensor(x)).numpy()
+    d = np.abs(x[:, None] - grid[None, :])
+    best = d.min(1)
+    assert np.all(np.abs(got - x) == best)
+    mid = np.isclose(d, best[:, None], rtol=0, atol=0).sum(1) == 2
+    assert mid.sum() >= 250
+    code_of = {}
+    for code in range(256):
+        e, m = (code >> 3) & 15, code & 7
+        if not (e == 15 and m == 7) and code < 128:
+            code_of[(m / 8.0) * 2.0 ** -6 if e == 0 else (1.0 + m / 8.0) * 2.0 ** (e - 7)] = code
+    for xv, g in zip(x[mid], got[mid]):
+        assert code_of[abs(float(g))] & 1 == 0, (xv, g)   # the even neighbour
+    # saturation: beyond the largest finite value
+    np.testing.assert_array_equal(ref.e4m3(torch.tensor([500.0, -1e6])).numpy(), [448.0, -448.0])
+
+
+def test_fp8_forward_is_exact_when_quantisation_is_exact():
+    # Weights and inputs that are e4m3 values after scaling lose nothing to
+    # the quantisation, so the FP8 path must equal the exact forward pass.
+    rng = np.random.default_rng(4)
+    grid = _e4m3_values()
+    small = grid[(np.abs(grid) >= 2.0 ** -3) & (np.abs(grid) <= 4.0)]
+    Ws = [torch.tensor(rng.choice(small, (64, 64)) / 8.0) for _ in range(4)]
+    Ws.append(torch.tensor(rng.choice(small, (64, 16)) / 8.0))
+    X = torch.tensor(rng.choice(small[small > 0], (32, 64)) / 4.0)
+    wscale = [8.0] * 5
+    ascale = [4.0, 1.0, 1.0, 1.0, 1.0]
+    got = ref.forward_fp8(X, Ws, wscale, ascale).numpy()
+    # layers 1..4 see the previous layer's output: keep them exact by
+    # quantising that output the same way in the exact reference
+    h = X
+    for L, W in enumerate(Ws):
+        hq = ref.e4m3(h.to(torch.float32) * ascale[L]) / ascale[L]
+        out = hq @ W
+        h = torch.relu(out) if L < 4 else out
+    np.testing.assert_allclose(got, h[:, :3].numpy(), rtol=1e-6, atol=1e-9)
+    # a power-of-two change of a scale keeps the quantisation lossless, and
+    # the dequantisation 1 / (ascale wscale) undoes it
+    np.testing.assert_allclose(ref.forward_fp8(X, Ws, [2.0] * 5, ascale).numpy(), got, rtol=1e-6, atol=1e-9)
+    # a scale that pushes the weights past 448 saturates them: results change
+    sat = ref.forward_fp8(X, Ws, [4096.0] * 5, ascale).numpy()
+    assert not np.allclose(sat, got, rtol=1e-2)

@@ -404,3 +404,132 @@ def test_light_table_by_hand(box_room, box_room_oracle):
     mats = box_room.materials()
     li = int(np.argmax(mats[:, 4]))
     np.testing.assert_allclose(mat_pdf[li], math.pi * lum / total, rtol=1e-4)
+
+
+# ------------------------------------------------------ point lights ------
+
+def _floor_and_camera(rho, look_at, tan_half=1e-5):
+    """A large diffuse floor at y = 0 facing +y and a 2x2 camera 3 units
+    above `look_at` looking straight down, so narrow that every pixel sees
+    the same floor point to ~1e-5."""
+    floor = np.float32([[-50, 0, -50, -50, 0, 50, 50, 0, 50], [-50, 0, -50, 50, 0, 50, 50, 0, -50]])
+    c = aqua.Camera()
+    x, z = look_at
+    c.cam_to_world[:] = [1, 0, 0, x, 0, 0, 1, 3, 0, -1, 0, z]
+    c.tan_half_fov = float(np.float32(tan_half))
+    c.aspect, c.width, c.height = 1.0, 2, 2
+    return floor, _mat12(base=(rho,) * 3, specular=0.0)[None], aqua.camera_to_array(c)
+
+
+@pytest.mark.parametrize("look_at,light", [((0.0, 0.0), (0.0, 2.0, 0.0)),      # on axis
+                                           ((0.0, 0.0), (1.5, 2.0, 0.5)),      # oblique
+                                           ((0.7, -0.4), (-1.0, 0.8, 1.2))])   # grazing-ish
+def test_point_light_radiance_closed_form(look_at, light):
+    # A delta light of intensity I (W/sr) over a Lambertian floor: the
+    # reflected radiance is rho / pi * I * cos(theta) / r^2 (inverse square
+    # law). Depth 1 = direct light only; the floor is the only geometry, so
+    # the continuation ray escapes.
+    rho, I = 0.6, 5.0
+    floor, mats, cam = _floor_and_camera(rho, look_at)
+    pl = np.float32([[*light, I, I, I]])
+    sc = oracle.Scene(floor, np.zeros(2, np.uint32), mats, pl)
+    img, st = sc.render(cam, 4, 1)
+    p = np.array([look_at[0], 0.0, look_at[1]])
+    dv = np.array(light) - p
+    r2 = float(dv @ dv)
+    cos = dv[1] / math.sqrt(r2)
+    expect = rho / math.pi * I * cos / r2
+    np.testing.assert_allclose(img.astype(np.float64), expect, rtol=5e-5)
+    assert st.shadow_rays == 4 * 4
+
+
+def test_two_point_lights_selected_by_power():
+    # Two grey point lights at mirror positions about the shaded point: each
+    # sample picks one light with probability proportional to its power and
+    # divides by that probability. With intensities 2 and 6 the two
+    # estimates I_i cos / (r^2 p_i) coincide (p_i = I_i / 8), so every sample
+    # equals the sum of both lights' radiance exactly. A missing 1 / p_sel
+    # (or a uniform pick) gives a different value on every sample.
+    rho = 0.5
+    floor, mats, cam = _floor_and_camera(rho, (0.0, 0.0))
+    pl = np.float32([[1.0, 2.0, 0.0, 2.0, 2.0, 2.0], [-1.0, 2.0, 0.0, 6.0, 6.0, 6.0]])
+    sc = oracle.Scene(floor, np.zeros(2, np.uint32), mats, pl)
+    n, pmf, *_ = sc.light_table()
+    assert n == 2
+    np.testing.assert_allclose(pmf[:2], [0.25, 0.75], rtol=1e-6)
+    img, _ = sc.render(cam, 64, 1)
+    r2, cos = 5.0, 2.0 / math.sqrt(5.0)
+    expect = rho / math.pi * (2.0 + 6.0) * cos / r2
+    np.testing.assert_allclose(img.astype(np.float64), expect, rtol=5e-5)
+
+
+# ----------------------------------------------------------- Fresnel ------
+
+def _fresnel_dielectric(cos_i, eta):
+    """Unpolarised Fresnel reflectance of a smooth dielectric (Fresnel's
+    equations; e.g. Born & Wolf, sec. 1.5.2)."""
+    sin_t2 = (1.0 - cos_i * cos_i) / (eta * eta)
+    cos_t = math.sqrt(max(0.0, 1.0 - sin_t2))
+    rs = (cos_i - eta * cos_t) / (cos_i + eta * cos_t)
+    rp = (eta * cos_i - cos_t) / (eta * cos_i + cos_t)
+    return 0.5 * (rs * rs + rp * rp)
+
+
+def _mirror_fresnel(mat, rough, cos_o):
+    """F(wo.h) the BSDF applies, read back from f in the mirror configuration
+    (wi = reflect(wo), so h = n and wo.h = cos_o): f = F D(1) G2 / (4 cos^2)
+    with D and Lambda the GGX terms pinned by the normalisation tests."""
+    s = math.sqrt(1.0 - cos_o * cos_o)
+    wo = np.float32([s, 0.0, cos_o])
+    wi = np.float32([-s, 0.0, cos_o])
+    n = np.float32([0, 0, 1])
+    f = oracle.bsdf_eval(mat, n, wo, wi)[:3].astype(np.float64)
+    D1, _ = oracle.ggx(rough, 1.0, 0.5)
+    _, lam = oracle.ggx(rough, 1.0, float(np.float32(cos_o)))
+    G2 = 1.0 / (1.0 + 2.0 * lam)
+    c = float(np.float32(cos_o))
+    return f * (4.0 * c * c) / (D1 * G2)
+
+
+def test_dielectric_fresnel_matches_ior_1_5():
+    # specular 0.5 is the usual dielectric default: F0 = 0.08 * 0.5 = 0.04,
+    # the normal-incidence reflectance of glass-like IOR 1.5,
+    # ((1.5 - 1) / (1.5 + 1))^2. With a black base only the specular lobe is
+    # left. Schlick's curve stays within 0.04 of Fresnel's equations at every
+    # angle (its largest gap, 0.036, is near 85 degrees) and reaches 1 at
+    # grazing incidence.
+    rough = 0.3
+    mat = _mat12(base=(0.0, 0.0, 0.0), metallic=0.0, roughness=rough, specular=0.5)
+    F_n = _mirror_fresnel(mat, rough, 1.0)
+    np.testing.assert_allclose(F_n, 0.04, rtol=2e-5)
+    assert abs(_fresnel_dielectric(1.0, 1.5) - 0.04) < 1e-15
+    for deg in (10, 30, 45, 60, 70, 80, 85, 88):
+        c = math.cos(math.radians(deg))
+        F = _mirror_fresnel(mat, rough, c)
+        assert np.all(np.abs(F - _fresnel_dielectric(c, 1.5)) < 0.04), (deg, F, _fresnel_dielectric(c, 1.5))
+    assert np.all(_mirror_fresnel(mat, rough, 0.002) > 0.98)
+
+
+@pytest.mark.parametrize("rough", [0.2, 0.5, 0.9])
+def test_metal_reflects_its_base_colour_at_normal_incidence(rough):
+    # Metallic = 1: the base colour IS the normal-incidence reflectance (F0),
+    # with no diffuse lobe, so f(n, n) = base * D(1) / 4 = base / (4 pi a^2)
+    # (G2 = 1 at normal incidence). Towards grazing, F rises to 1.
+    base = (0.9, 0.6, 0.3)
+    mat = _mat12(base=base, metallic=1.0, roughness=rough, specular=0.5)
+    n = np.float32([0, 0, 1])
+    f = oracle.bsdf_eval(mat, n, n, n)[:3].astype(np.float64)
+    a = max(float(np.float32(rough) * np.float32(rough)), 1e-3)
+    # D's FP32 form ((a^2 - 1) + 1 at n.h = 1) cancels: ~2^-23 / a^2 relative
+    tol = 1e-5 + 2.0 ** -22 / (a * a)
+    np.testing.assert_allclose(f, np.array(base) / (4 * math.pi * a * a), rtol=tol)
+    np.testing.assert_allclose(_mirror_fresnel(mat, rough, 1.0), base, rtol=2e-5)
+    Fg = _mirror_fresnel(mat, rough, 0.01)
+    assert np.all(Fg > 0.95) and np.all(Fg <= 1.0 + 1e-5), Fg
+    # half metallic: F0 is the average of the dielectric 0.04 and the base
+    half = _mat12(base=base, metallic=0.5, roughness=rough, specular=0.5)
+    spec_only = _mirror_fresnel(half, rough, 1.0)
+    diffuse = 0.5 * np.array(base) / math.pi * (1.0 - 0.04)   # (1 - metallic)(1 - Fd) base / pi
+    D1, _ = oracle.ggx(rough, 1.0, 0.5)
+    spec_only = spec_only - diffuse * 4.0 / D1
+    np.testing.assert_allclose(spec_only, 0.5 * 0.04 + 0.5 * np.array(base), rtol=2e-4)
