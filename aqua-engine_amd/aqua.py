@@ -40,13 +40,13 @@ class AquaError(RuntimeError):
 
 class LoadOpts(C.Structure):
     _fields_ = [("skip_missing", C.c_uint32), ("image_as_constant", C.c_uint32),
-                ("reserved", C.c_uint32 * 6)]
+                ("emission_from_mtl", C.c_uint32), ("reserved", C.c_uint32 * 5)]
 
 
 class Material(C.Structure):
     _fields_ = [("base_color", C.c_float * 3), ("metallic", C.c_float),
                 ("emission", C.c_float * 3), ("roughness", C.c_float),
-                ("specular", C.c_float), ("pad", C.c_float * 3)]
+                ("specular", C.c_float), ("base_color_tex", C.c_uint32), ("pad", C.c_float * 2)]
 
 
 class Camera(C.Structure):
@@ -64,7 +64,7 @@ class SceneInfo(C.Structure):
                 ("n_emissive_triangles", C.c_uint32), ("n_point_lights", C.c_uint32),
                 ("n_image_textures", C.c_uint32), ("n_vertices", C.c_uint32),
                 ("bounds_lo", C.c_float * 3), ("bounds_hi", C.c_float * 3),
-                ("reserved", C.c_uint32 * 2)]
+                ("n_images", C.c_uint32), ("has_texcoords", C.c_uint32)]
 
 
 class ProcgenParams(C.Structure):
@@ -91,7 +91,7 @@ class FrameStats(C.Structure):
                 ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64),
                 ("shadow_nodes_visited", C.c_uint64), ("shadow_tris_tested", C.c_uint64),
                 ("nrc_queries", C.c_uint32), ("nrc_train_records", C.c_uint32),
-                ("nrc_loss", C.c_float), ("ms_nrc", C.c_float), ("reserved", C.c_uint64 * 1)]
+                ("nrc_loss", C.c_float), ("ms_nrc", C.c_float), ("spill_node_visits", C.c_uint64)]
 
     @property
     def rays(self) -> int:
@@ -138,6 +138,14 @@ _SIGS = {
     "aqua_scene_get_materials": (C.c_int, [_P, _P]),
     "aqua_scene_get_camera": (C.c_int, [_P, C.POINTER(Camera)]),
     "aqua_scene_get_point_lights": (C.c_int, [_P, _P]),
+    "aqua_scene_get_texcoords": (C.c_int, [_P, _P]),
+    "aqua_scene_get_image": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _P]),
+    "aqua_scene_set_texcoords": (C.c_int, [_P, _P]),
+    "aqua_scene_add_image": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P, C.POINTER(C.c_uint32)]),
+    "aqua_image_load": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), _P]),
+    "aqua_image_write_png": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "aqua_image_write_pfm": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, _P]),
+    "aqua_film_tonemap": (C.c_int, [_P, C.c_float, _P]),
     "aqua_scene_destroy": (None, [_P]),
     "aqua_scene_bvh": (C.c_int, [_P, C.POINTER(C.c_uint32), _P, _P, C.POINTER(C.c_uint32)]),
     "aqua_scene_bvh_device": (C.c_int, [_P, C.POINTER(C.c_uint32), _P]),
@@ -221,8 +229,9 @@ class Scene:
         self._h = handle
 
     @classmethod
-    def load(cls, path: str, skip_missing: bool = False, image_as_constant: bool = False) -> "Scene":
-        o = LoadOpts(int(skip_missing), int(image_as_constant))
+    def load(cls, path: str, skip_missing: bool = False, image_as_constant: bool = False,
+             emission_from_mtl: bool = False) -> "Scene":
+        o = LoadOpts(int(skip_missing), int(image_as_constant), int(emission_from_mtl))
         h = C.c_void_p()
         _check(lib().aqua_scene_load(path.encode(), C.byref(o), C.byref(h)))
         return cls(h)
@@ -285,6 +294,35 @@ class Scene:
         _check(lib().aqua_scene_get_point_lights(self._h, _ptr(a) if n else None))
         return a
 
+    def texcoords(self) -> np.ndarray:
+        """float32[n_triangles, 6]: (u, v) of each corner, load order."""
+        uv = np.zeros((self.info().n_triangles, 6), np.float32)
+        _check(lib().aqua_scene_get_texcoords(self._h, _ptr(uv)))
+        return uv
+
+    def image(self, k: int) -> np.ndarray:
+        """uint8[h, w, 4]: image k's sRGB RGBA texels, row 0 = top."""
+        w, h = C.c_uint32(), C.c_uint32()
+        _check(lib().aqua_scene_get_image(self._h, k, C.byref(w), C.byref(h), None))
+        px = np.zeros((h.value, w.value, 4), np.uint8)
+        _check(lib().aqua_scene_get_image(self._h, k, C.byref(w), C.byref(h), _ptr(px)))
+        return px
+
+    def set_texcoords(self, uv: np.ndarray) -> None:
+        a = np.ascontiguousarray(uv, np.float32).reshape(-1, 6)
+        if len(a) != self.info().n_triangles:
+            raise ValueError("texcoords: one row of 6 per triangle")
+        _check(lib().aqua_scene_set_texcoords(self._h, _ptr(a)))
+
+    def add_image(self, rgba: np.ndarray) -> int:
+        """Adds an 8-bit sRGB RGBA image (h, w, 4); materials name it as base_color_tex = index + 1."""
+        a = np.ascontiguousarray(rgba, np.uint8)
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise ValueError("image must be (h, w, 4) uint8")
+        idx = C.c_uint32()
+        _check(lib().aqua_scene_add_image(self._h, a.shape[1], a.shape[0], _ptr(a), C.byref(idx)))
+        return idx.value
+
     def bvh_device(self) -> np.ndarray:
         """The same tree as stored on the device: uint32[n, 16] quantised 64-byte nodes."""
         n = C.c_uint32(0)
@@ -302,6 +340,27 @@ class Scene:
         order = np.zeros(self.info().n_triangles, np.uint32)
         _check(lib().aqua_scene_bvh(self._h, C.byref(n), _ptr(nodes), _ptr(order), C.byref(d)))
         return nodes, order, d.value
+
+
+def image_load(path: str) -> np.ndarray:
+    """uint8[h, w, 4]: a JPEG file decoded to RGBA, row 0 = top."""
+    w, h = C.c_uint32(), C.c_uint32()
+    _check(lib().aqua_image_load(path.encode(), C.byref(w), C.byref(h), None))
+    px = np.zeros((h.value, w.value, 4), np.uint8)
+    _check(lib().aqua_image_load(path.encode(), C.byref(w), C.byref(h), _ptr(px)))
+    return px
+
+
+def image_write_png(path: str, pixels: np.ndarray) -> None:
+    """Writes uint8 (h, w, 3) or (h, w, 4) pixels as PNG."""
+    a = np.ascontiguousarray(pixels, np.uint8)
+    _check(lib().aqua_image_write_png(path.encode(), a.shape[1], a.shape[0], a.shape[2], _ptr(a)))
+
+
+def image_write_pfm(path: str, rgb: np.ndarray) -> None:
+    """Writes float (h, w, 3) radiance as PFM."""
+    a = np.ascontiguousarray(rgb, np.float32)
+    _check(lib().aqua_image_write_pfm(path.encode(), a.shape[1], a.shape[0], _ptr(a)))
 
 
 def procgen(kind: int, out_dir: str, seed: int = 0, n_meshes: int = 0, target_tris: int = 0,
@@ -367,6 +426,13 @@ class Renderer:
         out = np.zeros((cam.height, cam.width, 3), np.float32)
         spp = C.c_uint32(0)
         _check(lib().aqua_film_read(self._h, _ptr(out), C.byref(spp)))
+        return out
+
+    def tonemap(self, exposure: float = 0.0, camera: Optional[Camera] = None) -> np.ndarray:
+        """uint8[h, w, 4]: the last frame x 2^exposure, clamped, sRGB-encoded (GPU)."""
+        cam = camera or self.scene_camera
+        out = np.zeros((cam.height, cam.width, 4), np.uint8)
+        _check(lib().aqua_film_tonemap(self._h, exposure, _ptr(out)))
         return out
 
     def render_oracle(self, spp_begin: int, spp_count: int, frame: int = 0,

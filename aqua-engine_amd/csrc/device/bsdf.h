@@ -27,6 +27,7 @@ struct Mat {
   float roughness;
   float specular;
   float light_pdf;  // pi * lum(Le) / total light power (area measure)
+  uint32_t tex;     // 0, or image index + 1 of a textured base colour
 };
 
 AQ_DEV Mat load_mat(const float4* mats, uint32_t m) {
@@ -38,6 +39,7 @@ AQ_DEV Mat load_mat(const float4* mats, uint32_t m) {
   r.roughness = b.w;
   r.specular = c.x;
   r.light_pdf = c.y;
+  r.tex = __float_as_uint(c.z);
   return r;
 }
 

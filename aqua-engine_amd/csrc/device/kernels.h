@@ -11,6 +11,7 @@
 
 #include "bsdf.h"
 #include "morton.h"
+#include "texture.h"
 #include "trace.h"
 
 namespace aqd {
@@ -41,6 +42,7 @@ enum : uint32_t {
   kStShNodes = 2,
   kStShTris = 3,
   kStNonfinite = 4,
+  kStSpill = 5,    // node visits with spilled stack entries (both traversal kernels)
   kStWords = 8
 };
 
@@ -51,6 +53,7 @@ struct DevScene {
   const float* light_cdf;
   const float* light_pmf;
   uint32_t n_area, n_lights;
+  DevTex tex;             // image textures (texels == nullptr: none)
 };
 
 struct CamParams {

@@ -106,6 +106,7 @@ struct DevBvh {
 
 struct TraceCount {
   uint32_t nodes, tris;
+  uint32_t deep;  // node visits while the ray's stack had entries spilled to global memory
 };
 
 // ---- traversal stack: a 16-entry ring in LDS (one column per thread) whose
@@ -303,6 +304,7 @@ AQ_DEV void closest_node(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCoun
 #pragma unroll
   for (int j = 3; j >= 1; --j)
     if (key[j] != 0xFFFFFFFFu) st_push(st, pick4(ref[0], ref[1], ref[2], ref[3], key[j] & 3u) >> 3);
+  if (COUNT && st.gsp > 0) cnt->deep++;
   if (key[0] != 0xFFFFFFFFu)
     c.node = pick4(ref[0], ref[1], ref[2], ref[3], key[0] & 3u) >> 3;
   else
@@ -396,6 +398,7 @@ AQ_DEV void any_node(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) 
     else
       st_push(st, ref[k] >> 3);
   }
+  if (COUNT && st.gsp > 0) cnt->deep++;
   a.node = next != kNoNode ? next : (st_empty(st) ? kNoNode : st_pop(st));
 }
 

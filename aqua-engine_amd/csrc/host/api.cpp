@@ -78,6 +78,47 @@ int aqua_scene_get_point_lights(const aqua_scene* s, aqua_point_light* out) {
   return AQUA_OK;
 }
 
+int aqua_scene_get_texcoords(const aqua_scene* s, float* uv) {
+  if (!s || !uv) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (s->uvs.empty()) return aqua::fail(AQUA_ERR_INVALID_ARG, "the scene has no texcoords");
+  memcpy(uv, s->uvs.data(), s->uvs.size() * sizeof(float));
+  return AQUA_OK;
+}
+
+int aqua_scene_get_image(const aqua_scene* s, uint32_t k, uint32_t* width, uint32_t* height, uint8_t* rgba) {
+  if (!s || !width || !height) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (k >= s->images.size()) return aqua::fail(AQUA_ERR_INVALID_ARG, "image %u of %zu", k, s->images.size());
+  const AquaImage& im = s->images[k];
+  *width = im.w;
+  *height = im.h;
+  if (rgba) memcpy(rgba, im.rgba.data(), im.rgba.size());
+  return AQUA_OK;
+}
+
+int aqua_scene_set_texcoords(aqua_scene* s, const float* uv) {
+  if (!s || !uv) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  const size_t n = 6 * s->mat_ids.size();
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(uv[i])) return aqua::fail(AQUA_ERR_NONFINITE, "texcoord %zu is not finite", i);
+  AQUA_GUARD(s->uvs.assign(uv, uv + n); aqua::finalize_scene_info(s); return AQUA_OK;)
+}
+
+int aqua_scene_add_image(aqua_scene* s, uint32_t width, uint32_t height, const uint8_t* rgba, uint32_t* index) {
+  if (!s || !rgba || !index) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (width == 0 || height == 0 || width > 32768 || height > 32768)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "image size %ux%u out of range", width, height);
+  AQUA_GUARD({
+    AquaImage im;
+    im.w = width;
+    im.h = height;
+    im.rgba.assign(rgba, rgba + size_t(width) * height * 4);
+    *index = uint32_t(s->images.size());
+    s->images.push_back(std::move(im));
+    aqua::finalize_scene_info(s);
+    return AQUA_OK;
+  })
+}
+
 void aqua_scene_destroy(aqua_scene* s) { delete s; }
 
 int aqua_scene_bvh(const aqua_scene* s, uint32_t* n_nodes, float* nodes, uint32_t* order, uint32_t* max_depth) {

@@ -9,6 +9,12 @@
 
 #include "aqua/aqua.h"
 
+// A decoded image texture: 8-bit sRGB-encoded RGBA, row 0 at the top.
+struct AquaImage {
+  uint32_t w = 0, h = 0;
+  std::vector<uint8_t> rgba;
+};
+
 struct aqua_scene {
   std::vector<float> verts;        // 9 floats per triangle, load order
   std::vector<uint32_t> mat_ids;   // one per triangle
@@ -16,14 +22,22 @@ struct aqua_scene {
   std::vector<std::string> material_names;
   aqua_camera camera{};
   std::vector<aqua_point_light> point_lights;
+  std::vector<float> uvs;          // 6 floats per triangle (u, v per corner), or empty
+  std::vector<AquaImage> images;   // textures named by aqua_material::base_color_tex
   aqua_scene_info info{};
 };
 
 namespace aqua {
 
+// JPEG (T.81 Huffman, baseline or progressive) -> 8-bit RGBA.
+int decode_jpeg(const std::vector<uint8_t>& data, uint32_t* w, uint32_t* h, std::vector<uint8_t>* rgba);
+// Reads and decodes an image file (JPEG).
+int read_image_file(const std::string& path, AquaImage* out);
+
 // Thread-local last error; returns `code` so call sites can `return fail(...)`.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
+const char* last_error_cstr();
 
 // Fills info (counts, bounds) from the arrays of a scene.
 void finalize_scene_info(aqua_scene* s);

@@ -7,9 +7,12 @@
 //     "lights": [ { "Point": { "pos", "emission" } } ],
 //     "shapes": [ { "Mesh": [ "<file>.mesh", { "Named": "<bsdf>" } ] } ] }
 //   <Tex> := {"Float": f} | {"Float3": [3]} | {"Srgb": [3]} | {"Image": path}
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <map>
 #include <memory>
@@ -85,6 +88,8 @@ void finalize_scene_info(aqua_scene* s) {
     if (mt.emission[0] > 0.f || mt.emission[1] > 0.f || mt.emission[2] > 0.f) ++ne;
   }
   in.n_emissive_triangles = ne;
+  in.n_images = uint32_t(s->images.size());
+  in.has_texcoords = s->uvs.empty() ? 0u : 1u;
   for (int k = 0; k < 3; ++k) {
     in.bounds_lo[k] = INFINITY;
     in.bounds_hi[k] = -INFINITY;
@@ -157,9 +162,35 @@ bool read_text(const std::string& path, std::string* out) {
   return true;
 }
 
+std::string slashes(std::string p) {
+  for (char& c : p)
+    if (c == '\\') c = '/';
+  return p;
+}
+
 struct Loader {
   const aqua_load_opts& opts;
+  std::string base;                           // directory of the scene JSON
+  std::vector<AquaImage>* images = nullptr;   // the scene's decoded images
+  std::map<std::string, uint32_t> image_of;   // path -> index (shared images load once)
   uint32_t n_images = 0;
+
+  // Loads (once per path) the image `rel` names; *index = its position.
+  int image(const std::string& rel, uint32_t* index) {
+    std::string file = slashes(rel);
+    std::string full = (!file.empty() && file[0] == '/') ? file : base + "/" + file;
+    auto it = image_of.find(full);
+    if (it != image_of.end()) {
+      *index = it->second;
+      return AQUA_OK;
+    }
+    AquaImage img;
+    if (int e = read_image_file(full, &img)) return e;
+    *index = uint32_t(images->size());
+    image_of[full] = *index;
+    images->push_back(std::move(img));
+    return AQUA_OK;
+  }
 
   int num(const Json* j, const char* what, double* v) {
     if (!j || !j->is_num()) return fail(AQUA_ERR_PARSE, "scene: '%s' must be a number", what);
@@ -175,8 +206,9 @@ struct Loader {
     }
     return AQUA_OK;
   }
-  // <Tex> -> linear RGB.
-  int tex_rgb(const Json* j, const char* what, double out[3]) {
+  // <Tex> -> linear RGB. `tex` (colours only) receives image index + 1 for an
+  // Image texture, whose mean linear colour goes to `out`.
+  int tex_rgb(const Json* j, const char* what, double out[3], uint32_t* tex = nullptr) {
     if (!j || j->kind != Json::Object || j->obj.size() != 1)
       return fail(AQUA_ERR_PARSE, "scene: texture '%s' must be a one-key object", what);
     const std::string& tag = j->obj[0].first;
@@ -191,11 +223,23 @@ struct Loader {
       if (int e = vec3(&v, what, out)) return e;
       for (int k = 0; k < 3; ++k) out[k] = srgb_to_linear(out[k]);
     } else if (tag == "Image") {
-      if (!opts.image_as_constant)
-        return fail(AQUA_ERR_UNSUPPORTED, "scene: texture '%s' is an Image; image textures are not "
-                    "supported (set image_as_constant to load it as 0.5 grey)", what);
+      if (v.kind != Json::String) return fail(AQUA_ERR_PARSE, "scene: Image '%s' must name a file", what);
       ++n_images;
-      out[0] = out[1] = out[2] = 0.5;
+      if (opts.image_as_constant) {
+        out[0] = out[1] = out[2] = 0.5;
+        return AQUA_OK;
+      }
+      if (!tex)
+        return fail(AQUA_ERR_UNSUPPORTED, "scene: '%s' is an Image; only colours may be image textures", what);
+      uint32_t k = 0;
+      if (int e = image(v.str, &k)) return e;
+      *tex = k + 1;
+      const AquaImage& im = (*images)[k];
+      double lut[256], sum[3] = {0, 0, 0};
+      for (int i = 0; i < 256; ++i) lut[i] = srgb_to_linear(i / 255.0);
+      for (size_t p = 0; p < size_t(im.w) * im.h; ++p)
+        for (int c = 0; c < 3; ++c) sum[c] += lut[im.rgba[4 * p + size_t(c)]];
+      for (int c = 0; c < 3; ++c) out[c] = sum[c] / double(size_t(im.w) * im.h);
     } else {
       return fail(AQUA_ERR_PARSE, "scene: unknown texture tag '%s' in '%s'", tag.c_str(), what);
     }
@@ -216,7 +260,7 @@ struct Loader {
       return fail(AQUA_ERR_PARSE, "scene: bsdf '%s' must be {\"Principled\": {...}}", name.c_str());
     memset(m, 0, sizeof *m);
     double rgb[3];
-    if (int e = tex_rgb(p->get("color"), "color", rgb)) return e;
+    if (int e = tex_rgb(p->get("color"), "color", rgb, &m->base_color_tex)) return e;
     for (int k = 0; k < 3; ++k) m->base_color[k] = float(rgb[k]);
     double v = 0.0;
     if (int e = tex_scalar(p->get("metallic"), "metallic", &v)) return e;
@@ -254,6 +298,38 @@ std::string dir_of(const std::string& path) {
   return s == std::string::npos ? std::string(".") : path.substr(0, s);
 }
 
+// `Ke` (emitted radiance, linear RGB) of every `newmtl` in the .mtl files of
+// directory `dir` (Wavefront MTL: `newmtl <name>` opens a material, `Ke r g
+// b` sets its emission; living_room.mtl:11-13 BackLight Ke 2 2 2).
+int read_mtl_emission(const std::string& dir, std::map<std::string, std::array<double, 3>>* out) {
+  std::error_code ec;
+  std::vector<std::string> files;
+  for (const auto& de : std::filesystem::directory_iterator(dir, ec))
+    if (de.path().extension() == ".mtl") files.push_back(de.path().string());
+  if (ec) return fail(AQUA_ERR_IO, "cannot list '%s' for .mtl files", dir.c_str());
+  std::sort(files.begin(), files.end());
+  for (const std::string& f : files) {
+    std::string text;
+    if (!read_text(f, &text)) return fail(AQUA_ERR_IO, "cannot read '%s'", f.c_str());
+    std::istringstream in(text);
+    std::string line, cur;
+    while (std::getline(in, line)) {
+      std::istringstream ls(line);
+      std::string key;
+      ls >> key;
+      if (key == "newmtl") {
+        ls >> cur;
+      } else if (key == "Ke" && !cur.empty()) {
+        std::array<double, 3> k{0, 0, 0};
+        ls >> k[0] >> k[1] >> k[2];
+        if (!ls) return fail(AQUA_ERR_PARSE, "%s: bad Ke line for '%s'", f.c_str(), cur.c_str());
+        (*out)[cur] = k;
+      }
+    }
+  }
+  return AQUA_OK;
+}
+
 }  // namespace
 
 int load_scene(const char* path, const aqua_load_opts* opts_in, aqua_scene** out) {
@@ -267,7 +343,8 @@ int load_scene(const char* path, const aqua_load_opts* opts_in, aqua_scene** out
   if (root.kind != Json::Object) return fail(AQUA_ERR_PARSE, "%s: top level must be an object", path);
 
   auto s = std::make_unique<aqua_scene>();
-  Loader L{opts};
+  std::string base = dir_of(path);
+  Loader L{opts, base, &s->images, {}, 0};
   std::map<std::string, uint32_t> by_name;
 
   const Json* bsdfs = root.get("named_bsdfs");
@@ -279,6 +356,16 @@ int load_scene(const char* path, const aqua_load_opts* opts_in, aqua_scene** out
     by_name[kv.first] = uint32_t(s->materials.size());
     s->materials.push_back(m);
     s->material_names.push_back(kv.first);
+  }
+  if (opts.emission_from_mtl) {
+    std::map<std::string, std::array<double, 3>> ke;
+    if (int e = read_mtl_emission(base, &ke)) return e;
+    for (size_t m = 0; m < s->materials.size(); ++m) {
+      aqua_material& mt = s->materials[m];
+      auto it = ke.find(s->material_names[m]);
+      if (it == ke.end() || mt.emission[0] > 0.f || mt.emission[1] > 0.f || mt.emission[2] > 0.f) continue;
+      for (int k = 0; k < 3; ++k) mt.emission[k] = float(it->second[size_t(k)] > 0.0 ? it->second[size_t(k)] : 0.0);
+    }
   }
 
   const Json* cam = root.get("camera");
@@ -336,7 +423,7 @@ int load_scene(const char* path, const aqua_load_opts* opts_in, aqua_scene** out
   const Json* shapes = root.get("shapes");
   if (!shapes || shapes->kind != Json::Array)
     return fail(AQUA_ERR_PARSE, "%s: 'shapes' must be an array", path);
-  std::string base = dir_of(path);
+  const bool want_uv = !s->images.empty();  // texcoords only matter to image textures
   uint32_t n_loaded = 0, n_skipped = 0, n_vertices = 0;
   for (const Json& sh : shapes->arr) {
     const Json* mesh = sh.get("Mesh");
@@ -349,9 +436,7 @@ int load_scene(const char* path, const aqua_load_opts* opts_in, aqua_scene** out
     auto it = by_name.find(named->str);
     if (it == by_name.end())
       return fail(AQUA_ERR_PARSE, "%s: unknown bsdf '%s'", path, named->str.c_str());
-    std::string file = mesh->arr[0].str;
-    for (char& c : file)
-      if (c == '\\') c = '/';
+    std::string file = slashes(mesh->arr[0].str);
     std::string full = (!file.empty() && file[0] == '/') ? file : base + "/" + file;
     FILE* probe = fopen(full.c_str(), "rb");
     if (!probe) {
@@ -365,10 +450,15 @@ int load_scene(const char* path, const aqua_load_opts* opts_in, aqua_scene** out
     MeshData md;
     if (int e = read_mesh_bson(full, &md)) return e;
     size_t nt = md.indices.size() / 3;
+    const bool has_uv = md.texcoords.size() == 2 * (md.vertices.size() / 3);
     for (size_t t = 0; t < nt; ++t) {
       for (int k = 0; k < 3; ++k) {
         size_t vi = size_t(md.indices[3 * t + size_t(k)]);
         for (int c = 0; c < 3; ++c) s->verts.push_back(float(md.vertices[3 * vi + size_t(c)]));
+        if (want_uv) {
+          s->uvs.push_back(has_uv ? float(md.texcoords[2 * vi]) : 0.0f);
+          s->uvs.push_back(has_uv ? float(md.texcoords[2 * vi + 1]) : 0.0f);
+        }
       }
       s->mat_ids.push_back(it->second);
     }

@@ -243,10 +243,40 @@ __device__ __forceinline__ uint32_t feed_take(LaneFeed& f, bool wants) {
 // a single atomic each, refilling once 16 lanes are idle beats waiting for 24.
 constexpr uint32_t kRefillBelow = 48;
 
-template <bool COUNT>
-__global__ void __launch_bounds__(kBlock, 6) k_intersect(DevBvh bvh, const float4* __restrict__ ro,
-                                                      const float4* __restrict__ rd,
-                                                      const uint32_t* count, uint32_t* cursor, uint2* hit,
+// Where the closest-hit kernel reads its rays and writes its answers.
+// Path queues: origin / direction arrays, and the hit record (triangle slot,
+// t) the shading kernel reads.
+struct QueueIO {
+  const float4* __restrict__ ro;
+  const float4* __restrict__ rd;
+  uint2* hit;
+  __device__ __forceinline__ void load(uint32_t r, float4* o, float4* d) const {
+    *o = ro[r];
+    *d = rd[r];
+  }
+  __device__ __forceinline__ void store(uint32_t r, const ClosestRay& c) const {
+    hit[r] = make_uint2(c.best_slot, __float_as_uint(c.tbest));
+  }
+};
+// Caller rays, 8 floats each (origin, tmax, direction, -), answered straight
+// into the caller's arrays: triangle id (load order; kMiss = none) and t
+// (tmax for a miss), the values aqua_trace_closest returns.
+struct RayIO {
+  const float4* __restrict__ rays;
+  uint32_t* ids;
+  float* ts;
+  __device__ __forceinline__ void load(uint32_t r, float4* o, float4* d) const {
+    *o = rays[2 * size_t(r)];
+    *d = rays[2 * size_t(r) + 1];
+  }
+  __device__ __forceinline__ void store(uint32_t r, const ClosestRay& c) const {
+    ids[r] = c.best_id;
+    ts[r] = c.tbest;
+  }
+};
+
+template <bool COUNT, class IO>
+__global__ void __launch_bounds__(kBlock, 6) k_intersect(DevBvh bvh, IO io, const uint32_t* count, uint32_t* cursor,
                                                       unsigned long long* stats) {
   __shared__ float4 lds_top[4 * kTopNodes];
   stage_top(bvh, lds_top);
@@ -254,7 +284,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_intersect(DevBvh bvh, const float
   TStack st;
   st_init(st, lds_stack, bvh.spill);
   const uint32_t n = *count;
-  TraceCount tc{0u, 0u};
+  TraceCount tc{0u, 0u, 0u};
   LaneFeed feed = feed_init(n, cursor);
   ClosestRay cr;
   bool active = false;
@@ -263,7 +293,8 @@ __global__ void __launch_bounds__(kBlock, 6) k_intersect(DevBvh bvh, const float
     if (feed.more) {
       uint32_t r = feed_take(feed, !active);
       if (r != kMiss) {
-        float4 o = ro[r], d = rd[r];
+        float4 o, d;
+        io.load(r, &o, &d);
         closest_begin(cr, xyz(o), xyz(d), o.w);
         ray = r;
         active = true;
@@ -275,7 +306,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_intersect(DevBvh bvh, const float
     while (live > keep) {
       closest_wave_step<COUNT>(bvh, st, cr, active, &tc);
       if (active && closest_done(cr)) {
-        hit[ray] = make_uint2(cr.best_slot, __float_as_uint(cr.tbest));
+        io.store(ray, cr);
         active = false;
       }
       live = uint32_t(__popcll(__ballot(active)));
@@ -284,6 +315,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_intersect(DevBvh bvh, const float
   if (COUNT) {
     atomicAdd(&stats[kStNodes], (unsigned long long)tc.nodes);
     atomicAdd(&stats[kStTris], (unsigned long long)tc.tris);
+    atomicAdd(&stats[kStSpill], (unsigned long long)tc.deep);
   }
 }
 
@@ -332,6 +364,11 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
         v3 v0 = xyz(p0), v1 = xyz(p1), v2 = xyz(p2);
         Mat m = load_mat(sc.mats, __float_as_uint(p1.w));
         v3 p = v3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z};
+        if (m.tex) {
+          float b1, b2;
+          bary_of(v0, v1, v2, p, &b1, &b2);
+          m.base = tex_base(sc.tex, h.x, m.tex - 1u, b1, b2);
+        }
         v3 wo = neg(d);
         v3 ng = normalize(cross(sub(v1, v0), sub(v2, v0)));
         float cg = dot(ng, wo);
@@ -570,7 +607,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_shadow(DevBvh bvh, Queues q, uint
   st_init(st, lds_stack, bvh.spill);
   const uint32_t nx = (b & 1u) ^ 1u;
   const uint32_t n = q.counters[cnt_shadow(b)];
-  TraceCount tc{0u, 0u};
+  TraceCount tc{0u, 0u, 0u};
   LaneFeed feed = feed_init(n, q.cursors + cursor_set(b, 1));
   AnyRay ar;
   bool active = false;
@@ -605,6 +642,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_shadow(DevBvh bvh, Queues q, uint
   if (COUNT) {
     atomicAdd(&q.stats[kStShNodes], (unsigned long long)tc.nodes);
     atomicAdd(&q.stats[kStShTris], (unsigned long long)tc.tris);
+    atomicAdd(&q.stats[kStSpill], (unsigned long long)tc.deep);
   }
 }
 
@@ -726,6 +764,36 @@ __global__ void k_err_final(const double* partial, uint32_t nblocks, double* out
   out[0] = a;
   out[1] = b;
   out[2] = c;
+}
+
+// Display encoding thresholds: code k + 1 starts at the smallest float x
+// whose sRGB encoding reaches (k + 0.5) / 255 (computed on the host in
+// double), so the 8-bit code of x is the number of thresholds <= x: the
+// correctly rounded round(255 * srgb(x)) with no transcendental on the GPU.
+struct ToneLut {
+  float t[256];
+};
+
+__global__ void k_tonemap(const float4* film, float inv_spp, float scale, uint32_t npix, ToneLut lut,
+                          uint32_t* out) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  float4 f = film[p];
+  float c[3] = {(f.x * inv_spp) * scale, (f.y * inv_spp) * scale, (f.z * inv_spp) * scale};
+  uint32_t px = 0xFF000000u;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    uint32_t lo = 0, hi = 255;  // code = #{thresholds <= c}, thresholds ascending
+    while (lo < hi) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (c[k] >= lut.t[mid])
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    px |= lo << (8 * k);
+  }
+  out[p] = px;
 }
 
 // Elementwise FP32 arithmetic of the device for host-side comparison:
@@ -901,6 +969,12 @@ struct aqua_renderer {
   std::vector<uint32_t> tri_order;
   DevBuf<float4> tris, nodes, mats, lights;
   DevBuf<float> cdf, pmf;
+  // image textures: texels of every image, (first texel, w, h) per image,
+  // texcoords per triangle slot, sRGB decode table
+  DevBuf<uint32_t> texels;
+  DevBuf<uint4> tex_info;
+  DevBuf<float4> tuv;
+  DevBuf<float> tex_lut;
   DevBuf<unsigned long long> stats;
   // pinned host copies of each pass's counters (read after the frame)
   uint32_t* pass_counters = nullptr;
@@ -958,6 +1032,10 @@ struct aqua_renderer {
     s.light_pmf = pmf.p;
     s.n_area = n_area;
     s.n_lights = n_lights;
+    s.tex.texels = texels.p;
+    s.tex.info = tex_info.p;
+    s.tex.uv = tuv.p;
+    s.tex.lut = tex_lut.p;
     return s;
   }
 
@@ -1123,9 +1201,9 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
     hipEvent_t* ev = &r->kev[4 * b];
     if (timing) HIPCHK(hipEventRecord(ev[0], s));
     if (count)
-      k_intersect<true><<<gp, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, cur, q.hit, q.stats);
+      k_intersect<true><<<gp, kBlock, 0, s>>>(sc.bvh, QueueIO{q.ro[c], q.rd[c], q.hit}, cnt, cur, q.stats);
     else
-      k_intersect<false><<<gp, kBlock, 0, s>>>(sc.bvh, q.ro[c], q.rd[c], cnt, cur, q.hit, q.stats);
+      k_intersect<false><<<gp, kBlock, 0, s>>>(sc.bvh, QueueIO{q.ro[c], q.rd[c], q.hit}, cnt, cur, q.stats);
     if (timing) HIPCHK(hipEventRecord(ev[1], s));
     if (b > 0 && shadow_on_s2(b - 1)) HIPCHK(hipStreamWaitEvent(s, lane.join_ev[b - 1], 0));
     if (pp.nrc)
@@ -1364,7 +1442,7 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   // waits for another to exit; a multiple of 8 (one work range per XCD).
   {
     int a = 0, b = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_intersect<false>, kBlock, 0));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_intersect<false, QueueIO>, kBlock, 0));
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kBlock, 0));
     int per_cu = std::max(1, std::min(a, b));
     r->persistent_blocks = std::max(8, (std::max(1, cus) * per_cu) & ~7);
@@ -1445,7 +1523,9 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
     const aqua_material& a = scene->materials[m];
     mats[3 * m + 0] = make_float4(a.base_color[0], a.base_color[1], a.base_color[2], a.metallic);
     mats[3 * m + 1] = make_float4(a.emission[0], a.emission[1], a.emission[2], a.roughness);
-    mats[3 * m + 2] = make_float4(a.specular, lt.mat_pdf[m], 0.0f, 0.0f);
+    float texf;
+    memcpy(&texf, &a.base_color_tex, 4);
+    mats[3 * m + 2] = make_float4(a.specular, lt.mat_pdf[m], texf, 0.0f);
   }
   r->n_area = uint32_t(lt.tri.size());
   r->n_lights = uint32_t(lt.pmf.size());
@@ -1485,6 +1565,47 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   if (int e = r->lights.upload(lights.data(), lights.size(), s)) return e;
   if (int e = r->cdf.upload(cdf.data(), cdf.size(), s)) return e;
   if (int e = r->pmf.upload(pmf.data(), pmf.size(), s)) return e;
+  // Image textures of base colours.
+  bool textured = false;
+  for (const aqua_material& a : scene->materials) {
+    if (!a.base_color_tex) continue;
+    if (a.base_color_tex - 1u >= scene->images.size())
+      return aqua::fail(AQUA_ERR_INVALID_ARG, "material names image %u of %zu", a.base_color_tex - 1u,
+                        scene->images.size());
+    textured = true;
+  }
+  if (textured) {
+    if (scene->uvs.size() != 6 * size_t(n))
+      return aqua::fail(AQUA_ERR_INVALID_ARG, "textured materials need texcoords (6 floats per triangle)");
+    std::vector<uint4> info;
+    size_t total = 0;
+    for (const AquaImage& im : scene->images) {
+      if (total + size_t(im.w) * im.h >= (size_t(1) << 32))
+        return aqua::fail(AQUA_ERR_UNSUPPORTED, "image textures exceed 2^32 texels");
+      info.push_back(make_uint4(uint32_t(total), im.w, im.h, 0u));
+      total += size_t(im.w) * im.h;
+    }
+    std::vector<uint32_t> px(std::max<size_t>(total, 1));
+    for (size_t k = 0; k < scene->images.size(); ++k) {
+      const AquaImage& im = scene->images[k];
+      for (size_t i = 0; i < size_t(im.w) * im.h; ++i) {
+        const uint8_t* c = &im.rgba[4 * i];
+        px[info[k].x + i] = uint32_t(c[0]) | uint32_t(c[1]) << 8 | uint32_t(c[2]) << 16 | uint32_t(c[3]) << 24;
+      }
+    }
+    std::vector<float4> uv(2 * size_t(std::max(n, 1u)), make_float4(0, 0, 0, 0));
+    for (uint32_t sl = 0; sl < n; ++sl) {
+      const float* u = &scene->uvs[6 * size_t(r->tri_order[sl])];
+      uv[2 * size_t(sl)] = make_float4(u[0], u[1], u[2], u[3]);
+      uv[2 * size_t(sl) + 1] = make_float4(u[4], u[5], 0.0f, 0.0f);
+    }
+    float lut[256];
+    for (int i = 0; i < 256; ++i) lut[i] = float(aqua::srgb_to_linear(i / 255.0));
+    if (int e = r->texels.upload(px.data(), px.size(), s)) return e;
+    if (int e = r->tex_info.upload(info.data(), info.size(), s)) return e;
+    if (int e = r->tuv.upload(uv.data(), uv.size(), s)) return e;
+    if (int e = r->tex_lut.upload(lut, 256, s)) return e;
+  }
   if (int e = r->stats.ensure(kStWords)) return e;
   if (cfg->type == AQUA_INTEGRATOR_NRC) {
     if (cfg->nrc_precision > AQUA_NRC_FP8)
@@ -1622,6 +1743,7 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
   st.shadow_nodes_visited = hs[kStShNodes];
   st.shadow_tris_tested = hs[kStShTris];
   st.nonfinite_samples = hs[kStNonfinite];
+  st.spill_node_visits = hs[kStSpill];
   r->film_w = cam.width;
   r->film_h = cam.height;
   if (fp64)
@@ -1665,6 +1787,34 @@ int aqua_film_read(aqua_renderer* r, float* rgb, uint32_t* spp) {
     rgb[3 * p + 2] = h[p].z * inv;
   }
   if (spp) *spp = r->film_spp;
+  return AQUA_OK;
+}
+
+int aqua_film_tonemap(aqua_renderer* r, float exposure, uint8_t* rgba) {
+  if (!r || !rgba) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (r->film_spp == 0) return aqua::fail(AQUA_ERR_INVALID_ARG, "no frame rendered yet");
+  if (!std::isfinite(exposure) || std::fabs(exposure) > 64.0f)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "exposure %g out of range", double(exposure));
+  HIPCHK(hipSetDevice(r->device));
+  ToneLut lut;
+  for (int k = 0; k < 255; ++k) {
+    // linear value whose sRGB encoding is (k + 0.5) / 255 (IEC 61966-2-1
+    // inverse), then the smallest float at or above it
+    double e = (k + 0.5) / 255.0;
+    double x = aqua::srgb_to_linear(e);
+    float xf = float(x);
+    if (double(xf) < x) xf = std::nextafter(xf, INFINITY);
+    lut.t[k] = xf;
+  }
+  lut.t[255] = INFINITY;  // never reached: codes stop at 255
+  const uint32_t npix = r->film_w * r->film_h;
+  DevBuf<uint32_t> out;
+  if (int e = out.ensure(npix)) return e;
+  k_tonemap<<<int((npix + kBlock - 1) / kBlock), kBlock, 0, r->stream>>>(
+      r->film.p, 1.0f / float(r->film_spp), std::exp2(exposure), npix, lut, out.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(rgba, out.p, size_t(npix) * 4, hipMemcpyDeviceToHost, r->stream));
+  HIPCHK(hipStreamSynchronize(r->stream));
   return AQUA_OK;
 }
 
@@ -1723,8 +1873,8 @@ int aqua_primary_hits(aqua_renderer* r, const aqua_camera* cam_in, uint32_t fram
   hipStream_t s = r->stream;
   k_reset<<<8, kBlock, 0, s>>>(q.counters, n, q.cursors, kCursorSet);
   k_camera<<<grid_for(n), kBlock, 0, s>>>(pp, q);
-  k_intersect<false><<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, q.ro[0], q.rd[0], q.counters, q.cursors, q.hit,
-                                                         q.stats);
+  k_intersect<false><<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, QueueIO{q.ro[0], q.rd[0], q.hit}, q.counters,
+                                                         q.cursors, q.stats);
   k_hits_out<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.hit, sc.bvh.tris, n, ids.p, ts.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(prim_ids, ids.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1765,8 +1915,8 @@ int aqua_trace_closest(aqua_renderer* r, uint32_t n, const float* rays, uint32_t
   if (int e = ts.ensure(n)) return e;
   hipStream_t s = r->stream;
   k_reset<<<8, kBlock, 0, s>>>(q.counters, n, q.cursors, kCursorSet);
-  k_intersect<false><<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, q.ro[0], q.rd[0], q.counters, q.cursors, q.hit,
-                                                         q.stats);
+  k_intersect<false><<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, QueueIO{q.ro[0], q.rd[0], q.hit}, q.counters,
+                                                         q.cursors, q.stats);
   k_hits_out<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.hit, sc.bvh.tris, n, ids.p, ts.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(prim_ids, ids.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));

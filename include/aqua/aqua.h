@@ -56,8 +56,14 @@ typedef struct aqua_load_opts {
                                  (room.json:3021 names the withheld mesh recorded in
                                  .MISSING_LARGE_BLOBS:1); 0: AQUA_ERR_MISSING_MESH        */
   uint32_t image_as_constant; /* 1: `{"Image": path}` colours load as constant 0.5 grey
-                                 (room.json:6 and 15 others); 0: AQUA_ERR_UNSUPPORTED     */
-  uint32_t reserved[6];
+                                 (room.json:6 and 15 others); 0: the image (JPEG, path
+                                 relative to the JSON) is decoded and sampled through the
+                                 meshes' texcoords                                         */
+  uint32_t emission_from_mtl; /* 1: a named BSDF whose emission is 0 takes the `Ke` of the
+                                 same-named material of a .mtl file next to the JSON (the
+                                 room's lamp quads: room.json:251-258 name MiddleLight,
+                                 living_room.mtl:193-194 gives it Ke 5 5 5); 0: as written */
+  uint32_t reserved[5];
 } aqua_load_opts;
 
 /* 48 bytes. The subset of the 17 Principled parameters (cbox.json:5-63) that
@@ -70,7 +76,10 @@ typedef struct aqua_material {
   float emission[3];   /* linear RGB radiance; > 0 makes the mesh an area light  */
   float roughness;     /* GGX alpha = max(roughness^2, 1e-3)                     */
   float specular;      /* dielectric F0 = 0.08 * specular                        */
-  float pad[3];
+  uint32_t base_color_tex; /* 0: base_color; k > 0: image k - 1 of the scene, sampled
+                              bilinearly (repeat) at the hit's texcoords, sRGB texels
+                              decoded to linear; base_color then holds its mean     */
+  float pad[2];
 } aqua_material;
 
 /* 64 bytes. Pinhole camera. Camera space looks down -Z, +Y up (the shipped
@@ -96,11 +105,13 @@ typedef struct aqua_scene_info {
   uint32_t n_skipped_shapes;    /* shapes skipped for a missing .mesh     */
   uint32_t n_emissive_triangles;
   uint32_t n_point_lights;
-  uint32_t n_image_textures;    /* Image colours replaced by a constant   */
+  uint32_t n_image_textures;    /* Image colours (decoded, or replaced by a
+                                   constant with image_as_constant)        */
   uint32_t n_vertices;          /* summed over the loaded meshes          */
   float bounds_lo[3];
   float bounds_hi[3];
-  uint32_t reserved[2];
+  uint32_t n_images;            /* decoded images the scene holds         */
+  uint32_t has_texcoords;       /* 1: per-triangle texcoords are present  */
 } aqua_scene_info;
 
 /* Load a scene JSON (schema of scenes/cbox.json) and every .mesh it names
@@ -120,7 +131,33 @@ int aqua_scene_get_triangles(const aqua_scene* scene, float* verts, uint32_t* ma
 int aqua_scene_get_materials(const aqua_scene* scene, aqua_material* out);
 int aqua_scene_get_camera(const aqua_scene* scene, aqua_camera* out);
 int aqua_scene_get_point_lights(const aqua_scene* scene, aqua_point_light* out);
+/* Texcoords in load order, 6 floats per triangle (u, v of v0, v1, v2; the
+ * .mesh `texcoords`, v up); AQUA_ERR_INVALID_ARG when the scene has none. */
+int aqua_scene_get_texcoords(const aqua_scene* scene, float* uv);
+/* Image k (0-based) of the scene: size, and 8-bit sRGB RGBA texels (row 0 =
+ * top) when rgba is not NULL. */
+int aqua_scene_get_image(const aqua_scene* scene, uint32_t k, uint32_t* width, uint32_t* height, uint8_t* rgba);
+/* For scenes built from arrays, before a renderer is created from them:
+ * attach texcoords (6 floats per triangle) / add an image (8-bit sRGB RGBA,
+ * row 0 = top) that materials name as base_color_tex = *index + 1. */
+int aqua_scene_set_texcoords(aqua_scene* scene, const float* uv);
+int aqua_scene_add_image(aqua_scene* scene, uint32_t width, uint32_t height, const uint8_t* rgba,
+                         uint32_t* index);
 void aqua_scene_destroy(aqua_scene* scene);
+
+/* ---- images ------------------------------------------------------------- */
+
+/* Decodes a JPEG file (the `{"Image": path}` texture format of
+ * scenes/room.json:6; baseline or progressive, 8-bit, grey or YCbCr/RGB) to
+ * 8-bit RGBA, row 0 = top. With rgba == NULL only the size is returned. */
+int aqua_image_load(const char* path, uint32_t* width, uint32_t* height, uint8_t* rgba);
+/* Writes 8-bit RGB (channels 3) or RGBA (channels 4) pixels, row 0 = top,
+ * as a PNG file (zlib-deflated, no filter). */
+int aqua_image_write_png(const char* path, uint32_t width, uint32_t height, uint32_t channels,
+                         const uint8_t* pixels);
+/* Writes float RGB pixels, row 0 = top, as a little-endian PFM file (the
+ * portable float map: HDR radiance without loss). */
+int aqua_image_write_pfm(const char* path, uint32_t width, uint32_t height, const float* rgb);
 
 /* ---- procedural scenes (written in the reference schema: JSON + .mesh) -- */
 
@@ -197,7 +234,9 @@ typedef struct aqua_frame_stats {
   uint32_t nrc_train_records;  /* training samples produced this frame  */
   float nrc_loss;              /* last training step's loss             */
   float ms_nrc;                /* cache inference + targets + training  */
-  uint64_t reserved[1];
+  uint64_t spill_node_visits;  /* node visits made while the ray's traversal
+                                  stack had entries spilled to global memory
+                                  (both traversal kernels; counting on only) */
 } aqua_frame_stats;
 
 /* 32 bytes. */
@@ -220,6 +259,11 @@ int aqua_render_frame(aqua_renderer* r, const aqua_camera* camera, uint32_t fram
                       aqua_frame_stats* stats);
 /* Mean radiance of the last frame, RGB float per pixel, row 0 = top. */
 int aqua_film_read(aqua_renderer* r, float* rgb, uint32_t* spp);
+/* The last frame for display, computed on the GPU: each channel of the mean
+ * radiance times 2^exposure, clamped to [0, 1], encoded with the sRGB curve
+ * (IEC 61966-2-1) and rounded to 8 bits; RGBA per pixel (alpha 255), row 0 =
+ * top (host array of width * height * 4). aqua_image_write_png stores it. */
+int aqua_film_tonemap(aqua_renderer* r, float exposure, uint8_t* rgba);
 
 /* FP64-accumulated render: adds samples [spp_begin, spp_begin + spp_count)
  * of `frame` into an FP64 film, so a long oracle render can be resumed;

@@ -170,6 +170,7 @@ typedef struct {
 
 typedef struct {
   float base[3], metallic, emission[3], roughness, specular, light_pdf;
+  uint32_t tex; /* 0, or image index + 1 of a textured base colour */
 } OMat;
 
 struct orc_scene {
@@ -188,6 +189,12 @@ struct orc_scene {
   float* light_area;
   float* light_pmf;
   float* light_cdf;
+  /* image textures */
+  float* uvs;        /* 6 per triangle (load order) */
+  uint32_t n_images;
+  uint32_t* img;     /* 3 per image: first texel, width, height */
+  uint8_t* texels;   /* RGBA8, row 0 = top */
+  float lut[256];    /* sRGB code -> linear */
 };
 
 typedef struct {
@@ -403,6 +410,7 @@ int orc_scene_create(uint32_t n_tris, const float* verts, const uint32_t* mat_id
     o->metallic = a[3];
     o->roughness = a[7];
     o->specular = a[8];
+    memcpy(&o->tex, &a[9], 4);
   }
   /* Culling slack: 2^-16 of the largest coordinate magnitude. */
   float mxc = 0.0f;
@@ -457,7 +465,36 @@ void orc_scene_destroy(orc_scene* s) {
   free(s->light_area);
   free(s->light_pmf);
   free(s->light_cdf);
+  free(s->uvs);
+  free(s->img);
+  free(s->texels);
   free(s);
+}
+
+int orc_scene_set_textures(orc_scene* s, const float* uvs, uint32_t n_images, const uint32_t* dims,
+                           const uint8_t* texels) {
+  size_t total = 0;
+  for (uint32_t k = 0; k < n_images; ++k) total += (size_t)dims[2 * k] * dims[2 * k + 1];
+  s->uvs = (float*)malloc(sizeof(float) * 6 * (size_t)s->n_tris + 4);
+  s->img = (uint32_t*)malloc(sizeof(uint32_t) * 3 * (size_t)n_images + 4);
+  s->texels = (uint8_t*)malloc(4 * total + 4);
+  if (!s->uvs || !s->img || !s->texels) return -1;
+  memcpy(s->uvs, uvs, sizeof(float) * 6 * (size_t)s->n_tris);
+  memcpy(s->texels, texels, 4 * total);
+  size_t at = 0;
+  for (uint32_t k = 0; k < n_images; ++k) {
+    s->img[3 * k] = (uint32_t)at;
+    s->img[3 * k + 1] = dims[2 * k];
+    s->img[3 * k + 2] = dims[2 * k + 1];
+    at += (size_t)dims[2 * k] * dims[2 * k + 1];
+  }
+  s->n_images = n_images;
+  /* IEC 61966-2-1 decode of each 8-bit code, in double, rounded once */
+  for (int i = 0; i < 256; ++i) {
+    double c = i / 255.0;
+    s->lut[i] = (float)(c <= 0.04045 ? c / 12.92 : pow((c + 0.055) / 1.055, 2.4));
+  }
+  return 0;
 }
 
 void orc_scene_bvh_info(const orc_scene* s, uint32_t* n_nodes, uint32_t* max_depth) {
@@ -838,6 +875,55 @@ static void mat_from12(const float* a, OMat* o) {
   o->roughness = a[7];
   o->specular = a[8];
   o->light_pdf = 0.0f;
+  o->tex = 0;
+}
+
+/* ---- image textures: texcoords from the barycentrics of the hit point
+ * (normal equations of p - v0 = b1 e1 + b2 e2, clamped to [0, 1]), then a
+ * bilinear, repeating fetch of sRGB texels decoded through the table; texel
+ * centres at half-integers, v = 0 the bottom row. */
+static void bary_of(V v0, V v1, V v2, V p, float* b1, float* b2) {
+  V e1 = vsub(v1, v0), e2 = vsub(v2, v0), w = vsub(p, v0);
+  float d00 = vdot(e1, e1), d01 = vdot(e1, e2), d11 = vdot(e2, e2);
+  float d20 = vdot(w, e1), d21 = vdot(w, e2);
+  float den = d00 * d11 - d01 * d01;
+  float x = 0.0f, y = 0.0f;
+  if (den != 0.0f) {
+    x = (d11 * d20 - d01 * d21) / den;
+    y = (d00 * d21 - d01 * d20) / den;
+  }
+  *b1 = x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f;
+  *b2 = y > 0.0f ? (y < 1.0f ? y : 1.0f) : 0.0f;
+}
+
+static float lerp2(const float* lut, const uint8_t* a, const uint8_t* b, const uint8_t* c, const uint8_t* d, int k,
+                   float fx, float fy) {
+  float top = lut[a[k]] * (1.0f - fx) + lut[b[k]] * fx;
+  float bot = lut[c[k]] * (1.0f - fx) + lut[d[k]] * fx;
+  return top * (1.0f - fy) + bot * fy;
+}
+
+static void tex_base(const orc_scene* s, uint32_t tri, uint32_t img, float b1, float b2, float out[3]) {
+  const float* uv = &s->uvs[6 * (size_t)tri];
+  float b0 = (1.0f - b1) - b2;
+  float u = (uv[0] * b0 + uv[2] * b1) + uv[4] * b2;
+  float vv = (uv[1] * b0 + uv[3] * b1) + uv[5] * b2;
+  u = u - floorf(u);
+  vv = vv - floorf(vv);
+  int w = (int)s->img[3 * img + 1], h = (int)s->img[3 * img + 2];
+  float x = u * (float)w - 0.5f;
+  float y = (1.0f - vv) * (float)h - 0.5f;
+  float xf = floorf(x), yf = floorf(y);
+  float fx = x - xf, fy = y - yf;
+  int x0 = (int)xf, y0 = (int)yf;
+  int xa = x0 < 0 ? w - 1 : x0, xb = x0 + 1 >= w ? 0 : x0 + 1;
+  int ya = y0 < 0 ? h - 1 : y0, yb = y0 + 1 >= h ? 0 : y0 + 1;
+  const uint8_t* t = s->texels + 4 * (size_t)s->img[3 * img];
+  const uint8_t* p00 = t + 4 * ((size_t)ya * w + xa);
+  const uint8_t* p10 = t + 4 * ((size_t)ya * w + xb);
+  const uint8_t* p01 = t + 4 * ((size_t)yb * w + xa);
+  const uint8_t* p11 = t + 4 * ((size_t)yb * w + xb);
+  for (int k = 0; k < 3; ++k) out[k] = lerp2(s->lut, p00, p10, p01, p11, k, fx, fy);
 }
 
 void orc_bsdf_eval(const float* mat, const float* n, const float* wo, const float* wi, float* out5) {
@@ -983,6 +1069,14 @@ static V trace_path(const orc_scene* s, const Cam* cam, const orc_cfg* cfg, uint
     V v0 = v(p9[0], p9[1], p9[2]), v1 = v(p9[3], p9[4], p9[5]), v2 = v(p9[6], p9[7], p9[8]);
     const OMat* m = &s->mats[s->mat_ids[id]];
     V p = v(o.x + t * d.x, o.y + t * d.y, o.z + t * d.z);
+    OMat mtex;
+    if (m->tex) {
+      float b1, b2;
+      mtex = *m;
+      bary_of(v0, v1, v2, p, &b1, &b2);
+      tex_base(s, id, m->tex - 1u, b1, b2, mtex.base);
+      m = &mtex;
+    }
     V wo = vneg(d);
     V ng = vnorm(vcross(vsub(v1, v0), vsub(v2, v0)));
     float cg = vdot(ng, wo);
@@ -1208,22 +1302,77 @@ int orc_render64(const orc_scene* s, const float* camera, const orc_cfg* cfg, ui
   return 0;
 }
 
+typedef struct {
+  const orc_scene* s;
+  const Cam* cam;
+  const orc_cfg* cfg;
+  uint32_t frame, sample, y0, y1;
+  uint32_t* ids;
+  float* t;
+  atomic_uint next_row;
+} HitJob;
+
+static void* hit_worker(void* arg) {
+  HitJob* J = (HitJob*)arg;
+  const Cam* cam = J->cam;
+  uint32_t dims = 8u * (J->cfg->max_depth + 2u);
+  for (;;) {
+    uint32_t y = atomic_fetch_add(&J->next_row, 1u);
+    if (y >= J->y1) break;
+    for (uint32_t x = 0; x < cam->width; ++x) {
+      uint32_t pixel = y * cam->width + x;
+      Pcg rng = pcg_seed(J->cfg->seed, pixel);
+      pcg_advance(&rng, (((uint64_t)J->frame << 20) | (uint64_t)J->sample) * (uint64_t)dims);
+      float u0 = unit_float(pcg_next(&rng));
+      float u1 = unit_float(pcg_next(&rng));
+      V o = v(cam->m[3], cam->m[7], cam->m[11]);
+      V d = camera_dir(cam, x, y, u0, u1);
+      float tt;
+      size_t k = (size_t)(y - J->y0) * cam->width + x;
+      J->ids[k] = closest(J->s, o, d, FLT_MAX_F, 0, &tt, NULL);
+      if (J->t) J->t[k] = tt;
+    }
+  }
+  return NULL;
+}
+
+int orc_primary_hits_rows(const orc_scene* s, const float* camera, const orc_cfg* cfg, uint32_t frame,
+                          uint32_t sample, uint32_t y0, uint32_t y1, uint32_t* ids, float* t) {
+  Cam cam;
+  cam_from16(camera, &cam);
+  if (y1 > cam.height || y0 > y1) return -1;
+  HitJob J;
+  memset(&J, 0, sizeof J);
+  J.s = s;
+  J.cam = &cam;
+  J.cfg = cfg;
+  J.frame = frame;
+  J.sample = sample;
+  J.y0 = y0;
+  J.y1 = y1;
+  J.ids = ids;
+  J.t = t;
+  atomic_init(&J.next_row, y0);
+  uint32_t threads = cfg->threads;
+  if (threads == 0) {
+    long n = sysconf(_SC_NPROCESSORS_ONLN);
+    threads = n > 0 ? (uint32_t)n : 1u;
+  }
+  if (threads > 256) threads = 256;
+  pthread_t th[256];
+  uint32_t started = 0;
+  for (uint32_t i = 0; i < threads; ++i) {
+    if (pthread_create(&th[i], NULL, hit_worker, &J) != 0) break;
+    ++started;
+  }
+  if (started == 0) hit_worker(&J);
+  for (uint32_t i = 0; i < started; ++i) pthread_join(th[i], NULL);
+  return 0;
+}
+
 int orc_primary_hits(const orc_scene* s, const float* camera, const orc_cfg* cfg, uint32_t frame,
                      uint32_t sample, uint32_t* ids, float* t) {
   Cam cam;
   cam_from16(camera, &cam);
-  uint32_t dims = 8u * (cfg->max_depth + 2u);
-  uint32_t npix = cam.width * cam.height;
-  for (uint32_t pixel = 0; pixel < npix; ++pixel) {
-    Pcg rng = pcg_seed(cfg->seed, pixel);
-    pcg_advance(&rng, (((uint64_t)frame << 20) | (uint64_t)sample) * (uint64_t)dims);
-    float u0 = unit_float(pcg_next(&rng));
-    float u1 = unit_float(pcg_next(&rng));
-    V o = v(cam.m[3], cam.m[7], cam.m[11]);
-    V d = camera_dir(&cam, pixel % cam.width, pixel / cam.width, u0, u1);
-    float tt;
-    ids[pixel] = closest(s, o, d, FLT_MAX_F, 0, &tt, NULL);
-    if (t) t[pixel] = tt;
-  }
-  return 0;
+  return orc_primary_hits_rows(s, camera, cfg, frame, sample, 0, cam.height, ids, t);
 }

@@ -43,6 +43,12 @@ typedef struct orc_scene orc_scene;
 int orc_scene_create(uint32_t n_tris, const float* verts, const uint32_t* mat_ids, uint32_t n_mats,
                      const float* mats, uint32_t n_point, const float* points, orc_scene** out);
 void orc_scene_destroy(orc_scene* s);
+/* Image textures: uvs 6 floats per triangle (u, v per corner, load order),
+ * dims (width, height) per image, texels the images' 8-bit sRGB RGBA
+ * concatenated (row 0 = top). Materials name image k as base_color_tex
+ * (float slot 9, as uint32 bits) = k + 1. */
+int orc_scene_set_textures(orc_scene* s, const float* uvs, uint32_t n_images, const uint32_t* dims,
+                           const uint8_t* texels);
 /* Oracle BVH statistics: nodes, max depth. */
 void orc_scene_bvh_info(const orc_scene* s, uint32_t* n_nodes, uint32_t* max_depth);
 
@@ -68,6 +74,10 @@ int orc_render64(const orc_scene* s, const float* camera, const orc_cfg* cfg, ui
 
 int orc_primary_hits(const orc_scene* s, const float* camera, const orc_cfg* cfg, uint32_t frame,
                      uint32_t sample, uint32_t* ids, float* t);
+/* Same over pixel rows [y0, y1) only (ids / t hold (y1 - y0) * width entries);
+ * rows are shared among cfg->threads threads (0 = all online CPUs). */
+int orc_primary_hits_rows(const orc_scene* s, const float* camera, const orc_cfg* cfg, uint32_t frame,
+                          uint32_t sample, uint32_t y0, uint32_t y1, uint32_t* ids, float* t);
 /* rays: 8 floats each (o, tmax, d, 0). brute != 0 tests every triangle. */
 int orc_trace_closest(const orc_scene* s, uint32_t n, const float* rays, int brute, uint32_t* ids, float* t);
 int orc_trace_occluded(const orc_scene* s, uint32_t n, const float* rays, int brute, uint8_t* occ);

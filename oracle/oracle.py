@@ -34,11 +34,14 @@ _P = C.c_void_p
 _SIGS = {
     "orc_scene_create": (C.c_int, [C.c_uint32, _P, _P, C.c_uint32, _P, C.c_uint32, _P, C.POINTER(_P)]),
     "orc_scene_destroy": (None, [_P]),
+    "orc_scene_set_textures": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "orc_scene_bvh_info": (None, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "orc_render": (C.c_int, [_P, _P, C.POINTER(Cfg), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                              _P, C.POINTER(Stats)]),
     "orc_render64": (C.c_int, [_P, _P, C.POINTER(Cfg), C.c_uint32, C.c_uint32, _P]),
     "orc_primary_hits": (C.c_int, [_P, _P, C.POINTER(Cfg), C.c_uint32, C.c_uint32, _P, _P]),
+    "orc_primary_hits_rows": (C.c_int, [_P, _P, C.POINTER(Cfg), C.c_uint32, C.c_uint32, C.c_uint32,
+                                        C.c_uint32, _P, _P]),
     "orc_trace_closest": (C.c_int, [_P, C.c_uint32, _P, C.c_int, _P, _P]),
     "orc_trace_occluded": (C.c_int, [_P, C.c_uint32, _P, C.c_int, _P]),
     "orc_pcg32_seq": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, _P]),
@@ -76,7 +79,9 @@ def f32(a):
 
 
 class Scene:
-    def __init__(self, verts, mat_ids, materials, point_lights=None):
+    def __init__(self, verts, mat_ids, materials, point_lights=None, uvs=None, images=()):
+        """uvs: float32 (n, 6); images: uint8 (h, w, 4) sRGB RGBA arrays that
+        materials name as base_color_tex = index + 1 (slot 9 as uint32 bits)."""
         self.verts = f32(verts).reshape(-1, 9)
         self.mat_ids = np.ascontiguousarray(mat_ids, dtype=np.uint32)
         self.materials = f32(materials).reshape(-1, 12)
@@ -88,6 +93,13 @@ class Scene:
                                     _p(pl) if len(pl) else None, C.byref(self._h))
         if rc != 0:
             raise MemoryError("orc_scene_create failed")
+        if uvs is not None:
+            self.uvs = f32(uvs).reshape(-1, 6)
+            dims = np.array([[im.shape[1], im.shape[0]] for im in images], np.uint32).reshape(-1)
+            tex = np.concatenate([np.ascontiguousarray(im, np.uint8).reshape(-1) for im in images]) \
+                if len(images) else np.zeros(4, np.uint8)
+            if lib().orc_scene_set_textures(self._h, _p(self.uvs), len(images), _p(dims), _p(tex)) != 0:
+                raise MemoryError("orc_scene_set_textures failed")
 
     def __del__(self):
         try:
@@ -125,15 +137,18 @@ class Scene:
             raise ValueError("orc_render64: bad arguments")
         return out
 
-    def primary_hits(self, camera16, sample=0, frame=0, max_depth=5, seed=0):
+    def primary_hits(self, camera16, sample=0, frame=0, max_depth=5, seed=0, y0=0, y1=None, threads=0):
+        """First-hit ids and distances of rows [y0, y1) (all rows by default)."""
         cam = f32(camera16)
         w = int(cam[14:15].view(np.uint32)[0])
         h = int(cam[15:16].view(np.uint32)[0])
-        ids = np.zeros(w * h, np.uint32)
-        t = np.zeros(w * h, np.float32)
-        cfg = Cfg(max_depth, 3, seed, 0)
-        lib().orc_primary_hits(self._h, _p(cam), C.byref(cfg), frame, sample, _p(ids), _p(t))
-        return ids.reshape(h, w), t.reshape(h, w)
+        y1 = h if y1 is None else y1
+        ids = np.zeros((y1 - y0) * w, np.uint32)
+        t = np.zeros((y1 - y0) * w, np.float32)
+        cfg = Cfg(max_depth, 3, seed, threads)
+        if lib().orc_primary_hits_rows(self._h, _p(cam), C.byref(cfg), frame, sample, y0, y1, _p(ids), _p(t)):
+            raise ValueError("orc_primary_hits_rows: bad arguments")
+        return ids.reshape(y1 - y0, w), t.reshape(y1 - y0, w)
 
     def trace_closest(self, rays, brute=False):
         r = f32(rays).reshape(-1, 8)

@@ -49,9 +49,13 @@ def box_room(box_room_dir):
 
 
 def oracle_scene_of(scene):
-    """The oracle's copy of a product scene (same flattened arrays)."""
+    """The oracle's copy of a product scene (same flattened arrays, texcoords
+    and decoded images)."""
     v, m = scene.triangles()
-    return oracle.Scene(v, m, scene.materials(), scene.point_lights())
+    info = scene.info()
+    uvs = scene.texcoords() if info.has_texcoords else None
+    images = [scene.image(k) for k in range(info.n_images)]
+    return oracle.Scene(v, m, scene.materials(), scene.point_lights(), uvs=uvs, images=images)
 
 
 @pytest.fixture(scope="session")

@@ -212,13 +212,46 @@ def test_malformed_json_and_missing_file(tmp_path):
 
 @pytest.mark.skipif(not os.path.isdir(REFERENCE_SCENES), reason="reference scenes not mounted")
 def test_room_json_loads_with_skip_missing():
+    # room.json:3021 names the mesh the snapshot withholds (.MISSING_LARGE_BLOBS:1)
     path = os.path.join(REFERENCE_SCENES, "room.json")
-    _expect(aqua.ERR_UNSUPPORTED, path)                 # Image textures
+    _expect(aqua.ERR_MISSING_MESH, path)
     _expect(aqua.ERR_MISSING_MESH, path, image_as_constant=True)
     sc = aqua.Scene.load(path, skip_missing=True, image_as_constant=True)
     i = sc.info()
-    assert (i.n_shapes, i.n_skipped_shapes, i.n_image_textures) == (152, 1, 16)
+    assert (i.n_shapes, i.n_skipped_shapes, i.n_image_textures, i.n_images) == (152, 1, 16, 0)
     assert i.n_triangles == 394269
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE_SCENES), reason="reference scenes not mounted")
+def test_room_json_as_shipped_textures_and_lamps():
+    # The 16 Image colours (room.json:6, 194, 382, ...) name 16 distinct JPEGs,
+    # decoded with the texcoords of every mesh; with emission_from_mtl the
+    # lamp quads (black, emission 0 in the JSON: room.json:251-258) take Ke
+    # from living_room.mtl (BackLight 2 2 2 at :11-13; Left / Middle / Right
+    # Light 5 5 5 at :167-168, :193-194, :287-288); the point light stays
+    # (room.json:2656-2672).
+    path = os.path.join(REFERENCE_SCENES, "room.json")
+    sc = aqua.Scene.load(path, skip_missing=True, emission_from_mtl=True)
+    i = sc.info()
+    assert (i.n_image_textures, i.n_images, i.has_texcoords, i.n_point_lights) == (16, 16, 1, 1)
+    mats = sc.materials()
+    names = json.load(open(path))["named_bsdfs"]
+    by = dict(zip(names.keys(), mats))
+    for lamp, ke in (("BackLight", 2.0), ("LeftLight", 5.0), ("MiddleLight", 5.0), ("RightLight", 5.0)):
+        np.testing.assert_array_equal(by[lamp][4:7], [ke, ke, ke])
+    assert i.n_emissive_triangles > 0
+    tex = mats[:, 9].view(np.uint32)
+    assert (tex > 0).sum() == 16 and sorted(tex[tex > 0]) == list(range(1, 17))
+    assert np.all(mats[tex == 0][:, 4:7].max(1) >= 0)
+    # without the option the lamps stay dark, as written
+    plain = aqua.Scene.load(path, skip_missing=True).materials()
+    assert plain[:, 4:7].max() == 0.0
+    uv = sc.texcoords()
+    assert uv.shape == (i.n_triangles, 6) and np.all(np.isfinite(uv))
+    # a decoded texture is the JPEG itself
+    wood = [k for k, v in names.items() if v["Principled"]["color"].get("Image", "").endswith("wood4.jpg")][0]
+    img = sc.image(int(by[wood][9:10].view(np.uint32)[0]) - 1)
+    np.testing.assert_array_equal(img, aqua.image_load(os.path.join(REFERENCE_SCENES, "textures", "wood4.jpg")))
 
 
 # ------------------------------------------------------------- procgen -----
