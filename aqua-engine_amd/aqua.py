@@ -178,6 +178,8 @@ _SIGS = {
     "aqua_nrc_gradients": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, C.POINTER(C.c_float)]),
     "aqua_renderer_nrc": (_P, [_P]),
     "aqua_renderer_set_nrc_enabled": (C.c_int, [_P, C.c_int]),
+    "aqua_renderer_nrc_checkpoint": (C.c_int, [_P, C.c_int]),
+    "aqua_nrc_checkpoint": (C.c_int, [_P, C.c_int]),
     "aqua_renderer_nrc_batch": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(_P), C.POINTER(_P)]),
     "aqua_camera_flythrough": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
                                          C.c_uint32, C.POINTER(Camera)]),
@@ -470,6 +472,10 @@ class Renderer:
 
     def set_nrc_enabled(self, enabled: bool) -> None:
         _check(lib().aqua_renderer_set_nrc_enabled(self._h, int(enabled)))
+
+    def nrc_checkpoint(self, restore: bool = False) -> None:
+        """Saves (restore False) or puts back (True) the cache's training state."""
+        _check(lib().aqua_renderer_nrc_checkpoint(self._h, int(restore)))
 
     def nrc_handle(self):
         return lib().aqua_renderer_nrc(self._h)

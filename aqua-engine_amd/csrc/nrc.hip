@@ -34,13 +34,17 @@
 namespace {
 
 constexpr int kLevels = 16;
-// Levels kFineLevel0.. (hashed with the default 2^19 table) accumulate their
-// table gradient as BF16 pairs: one packed atomic per entry instead of two
-// FP32 ones. BF16 rather than FP16 because the relative L2 gradient
-// 2 (y - t) / (y^2 + 0.01) reaches 200 t where the prediction is near zero,
-// and radiance targets reach 10^3 and more: FP16 sums overflow there. Levels
-// 0-3, whose few entries collect many rows each, stay FP32.
-constexpr int kFineLevel0 = 4;
+// Levels kFineLevel0.. (the finest, hashed with the default 2^19 table)
+// accumulate their table gradient as BF16 pairs: one packed atomic per entry
+// instead of two FP32 ones. BF16 rather than FP16 because the relative L2
+// gradient 2 (y - t) / (y^2 + 0.01) reaches 200 t where the prediction is
+// near zero, and radiance targets reach 10^3 and more: FP16 sums overflow
+// there. An 8-bit mantissa stops adding once a sum is ~256 addends large, so
+// the coarser levels 0-7, whose entries collect many rows each (thousands
+// when a batch clusters in a small region: levels 4-7 lost 30-80 % of their
+// gradient in BF16 there), accumulate FP32 pairs; they also merge equal
+// entries across a lane group first, so their atomics are few.
+constexpr int kFineLevel0 = 8;
 static_assert(kFineLevel0 <= 8, "levels 8..15 (the unmerged scatter) must accumulate in the BF16 pairs");
 constexpr int kHidden = 4;
 constexpr int kMlp = 4 * 64 * 64 + 64 * 16;    // 17408 floats
@@ -253,10 +257,53 @@ __device__ __forceinline__ float wave_sum(float v) {
 // Encodes this lane group's 16 input features of one row: f[ks*8 + i] is
 // feature kappa(g, ks, i) (hash levels 2g, 2g+1, 8+2g, 9+2g; SH 4g..4g+3;
 // extras 4g..4g+3).
+//
+// CP (queries): corner-parallel gather. The four lanes of a column (lane %
+// 16 = the row) hold the same row; for every level each fetches one of the
+// cell's four corner pairs (x, x + 1 at y + (g & 1), z + (g >> 1)), weighs
+// it, and the four partial sums are added across the lane groups. A gather
+// instruction then covers one level of 16 rows (about 2.3 lines a row)
+// instead of 64 unrelated cells, which is what an L1 that serves about a
+// line per clock needs. Lane group g keeps the features of its own levels.
+// Otherwise (training, whose scatter owns levels per lane) each lane sums
+// all eight corners of its four levels.
+template <bool CP>
 __device__ __forceinline__ void encode16(const FwdArgs& a, uint32_t T_, int g, bool valid, float px, float py,
                                          float pz, float4 r0, float4 r1, float4 r2, float4 r3, float f[16]) {
+  if constexpr (CP) {
+    // feature pair q of owner group o is level (q < 2 ? 2 o + q : 8 + 2 o + q - 2);
+    // q unrolled (static indices into f), o a loop (few live registers)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
 #pragma unroll 1
-  for (int q = 0; q < 4; ++q) {
+    for (int o = 0; o < 4; ++o) {
+      const int lv = q < 2 ? 2 * o + q : 8 + 2 * o + (q - 2);
+      Cell c = cell_of(a.G, lv, px, py, pz);
+      const uint32_t* tl = a.table16 + size_t(lv) * T_;
+      const bool pair = ((a.G.dense_mask >> lv) & 1u) || (c.c[0] & 3) != 3;
+      const int cy = c.c[1] + (g & 1), cz = c.c[2] + (g >> 1);
+      u2a u = *reinterpret_cast<const u2a*>(tl + grid_index(a.G, lv, c.c[0], cy, cz));
+      if (!pair) u.y = tl[grid_index(a.G, lv, c.c[0] + 1, cy, cz)];
+      float e0, e1;
+      unpack_entry(u.x, &e0, &e1);
+      float w = corner_w(c, 2 * g);
+      float s0 = w * e0, s1 = w * e1;
+      unpack_entry(u.y, &e0, &e1);
+      w = corner_w(c, 2 * g + 1);
+      s0 = __builtin_fmaf(w, e0, s0);
+      s1 = __builtin_fmaf(w, e1, s1);
+      s0 += __shfl_xor(s0, 16);
+      s1 += __shfl_xor(s1, 16);
+      s0 += __shfl_xor(s0, 32);
+      s1 += __shfl_xor(s1, 32);
+      if (o == g) {
+        f[2 * q] = valid ? s0 : 0.0f;
+        f[2 * q + 1] = valid ? s1 : 0.0f;
+      }
+    }
+  }
+#pragma unroll 1
+  for (int q = 0; q < 4 && !CP; ++q) {
     int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
     Cell c = cell_of(a.G, lv, px, py, pz);
     float s0 = 0.0f, s1 = 0.0f;
@@ -395,7 +442,7 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
     float cur[4][4];  // current layer input, feature 16 mt + 4 g + j
     {
       float f[16];
-      encode16(a, T_, g, valid, px, py, pz, r0, r1, r2, r3, f);
+      encode16<!TRAIN>(a, T_, g, valid, px, py, pz, r0, r1, r2, r3, f);
 #pragma unroll
       for (int i = 0; i < 16; ++i) cur[2 * (i >> 3) + ((i & 7) >> 2)][i & 3] = f[i];
     }
@@ -877,6 +924,12 @@ struct aqua_nrc {
   Buf<float4> rec_buf, tgt_buf;
   Buf<float> out_buf;
   hipEvent_t ev[8] = {};
+  // checkpoint (aqua_nrc_checkpoint): parameters, Adam moments, FP16 table,
+  // activation amax and the step count
+  Buf<float> ck_tmaster, ck_tm, ck_tv, ck_mmaster, ck_mm, ck_mv;
+  Buf<uint32_t> ck_t16, ck_amax;
+  uint32_t ck_step = 0;
+  bool ck_valid = false;
 
   ~aqua_nrc() {
     if (stream) {
@@ -1220,6 +1273,46 @@ int aqua_nrc_get_params(aqua_nrc* nrc, float* table, float* mlp) {
                           nrc->stream));
   if (mlp) HIPCHK(hipMemcpyAsync(mlp, nrc->mmaster.p, kMlp * sizeof(float), hipMemcpyDeviceToHost, nrc->stream));
   HIPCHK(hipStreamSynchronize(nrc->stream));
+  return AQUA_OK;
+}
+
+int aqua_nrc_checkpoint(aqua_nrc* nrc, int restore) {
+  if (!nrc) return aqua::fail(AQUA_ERR_INVALID_ARG, "null nrc");
+  HIPCHK(hipSetDevice(nrc->device));
+  hipStream_t s = nrc->stream;
+  HIPCHK(hipDeviceSynchronize());  // the cache may have been used on any stream
+  const size_t nt = 2 * nrc->entries;
+  struct Pair {
+    void* live;
+    void* saved;
+    size_t bytes;
+  };
+  if (!restore) {
+    if (int e = nrc->ck_tmaster.alloc(nt)) return e;
+    if (int e = nrc->ck_tm.alloc(nt)) return e;
+    if (int e = nrc->ck_tv.alloc(nt)) return e;
+    if (int e = nrc->ck_t16.alloc(nrc->entries)) return e;
+    if (int e = nrc->ck_mmaster.alloc(kMlp)) return e;
+    if (int e = nrc->ck_mm.alloc(kMlp)) return e;
+    if (int e = nrc->ck_mv.alloc(kMlp)) return e;
+    if (int e = nrc->ck_amax.alloc(8)) return e;
+  } else if (!nrc->ck_valid) {
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "no checkpoint to restore");
+  }
+  const Pair pairs[] = {{nrc->tmaster.p, nrc->ck_tmaster.p, nt * 4},   {nrc->tm.p, nrc->ck_tm.p, nt * 4},
+                        {nrc->tv.p, nrc->ck_tv.p, nt * 4},             {nrc->t16.p, nrc->ck_t16.p, nrc->entries * 4},
+                        {nrc->mmaster.p, nrc->ck_mmaster.p, kMlp * 4}, {nrc->mm.p, nrc->ck_mm.p, kMlp * 4},
+                        {nrc->mv.p, nrc->ck_mv.p, kMlp * 4},           {nrc->act_amax.p, nrc->ck_amax.p, 5 * 4}};
+  for (const Pair& p : pairs)
+    HIPCHK(hipMemcpyAsync(restore ? p.live : p.saved, restore ? p.saved : p.live, p.bytes, hipMemcpyDeviceToDevice, s));
+  if (restore) {
+    nrc->step = nrc->ck_step;
+    if (int e = nrc->repack(s)) return e;  // operand packs and FP8 scales of the restored weights
+  } else {
+    nrc->ck_step = nrc->step;
+    nrc->ck_valid = true;
+  }
+  HIPCHK(hipStreamSynchronize(s));
   return AQUA_OK;
 }
 

@@ -991,6 +991,8 @@ struct aqua_renderer {
   aqua_nrc* nrc = nullptr;
   bool nrc_enabled = true;
   uint64_t nrc_steps = 0;  // Adam steps the renderer has taken
+  uint64_t ck_nrc_steps = 0;  // ... and cache on / off at the last checkpoint
+  bool ck_nrc_enabled = true;
   float bmin[3] = {0, 0, 0}, bscale[3] = {1, 1, 1};
   DevBuf<float4> trec, taux, brec, btgt;  // training slots of a frame's first pass, batch
   uint32_t batch_n = 0;
@@ -1947,6 +1949,21 @@ int aqua_renderer_set_nrc_enabled(aqua_renderer* r, int enabled) {
   if (!r) return aqua::fail(AQUA_ERR_INVALID_ARG, "null renderer");
   if (!r->nrc && enabled) return aqua::fail(AQUA_ERR_INVALID_ARG, "the renderer has no cache (type is not nrc)");
   r->nrc_enabled = enabled != 0;
+  return AQUA_OK;
+}
+
+int aqua_renderer_nrc_checkpoint(aqua_renderer* r, int restore) {
+  if (!r) return aqua::fail(AQUA_ERR_INVALID_ARG, "null renderer");
+  if (!r->nrc) return aqua::fail(AQUA_ERR_INVALID_ARG, "the renderer has no cache (type is not nrc)");
+  HIPCHK(hipSetDevice(r->device));
+  if (int e = aqua_nrc_checkpoint(r->nrc, restore)) return e;
+  if (restore) {
+    r->nrc_steps = r->ck_nrc_steps;
+    r->nrc_enabled = r->ck_nrc_enabled;
+  } else {
+    r->ck_nrc_steps = r->nrc_steps;
+    r->ck_nrc_enabled = r->nrc_enabled;
+  }
   return AQUA_OK;
 }
 

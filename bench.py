@@ -1,27 +1,34 @@
-"""Benchmark: plain GPU path tracer on the 1M-triangle interior at 1080p.
+"""Benchmark: plain GPU path tracer on the 1M-triangle interior at 1080p, and
+the neural-radiance-cache legs the headline metric is quoted for.
 
 Workload (one step = one frame): procedural interior of 150 seeded meshes
-(~1.09 M FP32 triangles, GGX metallic-roughness materials, 3 quad area lights),
-1920x1080, 64 spp, max depth 8, binned-SAH BVH (32 bins). The scene is written
-in the reference's schema (scene JSON + BSON .mesh), loaded, and uploaded once;
-every timed frame traces all 64 samples of every pixel with the scene already
-resident in HBM.
+(~1.09 M FP32 triangles, GGX metallic-roughness materials, 3 quad area
+lights), 1920x1080, 64 spp, max depth 8, binned-SAH BVH (32 bins). The scene
+is written in the reference's schema (scene JSON + BSON .mesh), loaded, and
+uploaded once; every timed frame traces all 64 samples of every pixel with
+the scene already resident in HBM.
 
 Prints one JSON line: Mrays/s (camera + extension + shadow rays per second of
-wall time, the shadow rays of each bounce overlapping the next bounce's
-closest-hit rays), ms/frame, the roofline of the closest-hit traversal kernel
-(its launch time from HIP events in one more frame whose kernels run one at a
-time, algorithmic bytes from an instrumented frame),
-and the CPU restatement's rate on a bounded sample of the same frame. Unless
---no-nrc, two cache legs follow: one training step on 2^18 samples
-("nrc_train") and the ms/frame-to-relMSE-1e-2 comparison of the cache against
-the plain tracer on the box-room fly-through ("nrc_eval", whose "nrc_over_pt"
-is the cache's time over the tracer's, querying at the second path vertex).
+wall time), ms/frame, the frame's relMSE against a 256-spp FP64 render of the
+same view (GPU, FP64 accumulation), the roofline of the closest-hit traversal
+kernel, and the CPU restatement's rate on a bounded sample of the same frame
+on all host cores. Unless --no-nrc, further legs follow:
+  box_room       the 36-triangle box room, 512x512, 16 spp, depth 5, with its
+                 relMSE against a 4096-spp FP64 render;
+  nrc_train      one cache training step on 2^18 tracer samples;
+  nrc_eval       ms/frame to relMSE <= 1e-2 against a 4096-spp FP64 render,
+                 plain tracer vs cache (FP16 and BF16, query at vertex 2; FP16
+                 at vertex 1 beside it) on the box-room fly-through at 1080p;
+  large          the 400-mesh / 5.4 M-triangle interior at 3840x2160, 1 spp
+                 with an FP8 (e4m3) and an FP16 cache over a 60-frame
+                 fly-through, error against a 64-spp FP64 render.
 """
 import argparse
+import hashlib
 import importlib.util
 import json
 import os
+import platform
 import sys
 import tempfile
 import time
@@ -30,11 +37,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 
 METRIC = "ms/frame to relMSE 1e-2 vs FP64 oracle (NRC vs plain PT); Mrays/s"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per
+# SIMD every 4 clocks (16 lanes), at 2.4 GHz.
+VALU_PEAK_GINSTR = 256 * 4 * 2.4 / 4.0
 # Algorithmic bytes of one closest-hit ray: the ray (origin+tmax, direction:
 # 32 B) read once, the hit record (slot, t: 8 B) written once, one 64-byte
 # quantised 4-wide node per node visit and one 48-byte triangle per triangle
 # test.
 RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 32, 8, 64, 48
+# The sources the traversal kernel's instruction stream comes from: a
+# committed counter profile counts only while their digest matches.
+TRAVERSAL_SOURCES = ("aqua-engine_amd/csrc/device/trace.h", "aqua-engine_amd/csrc/device/common.h",
+                     "aqua-engine_amd/csrc/renderer.hip")
+VALU_PROFILE = "profiles/k_intersect_interior1m_1080p_64spp_valu.json"
+TRAFFIC_PROFILE = "profiles/k_intersect_interior1m_1080p_64spp_traffic.json"
 
 
 def _load(name, path):
@@ -45,8 +61,39 @@ def _load(name, path):
     return mod
 
 
+def traversal_digest():
+    h = hashlib.sha256()
+    for rel in TRAVERSAL_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def _profile(rel):
+    """A committed counter summary, or None when absent or measured on other
+    traversal sources than these."""
+    try:
+        with open(os.path.join(ROOT, rel)) as f:
+            p = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return p if p.get("source_digest") == traversal_digest() else None
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(scene, aqua, max_depth, target_s=12.0):
-    """CPU restatement (oracle/) on a band of rows of the same frame at 1 spp."""
+    """CPU restatement (oracle/) on a band of rows of the same frame, on every
+    host core the process may use."""
     orc = _load("aqua_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
     v, m = scene.triangles()
     t0 = time.perf_counter()
@@ -54,7 +101,7 @@ def cpu_baseline(scene, aqua, max_depth, target_s=12.0):
     build_s = time.perf_counter() - t0
     cam = scene.camera()
     cam16 = aqua.camera_to_array(cam)
-    threads = min(16, os.cpu_count() or 1)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     mid = cam.height // 2
     rows, spp = 8, 1
     while True:
@@ -73,22 +120,40 @@ def cpu_baseline(scene, aqua, max_depth, target_s=12.0):
         "value": round(st.rays / dt / 1e6, 3),
         "unit": "Mrays/s",
         "cores": threads,
+        "nproc": os.cpu_count(),
+        "cpu": _cpu_model(),
         "kind": "port",
         "sample": f"rows {y0}-{y1} of the {cam.width}x{cam.height} frame at {spp} spp, max depth {max_depth} "
-                  f"({st.rays} rays in {dt:.2f} s; its own BVH build, {build_s:.2f} s, not timed)",
+                  f"({st.rays} rays in {dt:.2f} s on {threads} threads; its own BVH build, {build_s:.2f} s, "
+                  f"not timed)",
     }
 
 
-def _traffic():
-    """HBM bytes per k_intersect launch of this workload, from the committed
-    rocprofv3 FETCH_SIZE + WRITE_SIZE summary (counters scaled by a plain copy
-    of known size), or None when that summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_intersect_interior1m_1080p_64spp_traffic.json")
-    try:
-        with open(path) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except (OSError, ValueError, KeyError):
-        return None
+def _relmse(img, ref):
+    import numpy as np
+    return float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def box_room_leg(aqua, oracle_spp=4096, steps=10):
+    """The 36-triangle box room at 512x512, 16 spp, depth 5 (PCG seed 0)."""
+    with tempfile.TemporaryDirectory() as d:
+        scene = aqua.Scene.load(aqua.procgen(aqua.PROCGEN_BOX_ROOM, d))
+    cfg = aqua.integrator_default()
+    cfg.spp, cfg.max_depth = 16, 5
+    r = aqua.Renderer(scene, cfg)
+    r.set_opts(timing=False)
+    for w in range(2):
+        r.render(16, frame=w)
+    rays, t0 = 0, time.perf_counter()
+    for k in range(steps):
+        st = r.render(16, frame=0)
+        rays += st.rays
+    wall = time.perf_counter() - t0
+    r.render_oracle(0, oracle_spp, frame=777)
+    e, _ = r.film_error()
+    return {"scene": "box room, 36 triangles, 512x512, 16 spp, max depth 5",
+            "ms_per_frame": round(wall * 1e3 / steps, 3), "mrays_per_s": round(rays / wall / 1e6, 1),
+            "relmse_vs_fp64": round(e, 6), "oracle_spp": oracle_spp}
 
 
 def nrc_train_leg(aqua, rows=1 << 18, steps=20, precision=0):
@@ -138,16 +203,14 @@ def nrc_train_leg(aqua, rows=1 << 18, steps=20, precision=0):
     }
 
 
-def _relmse(img, ref):
-    import numpy as np
-    return float(np.mean((img.astype(np.float64) - ref) ** 2 / (ref ** 2 + 1e-2)))
-
-
-def _ladder(render, oracle, max_spp):
-    """Smallest spp on the ladder 1, 2, 4, ... whose frame reaches relMSE <= 1e-2."""
+def _ladder(render, oracle, max_spp, before=None):
+    """Smallest spp on the ladder 1, 2, 4, ... whose frame reaches relMSE <= 1e-2.
+    `before` runs ahead of every rung (the cache puts its state back there)."""
     res = None
     spp = 1
     while spp <= max_spp:
+        if before:
+            before()
         st, img = render(spp)
         e = _relmse(img, oracle)
         res = {"spp": spp, "relmse": round(e, 5), "ms_per_frame": round(st.ms_frame, 3)}
@@ -159,15 +222,21 @@ def _ladder(render, oracle, max_spp):
     return res
 
 
-def nrc_eval_leg(aqua, frames=tuple(range(15, 240, 30)), n_frames=240, oracle_spp=1024, width=1920,
-                 height=1080, max_spp=256, precision=0):
-    """ms/frame to relMSE <= 1e-2 against an FP64 oracle, plain path tracer vs
-    cache, on the box room fly-through: max depth 8, 2^16 training suffix
-    paths per frame, online training at 1 spp through every frame before an
-    evaluated one. The cache queries at path vertex 2 (the configuration the
-    ratio is quoted for); a second cache querying at vertex 1 (the primary
-    hit, so the image carries the cache's bias instead of the indirect
-    noise) is reported beside it."""
+def nrc_eval_leg(aqua, frames=tuple(range(15, 240, 30)), n_frames=240, oracle_spp=4096, width=1920,
+                 height=1080, max_spp=256):
+    """ms/frame to relMSE <= 1e-2 against an FP64 render of `oracle_spp` samples,
+    plain path tracer vs cache, on the box room fly-through: max depth 8, 2^16
+    training suffix paths per frame, online training at 1 spp through every
+    frame before an evaluated one.
+
+    Each evaluated frame uses three disjoint sample sets: the oracle renders
+    frame index f, the plain tracer f + 100000, the cache f + 200000. The
+    cache's state is checkpointed after the frames before f and restored
+    before every rung of the spp ladder, so each rung sees exactly the cache
+    those earlier frames trained -- and pays for its own training step, as
+    in online use. The caches query at path vertex 2 in FP16 and in BF16 (the
+    configuration the ratio is quoted for); an FP16 cache querying at vertex
+    1 is reported beside them."""
     with tempfile.TemporaryDirectory() as d:
         scene = aqua.Scene.load(aqua.procgen(aqua.PROCGEN_BOX_ROOM, d, width=width, height=height))
     pt = aqua.integrator_default()
@@ -175,64 +244,75 @@ def nrc_eval_leg(aqua, frames=tuple(range(15, 240, 30)), n_frames=240, oracle_sp
     ref = aqua.Renderer(scene, pt)
     ref.set_opts(timing=False)  # ms/frame from the frame's own events; kernels overlapped
     caches = {}
-    for qv in (2, 1):
+    for key, qv, prec in (("nrc_v2_f16", 2, aqua.NRC_F16), ("nrc_v2_bf16", 2, aqua.NRC_BF16),
+                          ("nrc_v1_f16", 1, aqua.NRC_F16)):
         c = aqua.integrator_default()
         c.type = aqua.INTEGRATOR_NRC
         c.max_depth = 8
         c.learning_rate = 1e-2
-        c.nrc_precision = precision
+        c.nrc_precision = prec
         c.nrc_train_paths = 1 << 16
         c.nrc_query_vertex = qv
-        caches[qv] = aqua.Renderer(scene, c)
-        caches[qv].set_opts(timing=False)
+        caches[key] = aqua.Renderer(scene, c)
+        caches[key].set_opts(timing=False)
     out = []
     next_frame = 0
+    t_oracle = 0.0
     for fe in frames:
         cam = aqua.camera_flythrough(scene, fe, n_frames, width=width, height=height)
         for f in range(next_frame, fe):
             cf = aqua.camera_flythrough(scene, f, n_frames, width=width, height=height)
             for r in caches.values():
                 r.render(1, frame=f, camera=cf)
-        next_frame = fe + 1
+        next_frame = fe
+        t0 = time.perf_counter()
         ref.render_oracle(0, oracle_spp, frame=fe, camera=cam)
         oracle = ref.oracle_film(cam)
+        t_oracle += time.perf_counter() - t0
         row = {"frame": fe}
 
         def pt_render(spp):
             st = ref.render(spp, frame=fe + 100000, camera=cam)
             return st, ref.film(cam)
         row["pt"] = _ladder(pt_render, oracle, max_spp)
-        for qv, r in caches.items():
+        for key, r in caches.items():
+            r.nrc_checkpoint()
+
             def nrc_render(spp, r=r):
-                st = r.render(spp, frame=fe, camera=cam)
+                st = r.render(spp, frame=fe + 200000, camera=cam)
                 return st, r.film(cam)
-            row["nrc_v%d" % qv] = _ladder(nrc_render, oracle, max_spp)
+            row[key] = _ladder(nrc_render, oracle, max_spp, before=lambda r=r: r.nrc_checkpoint(restore=True))
+            r.nrc_checkpoint(restore=True)   # the online run goes on from the frames before fe
         out.append(row)
     pt_ms = sum(o["pt"]["ms_per_frame"] for o in out) / len(out)
     res = {
         "scene": "box room (36 tris), fly-through of %d frames, %dx%d, max depth 8" % (n_frames, width, height),
         "frames_evaluated": list(frames),
         "oracle_spp": oracle_spp,
-        "precision": ["f16", "bf16", "fp8"][precision],
+        "oracle_s_per_frame": round(t_oracle / len(frames), 2),
+        "sample_sets": "oracle frame f, plain tracer f + 100000, cache f + 200000; cache state restored per rung",
         "frames": out,
         "pt_ms_to_1e-2": round(pt_ms, 3),
+        "pt_reached": all(o["pt"]["relmse"] <= 1e-2 for o in out),
     }
-    for qv in caches:
-        k = "nrc_v%d" % qv
-        ms = sum(o[k]["ms_per_frame"] for o in out) / len(out)
-        res[k + "_ms_to_1e-2"] = round(ms, 3)
-        res[k + "_over_pt"] = round(ms / pt_ms, 4) if pt_ms > 0 else None
-        res[k + "_reached"] = all(o[k]["relmse"] <= 1e-2 for o in out)
-    res["nrc_over_pt"] = res["nrc_v2_over_pt"]
-    res["pt_reached"] = all(o["pt"]["relmse"] <= 1e-2 for o in out)
+    for key in caches:
+        ms = sum(o[key]["ms_per_frame"] for o in out) / len(out)
+        res[key + "_ms_to_1e-2"] = round(ms, 3)
+        res[key + "_over_pt"] = round(ms / pt_ms, 4) if pt_ms > 0 else None
+        res[key + "_reached"] = all(o[key]["relmse"] <= 1e-2 for o in out)
+    res["nrc_over_pt"] = res["nrc_v2_f16_over_pt"]  # the configured default: FP16, vertex 2
     return res
 
 
-def large_leg(aqua, meshes=400, tris=5000000, width=3840, height=2160, frames=60, oracle_spp=64):
-    """Large interior (about 5 M triangles, 3840x2160): 1 spp per frame with an
-    online cache over a `frames`-frame fly-through, once with FP8 (e4m3)
-    inference + BF16 training and once with FP16 throughout; then the last
-    frame's relMSE of both against an FP64 oracle, and of FP8 against FP16."""
+def large_leg(aqua, meshes=400, tris=5000000, width=3840, height=2160, frames=60, oracle_spp=64,
+              eval_frames=(50, 55, 59)):
+    """Large interior (about 5.4 M triangles, 3840x2160): 1 spp per frame with
+    an online cache over a `frames`-frame fly-through, once with FP8 (e4m3)
+    queries + BF16 training and once with FP16 throughout. At the evaluated
+    frames each cache's frame is compared with a 64-spp FP64 render and with
+    the other cache's; the plain tracer's 1-spp frame on the same samples
+    is compared too. The caches take the same number of Adam steps (the
+    frames' schedule); relMSE figures are means over the evaluated frames."""
     import numpy as np
     t0 = time.perf_counter()
     with tempfile.TemporaryDirectory() as d:
@@ -246,10 +326,12 @@ def large_leg(aqua, meshes=400, tris=5000000, width=3840, height=2160, frames=60
     t0 = time.perf_counter()
     ref = aqua.Renderer(scene, pt)
     build_s = time.perf_counter() - t0
+    ref.set_opts(timing=False)
     res = {"scene": "procedural interior, %d meshes, %d triangles, %dx%d, max depth 8" % (
                meshes, info.n_triangles, width, height),
-           "load_s": round(load_s, 2), "bvh_build_s": round(build_s, 2), "frames": frames}
-    cam_last = aqua.camera_flythrough(scene, frames - 1, frames, width=width, height=height)
+           "load_s": round(load_s, 2), "bvh_build_s": round(build_s, 2), "frames": frames,
+           "eval_frames": list(eval_frames), "oracle_spp": oracle_spp}
+    cams = [aqua.camera_flythrough(scene, f, frames, width=width, height=height) for f in range(frames)]
     imgs = {}
     for name, prec in (("fp8", aqua.NRC_FP8), ("f16", aqua.NRC_F16)):
         c = aqua.integrator_default()
@@ -259,33 +341,39 @@ def large_leg(aqua, meshes=400, tris=5000000, width=3840, height=2160, frames=60
         c.nrc_precision = prec
         c.nrc_train_paths = 1 << 16
         r = aqua.Renderer(scene, c)
+        r.set_opts(timing=False)
         ms, ms_cache, rays, wall = [], [], 0, 0.0
+        imgs[name] = {}
         for f in range(frames):
-            cam = aqua.camera_flythrough(scene, f, frames, width=width, height=height)
             t1 = time.perf_counter()
-            st = r.render(1, frame=f, camera=cam)
+            st = r.render(1, frame=f, camera=cams[f])
             wall += time.perf_counter() - t1
             if f >= 2:
                 ms.append(st.ms_frame)
-                ms_cache.append(st.ms_nrc)
             rays += st.rays
-        imgs[name] = r.film(cam_last).astype(np.float64)
+            if f in eval_frames:
+                imgs[name][f] = r.film(cams[f]).astype(np.float64)
         res[name] = {"ms_per_frame": round(float(np.mean(ms)), 3),
-                     "ms_cache_per_frame": round(float(np.mean(ms_cache)), 3),
                      "mrays_per_s": round(rays / wall / 1e6, 1),
                      "queries_last_frame": int(st.nrc_queries),
                      "loss_last_frame": round(float(st.nrc_loss), 5)}
         del r
-    st = ref.render(1, frame=frames - 1 + 100000, camera=cam_last)
-    pt1 = ref.film(cam_last).astype(np.float64)
-    res["pt_1spp_ms"] = round(st.ms_frame, 3)
-    ref.render_oracle(0, oracle_spp, frame=frames - 1, camera=cam_last)
-    oracle = ref.oracle_film(cam_last)
-    res["oracle_spp"] = oracle_spp
-    res["relmse_vs_oracle"] = {"pt_1spp": round(_relmse(pt1, oracle), 5),
-                               "nrc_fp8": round(_relmse(imgs["fp8"], oracle), 5),
-                               "nrc_f16": round(_relmse(imgs["f16"], oracle), 5)}
-    res["relmse_fp8_vs_f16"] = round(_relmse(imgs["fp8"], imgs["f16"]), 6)
+    e = {"pt_1spp": [], "nrc_fp8": [], "nrc_f16": [], "fp8_vs_f16": []}
+    pt_ms = []
+    for f in eval_frames:
+        st = ref.render(1, frame=f, camera=cams[f])
+        pt_ms.append(st.ms_frame)
+        pt1 = ref.film(cams[f]).astype(np.float64)
+        ref.render_oracle(0, oracle_spp, frame=f + 300000, camera=cams[f])
+        oracle = ref.oracle_film(cams[f])
+        e["pt_1spp"].append(_relmse(pt1, oracle))
+        e["nrc_fp8"].append(_relmse(imgs["fp8"][f], oracle))
+        e["nrc_f16"].append(_relmse(imgs["f16"][f], oracle))
+        e["fp8_vs_f16"].append(_relmse(imgs["fp8"][f], imgs["f16"][f]))
+    res["pt_1spp_ms"] = round(float(np.mean(pt_ms)), 3)
+    res["relmse_vs_fp64"] = {k: round(float(np.mean(v)), 5) for k, v in e.items() if k != "fp8_vs_f16"}
+    res["relmse_fp8_vs_f16"] = round(float(np.mean(e["fp8_vs_f16"])), 6)
+    res["relmse_per_frame"] = {k: [round(x, 5) for x in v] for k, v in e.items()}
     return res
 
 
@@ -302,17 +390,15 @@ def main():
     ap.add_argument("--max-depth", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-nrc", action="store_true",
-                    help="skip the cache legs (training step, ms/frame-to-relMSE evaluation)")
-    ap.add_argument("--nrc-train", action="store_true", help="(default on) the cache training-step leg")
-    ap.add_argument("--nrc-eval", action="store_true", help="(default on) the ms/frame-to-relMSE leg")
-    ap.add_argument("--only-nrc", action="store_true", help="skip the path-tracer workload")
-    ap.add_argument("--large", action="store_true",
-                    help="add the 5 M-triangle 3840x2160 leg (FP8 vs FP16 cache)")
+                    help="skip the extra legs (box room, cache training step, cache evaluation, large scene)")
+    ap.add_argument("--only-nrc", action="store_true", help="skip the interior workload, run the extra legs")
+    ap.add_argument("--no-large", action="store_true", help="skip the 5.4 M-triangle 3840x2160 leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (the reference kernel for HBM counters)")
+    ap.add_argument("--profile-frame", action="store_true",
+                    help="render frame 0 of the interior workload once (kernels overlapped, as in the timed "
+                         "frames), print its ray counts as JSON and exit: the run counter profiles use")
     args = ap.parse_args()
-    if not args.no_nrc and not args.only_nrc:
-        args.nrc_train = args.nrc_eval = True
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the path tracer would shard by interleaved pixel tiles, one process per GPU)", file=sys.stderr)
@@ -333,14 +419,21 @@ def main():
     if aqua.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+
+    def extra_legs(out):
+        print("bench: box room leg", file=sys.stderr, flush=True)
+        out["box_room"] = box_room_leg(aqua)
+        print("bench: cache training leg", file=sys.stderr, flush=True)
+        out["nrc_train"] = nrc_train_leg(aqua)
+        print("bench: cache evaluation leg", file=sys.stderr, flush=True)
+        out["nrc_eval"] = nrc_eval_leg(aqua)
+        if not args.no_large:
+            print("bench: large-scene leg", file=sys.stderr, flush=True)
+            out["large"] = large_leg(aqua)
+
     if args.only_nrc:
         out = {}
-        if args.nrc_train:
-            out["nrc_train"] = nrc_train_leg(aqua)
-        if args.nrc_eval:
-            out["nrc_eval"] = nrc_eval_leg(aqua)
-        if args.large:
-            out["large"] = large_leg(aqua)
+        extra_legs(out)
         print(json.dumps(out))
         return
 
@@ -355,15 +448,25 @@ def main():
     r = aqua.Renderer(scene, cfg, device=0)
     setup_s = time.perf_counter() - t0
 
+    if args.profile_frame:
+        r.set_opts(count_traversal=False, timing=False)
+        st = r.render(args.spp, frame=0)
+        print(json.dumps({"frame": 0, "closest_rays": st.camera_rays + st.extension_rays,
+                          "shadow_rays": st.shadow_rays, "launches": st.n_intersect_launches,
+                          "ms_frame": round(st.ms_frame, 3), "source_digest": traversal_digest()}))
+        return
+
     # Instrumented frame (untimed): nodes / triangles per closest-hit ray.
     r.set_opts(count_traversal=True, timing=False)
     cst = r.render(args.spp, frame=0)
     closest_rays = cst.camera_rays + cst.extension_rays
-    bytes_per_ray = (RAY_BYTES + HIT_BYTES + NODE_BYTES * cst.nodes_visited / closest_rays
-                     + TRI_BYTES * cst.tris_tested / closest_rays)
+    nodes_per_ray = cst.nodes_visited / closest_rays
+    tris_per_ray = cst.tris_tested / closest_rays
+    bytes_per_ray = RAY_BYTES + HIT_BYTES + NODE_BYTES * nodes_per_ray + TRI_BYTES * tris_per_ray
 
     # Timed frames: no per-kernel events, so the shadow rays of each bounce
     # overlap the next bounce's closest-hit rays (wall clock only).
+    print("bench: interior, timed frames", file=sys.stderr, flush=True)
     r.set_opts(count_traversal=False, timing=False)
     for w in range(args.warmup):
         r.render(args.spp, frame=w)
@@ -382,15 +485,53 @@ def main():
     ms_isect = st.ms_intersect
     launches = st.n_intersect_launches
     isect_rays = st.camera_rays + st.extension_rays
+    # The frame's error against a 256-spp FP64 render of the same view
+    # (its own sample set), reduced on the GPU in FP64.
+    r.render_oracle(0, 256, frame=777)
+    r.set_opts(count_traversal=False, timing=False)
+    r.render(args.spp, frame=0)
+    relmse, relrmse = r.film_error()
 
     avg_launch_ms = ms_isect / max(1, launches)
-    bytes_per_launch = bytes_per_ray * isect_rays / max(1, launches)
-    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
-    nodes_per_launch = cst.nodes_visited / closest_rays * isect_rays / max(1, launches)
-    node_rate = nodes_per_launch / (avg_launch_ms * 1e-3) if avg_launch_ms > 0 else 0.0
+    rays_per_launch = isect_rays / max(1, launches)
+    bytes_per_launch = bytes_per_ray * rays_per_launch
+    alg_gbs = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    node_rate = nodes_per_ray * rays_per_launch / (avg_launch_ms * 1e-3) if avg_launch_ms > 0 else 0.0
 
-    traffic = _traffic()
-    hbm_gbs = traffic / (avg_launch_ms * 1e-3) / 1e9 if (traffic and avg_launch_ms > 0) else None
+    valu = _profile(VALU_PROFILE)
+    traffic = _profile(TRAFFIC_PROFILE)
+    roof = {"kernel": "k_intersect (closest hit)", "bound": "valu", "unit": "G VALU instr/s",
+            "peak": VALU_PEAK_GINSTR, "achieved": None, "frac": None,
+            "peak_derivation": "256 CUs x 4 SIMDs x 2.4 GHz / 4 clocks per wave64 VALU instruction",
+            "source_digest": traversal_digest()}
+    if valu:
+        # wave64 VALU instructions per closest-hit ray (PMC SQ_INSTS_VALU of
+        # the same workload) x rays per launch / the launch time measured here
+        ginstr = valu["valu_insts_per_ray"] * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9
+        roof.update(achieved=round(ginstr, 1), frac=round(ginstr / VALU_PEAK_GINSTR, 4),
+                    valu_insts_per_ray=valu["valu_insts_per_ray"],
+                    lane_utilisation=valu["lane_utilisation"], valu_source=VALU_PROFILE)
+    roof["traffic"] = traffic["hbm_bytes_per_launch"] if traffic else None
+    roof.update({
+        "traffic_source": TRAFFIC_PROFILE if traffic else None,
+        "hbm_achieved": round(roof["traffic"] / (avg_launch_ms * 1e-3) / 1e9, 1) if traffic else None,
+        "hbm_peak": HBM_PEAK_GBS,
+        "algorithmic_bytes_per_launch": round(bytes_per_launch),
+        "algorithmic_gbs": round(alg_gbs, 1),
+        "bytes_per_ray": round(bytes_per_ray, 1),
+        "nodes_per_ray": round(nodes_per_ray, 2),
+        "tris_per_ray": round(tris_per_ray, 2),
+        "avg_launch_ms": round(avg_launch_ms, 4),
+        "launches": launches,
+        "rays_per_launch": round(rays_per_launch),
+        "node_fetch_rate": {"nodes_per_s": round(node_rate),
+                            "hbm_fed_nodes_per_s": round(HBM_PEAK_GBS * 1e9 / NODE_BYTES)},
+        "note": "traversal is bound by VALU issue (divergent node / leaf steps); most node and triangle "
+                "bytes come from LDS and L2, so the algorithmic byte rate can pass HBM's peak while the "
+                "measured DRAM traffic (traffic / launch time) stays well below it",
+    })
+    if traffic:
+        roof["hbm_frac"] = round(roof["hbm_achieved"] / HBM_PEAK_GBS, 4)
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -416,47 +557,17 @@ def main():
             "rays_per_frame": rays // args.steps,
             "ms_per_frame": round(ms_per_step, 3),
             "setup_s": round(setup_s, 3),
+            "relmse_vs_fp64_256spp": round(relmse, 6),
+            "relrmse_vs_fp64_256spp": round(relrmse, 6),
         },
-        "roofline": {
-            "kernel": "k_intersect (closest hit)",
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/k_intersect_interior1m_1080p_64spp_traffic.json",
-            # `achieved` counts every node and triangle byte a ray reads; most
-            # are served by LDS (top levels) and L2, so it can pass HBM's
-            # peak. The measured L2-miss traffic over the same launch time:
-            "hbm_achieved": round(hbm_gbs, 1) if hbm_gbs else None,
-            "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4) if hbm_gbs else None,
-            "note": "algorithmic bytes / launch time exceeds HBM peak when caches serve them; "
-                    "hbm_achieved is the measured DRAM traffic; traversal is VALU-issue bound",
-            "algorithmic_bytes_per_launch": round(bytes_per_launch),
-            "bytes_per_ray": round(bytes_per_ray, 1),
-            "nodes_per_ray": round(cst.nodes_visited / closest_rays, 2),
-            "tris_per_ray": round(cst.tris_tested / closest_rays, 2),
-            "avg_launch_ms": round(avg_launch_ms, 4),
-            "launches": launches,
-            # node fetches per second against the rate HBM alone could feed
-            # (8 TB/s / 64-byte nodes = 125 G nodes/s)
-            "node_fetch_rate": {
-                "nodes_per_s": round(node_rate),
-                "peak_nodes_per_s": round(HBM_PEAK_GBS * 1e9 / NODE_BYTES),
-                "frac": round(node_rate / (HBM_PEAK_GBS * 1e9 / NODE_BYTES), 4),
-            },
-        },
+        "roofline": roof,
     }
     if not args.no_cpu_baseline:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(scene, aqua, args.max_depth)
     del r
-    if args.nrc_train:
-        out["nrc_train"] = nrc_train_leg(aqua)
-    if args.nrc_eval:
-        out["nrc_eval"] = nrc_eval_leg(aqua)
-    if args.large:
-        out["large"] = large_leg(aqua)
+    if not args.no_nrc:
+        extra_legs(out)
     print(json.dumps(out))
 
 

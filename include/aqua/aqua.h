@@ -347,6 +347,11 @@ int aqua_nrc_set_params(aqua_nrc* nrc, const float* table, const float* mlp);
  * 448 / amax |layer input| seen by the last training forward pass (1 before
  * any training). Diagnostics for checking the FP8 path. */
 int aqua_nrc_fp8_scales(aqua_nrc* nrc, float* wscale, float* ascale);
+/* restore == 0: saves the cache's whole training state on the device
+ * (parameters, Adam moments, step count, FP8 activation ranges); restore ==
+ * 1: puts the saved state back, so later queries and training steps run
+ * exactly as they would have from the save. One checkpoint per cache. */
+int aqua_nrc_checkpoint(aqua_nrc* nrc, int restore);
 /* Cache query: rgb[3i..3i+2] for each record (host arrays). */
 int aqua_nrc_infer(aqua_nrc* nrc, uint32_t n, const float* records, uint32_t precision, float* rgb);
 /* `steps` Adam steps on one batch (host arrays; targets 3 floats per row). */
@@ -367,6 +372,9 @@ aqua_nrc* aqua_renderer_nrc(aqua_renderer* r);
 /* 0 disables the cache (frames render as the plain path tracer, which is
  * also the fallback when training diverges); 1 re-enables it. */
 int aqua_renderer_set_nrc_enabled(aqua_renderer* r, int enabled);
+/* aqua_nrc_checkpoint of the renderer's cache, plus the renderer's own cache
+ * schedule (training steps taken, cache on / off), between frames. */
+int aqua_renderer_nrc_checkpoint(aqua_renderer* r, int restore);
 /* The last frame's training batch, device-resident on the renderer's device:
  * records (4 float4 per row) and targets (float4 per row). Valid until the
  * next frame. */

@@ -289,15 +289,17 @@ def test_training_gradients_at_full_table_size(gpu, prec):
     e_mlp = np.linalg.norm(gm - gm_r) / np.linalg.norm(gm_r)
     assert e_mlp < tol_mlp, e_mlp
     tight = 3e-3 if prec == aqua.NRC_F16 else 1e-2
-    assert max(errs[:4]) < tight, errs[:4]         # FP32 accumulation
-    assert max(errs[4:]) < 2e-2, errs[4:]          # BF16-pair accumulation
+    assert max(errs[:8]) < tight, errs[:8]         # FP32 accumulation
+    assert max(errs[8:]) < 2e-2, errs[8:]          # BF16-pair accumulation
 
 
 def test_bf16_pair_gradients_with_many_rows_per_entry(gpu):
-    # 2^18 rows packed into a cube of side 0.03: on the coarse hashed levels
-    # every entry collects thousands of rows, the worst case for the BF16
-    # pairs levels 4-15 accumulate in. The bound is on the relative L2 of
-    # each level's gradient against the float64 restatement.
+    # 2^18 rows packed into a cube of side 0.03: on the coarse levels every
+    # entry collects thousands of rows. Levels 0-7 accumulate FP32 pairs and
+    # must stay tight; the BF16 pairs of levels 8-15 (an 8-bit mantissa) drift
+    # with the rows per entry -- about a hundred at level 8 here, against the
+    # few a batch spread over a scene gives them. The bound is on the relative
+    # L2 of each level's gradient against the float64 restatement.
     rng = np.random.default_rng(6)
     n = 1 << 18
     rec = np.zeros((n, 16), np.float32)
@@ -323,8 +325,8 @@ def test_bf16_pair_gradients_with_many_rows_per_entry(gpu):
     gt_r = gt_r.reshape(16, -1)
     errs = [np.linalg.norm(gt[lv] - gt_r[lv]) / max(np.linalg.norm(gt_r[lv]), 1e-30) for lv in range(16)]
     print("clustered rows, table-gradient rel. L2 per level:", " ".join(f"{e:.1e}" for e in errs))
-    assert max(errs[:4]) < 1e-2, errs[:4]
-    assert max(errs[4:]) < 3e-2, errs[4:]
+    assert max(errs[:8]) < 1e-2, errs[:8]
+    assert max(errs[8:]) < 8e-2, errs[8:]
 
 
 # ------------------------------------- online cache at 1080p, 2^16 paths ---

@@ -164,3 +164,32 @@ def test_diverged_cache_falls_back_to_plain_tracing(gpu, box_room):
     st = r.render(4, camera=cam, frame=9)
     assert st.nrc_queries == 0 and st.nrc_train_records == 0
     np.testing.assert_array_equal(r.film(cam), a.film(cam))
+
+
+def test_cache_checkpoint_restores_the_training_state(gpu, box_room):
+    # A frame depends only on the cache state before it (its own training
+    # step follows the film), so after a checkpoint, more training and a
+    # restore, the same frame renders bit for bit as it did at the
+    # checkpoint; without the restore it does not.
+    cam = small_camera(box_room.camera(), 96, 64)
+    c = aqua.integrator_default()
+    c.type = aqua.INTEGRATOR_NRC
+    c.spp, c.max_depth, c.learning_rate, c.nrc_train_paths = 2, 8, 1e-2, 2048
+    r = aqua.Renderer(box_room, c)
+    for f in range(6):
+        r.render(2, frame=f, camera=cam)
+    r.nrc_checkpoint()
+    r.render(2, frame=100, camera=cam)
+    at_ckpt = r.film(cam)
+    for f in range(6, 12):
+        r.render(2, frame=f, camera=cam)
+    r.render(2, frame=100, camera=cam)
+    assert not np.array_equal(r.film(cam), at_ckpt)
+    r.nrc_checkpoint(restore=True)
+    r.render(2, frame=100, camera=cam)
+    np.testing.assert_array_equal(r.film(cam), at_ckpt)
+    # restoring twice from the same checkpoint gives the same frame again
+    r.render(2, frame=50, camera=cam)
+    r.nrc_checkpoint(restore=True)
+    r.render(2, frame=100, camera=cam)
+    np.testing.assert_array_equal(r.film(cam), at_ckpt)

@@ -269,6 +269,9 @@ def test_box_room_is_deterministic_and_shaped(tmp_path):
     aqua.procgen(aqua.PROCGEN_BOX_ROOM, str(a))
     aqua.procgen(aqua.PROCGEN_BOX_ROOM, str(b))
     assert _digest(a) == _digest(b)
+    # the committed digest of the emitted files (names + bytes, sorted by
+    # name): the same scene on every machine and every build
+    assert _digest(a) == "72cf38d4d185f4ea1a677e1fa2fc15d7b067c1dd2a252ef05401a0e7cf6c74e7"
     sc = aqua.Scene.load(str(a / "scene.json"))
     i = sc.info()
     assert (i.n_triangles, i.n_emissive_triangles, i.n_shapes) == (36, 2, 8)
