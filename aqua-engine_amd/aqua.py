@@ -162,6 +162,7 @@ _SIGS = {
     "aqua_film_error": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "aqua_primary_hits": (C.c_int, [_P, C.POINTER(Camera), C.c_uint32, C.c_uint32, _P, _P]),
     "aqua_trace_closest": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
+    "aqua_trace_closest_noalloc": (C.c_int, [_P, C.c_uint32, _P, _P, _P]),
     "aqua_trace_occluded": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "aqua_renderer_destroy": (None, [_P]),
     "aqua_fp_probe": (C.c_int, [C.c_uint32, _P, _P, _P, _P, _P]),
@@ -469,6 +470,10 @@ class Renderer:
         t = np.zeros(len(r), np.float32)
         _check(lib().aqua_trace_closest(self._h, len(r), _ptr(r), _ptr(ids), _ptr(t)))
         return ids, t
+
+    def trace_closest_noalloc(self, n: int, d_rays: int, d_prim_ids: int, d_t_hit: int) -> None:
+        """aqua_trace_closest_noalloc on caller-owned device arrays (addresses)."""
+        _check(lib().aqua_trace_closest_noalloc(self._h, n, d_rays, d_prim_ids, d_t_hit))
 
     def set_nrc_enabled(self, enabled: bool) -> None:
         _check(lib().aqua_renderer_set_nrc_enabled(self._h, int(enabled)))

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "aqua/aqua.h"
@@ -811,7 +812,29 @@ __global__ void k_fp_probe(const float* a, const float* b, const float* cc, cons
   out[5 * i + 4] = c;
 }
 
+// ---- allocation-free closest hit (aqua_trace_closest_noalloc) --------------
+// Its scratch: the queue counters (k_reset writes them, with the ray count,
+// at every call), one set of work cursors (cleared likewise) and the
+// traversal spill stacks of its fixed grid of kNaGrid blocks.
+constexpr uint32_t kNaGrid = 512;
+constexpr size_t kNaCounterWord = 0;     // kCntWords counters
+constexpr size_t kNaCursorWord = 512;    // kCursorSet cursor words
+constexpr size_t kNaSpillWord = 1024;    // kSpillStack x kNaGrid x kBlock words
+constexpr size_t kNaWords = kNaSpillWord + size_t(kSpillStack) * kNaGrid * kBlock;
+static_assert(kNaCounterWord + kCntWords <= kNaCursorWord && kNaCursorWord + kCursorSet <= kNaSpillWord,
+              "scratch regions overlap");
+static_assert(kNaGrid % 8 == 0, "traversal grids are a multiple of 8 blocks (one work range per XCD)");
+// Largest ray count one call takes (32 GiB of rays); the work ranges index
+// rays in 32 bits.
+constexpr uint32_t kNaMaxRays = 1u << 30;
+
 }  // namespace
+
+// Device scratch of aqua_trace_closest_noalloc: 33.6 MB, held by every GPU
+// the library runs on until the process ends. Host code never uses this
+// name as an address (it is a host-side stand-in): the per-device address
+// comes from hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) uint32_t scratch[kNaWords];
 
 // ---------------------------------------------------------------------------
 // host side
@@ -966,6 +989,7 @@ struct aqua_renderer {
   uint32_t n_tris = 0, n_area = 0, n_lights = 0;
   float delta = 0.0f;
   int persistent_blocks = 2048;  // resident blocks of the traversal kernels
+  uint32_t* na_scratch = nullptr;  // this device's `scratch` (aqua_trace_closest_noalloc)
   std::vector<uint32_t> tri_order;
   DevBuf<float4> tris, nodes, mats, lights;
   DevBuf<float> cdf, pmf;
@@ -1130,6 +1154,25 @@ int select_device(int device, int* cu_count = nullptr) {
     return aqua::fail(AQUA_ERR_NO_DEVICE, "device %d is %s; this build targets gfx950", device,
                       prop.gcnArchName);
   HIPCHK(hipSetDevice(device));
+  return AQUA_OK;
+}
+
+// Device address of `scratch` on `device` (which must be current): looked up
+// once per device, then kept. Each GPU holds its own copy of the array.
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[64] = {};
+
+int noalloc_scratch(int device, uint32_t** out) {
+  if (device < 0 || device >= 64) return aqua::fail(AQUA_ERR_NO_DEVICE, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return aqua::fail(AQUA_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[device];
   return AQUA_OK;
 }
 
@@ -1439,6 +1482,7 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   if (int e = select_device(device, &cus)) return e;
   auto r = std::make_unique<aqua_renderer>();
   r->device = device;
+  if (int e = noalloc_scratch(device, &r->na_scratch)) return e;
   // Persistent traversal grids: as many blocks as are resident at once for
   // both traversal kernels (their register / LDS footprint), so no block
   // waits for another to exit; a multiple of 8 (one work range per XCD).
@@ -1923,6 +1967,29 @@ int aqua_trace_closest(aqua_renderer* r, uint32_t n, const float* rays, uint32_t
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(prim_ids, ids.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   if (t_hit) HIPCHK(hipMemcpyAsync(t_hit, ts.p, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return AQUA_OK;
+}
+
+int aqua_trace_closest_noalloc(aqua_renderer* r, uint32_t n, const float* d_rays, uint32_t* d_prim_ids,
+                               float* d_t_hit) {
+  if (!r || (n && (!d_rays || !d_prim_ids || !d_t_hit))) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (n > kNaMaxRays) return aqua::fail(AQUA_ERR_INVALID_ARG, "%u rays exceed the limit of %u per call", n, kNaMaxRays);
+  if (reinterpret_cast<uintptr_t>(d_rays) % 16 || reinterpret_cast<uintptr_t>(d_prim_ids) % 4 ||
+      reinterpret_cast<uintptr_t>(d_t_hit) % 4)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "misaligned device array (rays need 16 bytes, outputs 4)");
+  if (n == 0) return AQUA_OK;
+  if (!r->na_scratch) return aqua::fail(AQUA_ERR_HIP, "no scratch address for device %d", r->device);
+  HIPCHK(hipSetDevice(r->device));
+  hipStream_t s = r->stream;
+  uint32_t* counters = r->na_scratch + kNaCounterWord;
+  uint32_t* cursors = r->na_scratch + kNaCursorWord;
+  k_reset<<<8, kBlock, 0, s>>>(counters, n, cursors, kCursorSet);
+  DevBvh bvh = r->dev_scene().bvh;
+  bvh.spill = r->na_scratch + kNaSpillWord;
+  k_intersect<false><<<kNaGrid, kBlock, 0, s>>>(bvh, RayIO{reinterpret_cast<const float4*>(d_rays), d_prim_ids, d_t_hit},
+                                                counters + cnt_rays(0), cursors, nullptr);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
   return AQUA_OK;
 }

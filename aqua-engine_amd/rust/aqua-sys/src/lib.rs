@@ -232,6 +232,7 @@ pub mod ffi {
         pub fn aqua_film_error(r: *mut aqua_renderer, relmse: *mut f64, relrmse: *mut f64) -> c_int;
         pub fn aqua_primary_hits(r: *mut aqua_renderer, camera: *const aqua_camera, frame: u32, sample: u32, prim_ids: *mut u32, t_hit: *mut f32) -> c_int;
         pub fn aqua_trace_closest(r: *mut aqua_renderer, n: u32, rays: *const f32, prim_ids: *mut u32, t_hit: *mut f32) -> c_int;
+        pub fn aqua_trace_closest_noalloc(r: *mut aqua_renderer, n: u32, d_rays: *const f32, d_prim_ids: *mut u32, d_t_hit: *mut f32) -> c_int;
         pub fn aqua_trace_occluded(r: *mut aqua_renderer, n: u32, rays: *const f32, occluded: *mut u8) -> c_int;
         pub fn aqua_renderer_destroy(r: *mut aqua_renderer);
         // ---- neural radiance cache

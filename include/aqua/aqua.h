@@ -286,6 +286,20 @@ int aqua_primary_hits(aqua_renderer* r, const aqua_camera* camera, uint32_t fram
  * ox oy oz tmax dx dy dz 0 (directions need not be unit length). */
 int aqua_trace_closest(aqua_renderer* r, uint32_t n, const float* rays, uint32_t* prim_ids,
                        float* t_hit);
+/* aqua_trace_closest without any allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * caller passes device arrays on the renderer's device: d_rays (8 floats per
+ * ray as above, 16-byte aligned; finite, non-zero directions -- not checked
+ * here), d_prim_ids and d_t_hit (n each); results are those of
+ * aqua_trace_closest. At most 2^30 rays per call (larger n:
+ * AQUA_ERR_INVALID_ARG). The kernels' counters, work cursors and traversal
+ * spill stacks live in a fixed 33.6 MB device array that every GPU the
+ * library runs on holds until the process ends; all calls on one GPU share
+ * it, so calls that overlap on one GPU (other threads, other renderers) race.
+ * Queued on the renderer's stream; returns once the results are written. */
+int aqua_trace_closest_noalloc(aqua_renderer* r, uint32_t n, const float* d_rays, uint32_t* d_prim_ids,
+                               float* d_t_hit);
 /* Any hit in (0, tmax): occluded[i] = 1 / 0. */
 int aqua_trace_occluded(aqua_renderer* r, uint32_t n, const float* rays, uint8_t* occluded);
 
