@@ -46,6 +46,12 @@ constexpr int kLevels = 16;
 // entries across a lane group first, so their atomics are few.
 constexpr int kFineLevel0 = 8;
 static_assert(kFineLevel0 <= 8, "levels 8..15 (the unmerged scatter) must accumulate in the BF16 pairs");
+// Gather / scatter layout: corner-parallel (the four lanes of a row each take
+// one corner pair of every level) or per level (each lane all eight corners
+// of its four levels). Per level measures faster for queries (2.92 against
+// 3.24 ms for the 6.7 M queries of a 4-spp 1080p box-room frame).
+constexpr bool kCpGather = false;
+constexpr bool kCpScatter = false;
 constexpr int kHidden = 4;
 constexpr int kMlp = 4 * 64 * 64 + 64 * 16;    // 17408 floats
 static_assert(kMlp % 64 == 0, "partial sums run 64 weights per block");
@@ -378,6 +384,41 @@ __device__ __forceinline__ void split_b(const float h[4][4], float v0[8], float 
   }
 }
 
+// Adds (v0, v1) to entry idx of level lv's table gradient. MERGE (levels
+// 0..7, whose entries the 16 spatially sorted rows of a lane group often
+// share): runs of equal entries across the group's 16 lanes are summed first
+// (segmented suffix sum) and only the first lane of a run adds, FP32 pairs
+// below kFineLevel0; every lane of the wave calls it (the merge shuffles).
+// Otherwise one packed BF16 pair per call.
+template <bool MERGE>
+__device__ __forceinline__ void scatter_add(const FwdArgs& a, uint32_t T_, int lv, uint32_t idx, float v0, float v1,
+                                            int r16, int g) {
+  if constexpr (MERGE) {
+    const uint32_t prev = __shfl_up(idx, 1, 16);
+    const bool head = r16 == 0 || prev != idx;
+    const uint32_t hm = uint32_t(__ballot(head) >> (16 * g)) & 0xFFFFu;
+    const uint32_t after = hm >> (r16 + 1);  // bit j: lane r16 + 1 + j starts a run
+#pragma unroll
+    for (int st = 1; st < 16; st <<= 1) {
+      float o0 = __shfl_down(v0, st, 16), o1 = __shfl_down(v1, st, 16);
+      if (r16 + st < 16 && (after & ((1u << st) - 1u)) == 0u) {
+        v0 += o0;
+        v1 += o1;
+      }
+    }
+    if (!head || (v0 == 0.0f && v1 == 0.0f)) return;
+  }
+  if (lv >= kFineLevel0) {
+    g2v hv = {(__bf16)v0, (__bf16)v1};
+    __builtin_amdgcn_global_atomic_fadd_v2bf16(
+        reinterpret_cast<g2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_ + idx, hv);
+  } else {
+    float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
+    atomicAdd(gp, v0);
+    atomicAdd(gp + 1, v1);
+  }
+}
+
 template <int P, bool TRAIN>
 __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
   typedef Op<P> O;
@@ -442,7 +483,7 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
     float cur[4][4];  // current layer input, feature 16 mt + 4 g + j
     {
       float f[16];
-      encode16<!TRAIN>(a, T_, g, valid, px, py, pz, r0, r1, r2, r3, f);
+      encode16<kCpGather && !TRAIN>(a, T_, g, valid, px, py, pz, r0, r1, r2, r3, f);
 #pragma unroll
       for (int i = 0; i < 16; ++i) cur[2 * (i >> 3) + ((i & 7) >> 2)][i & 3] = f[i];
     }
@@ -550,57 +591,56 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
       }
       // ---- hash-grid gradient scatter: dz now holds d loss / d input
       // feature; level q's two features are dz[q < 2 ? 0 : 1][2 (q % 2) + {0,1}]
+      if constexpr (kCpScatter) {
+        // Corner-parallel, as the query gather: the four lanes of a column
+        // share a row; for every level each adds one corner pair (the
+        // gradient of that level comes from its owner lane), so an atomic
+        // instruction covers one level of 16 rows.
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float m0 = q < 2 ? dz[0][2 * (q & 1)] : dz[1][2 * (q & 1)];
+          const float m1 = q < 2 ? dz[0][2 * (q & 1) + 1] : dz[1][2 * (q & 1) + 1];
 #pragma unroll 1
-      for (int q = 0; q < 4; ++q) {
-        int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
-        float d0 = q < 2 ? dz[0][2 * (q & 1)] : dz[1][2 * (q & 1)];
-        float d1 = q < 2 ? dz[0][2 * (q & 1) + 1] : dz[1][2 * (q & 1) + 1];
-        if (!valid) d0 = d1 = 0.0f;
-        Cell c = cell_of(a.G, lv, px, py, pz);
-        if (q < 2) {
-          // Levels 0..7: the 16 rows of a tile (spatially sorted by the
-          // caller) often share an entry. Runs of equal entries across the
-          // lane group's 16 lanes are summed first (segmented suffix sum),
-          // and only the first lane of a run issues the atomic(s).
+          for (int o = 0; o < 4; ++o) {
+            const int lv = q < 2 ? 2 * o + q : 8 + 2 * o + (q - 2);
+            float d0 = __shfl(m0, o * 16 + r16), d1 = __shfl(m1, o * 16 + r16);
+            if (!valid) d0 = d1 = 0.0f;
+            Cell c = cell_of(a.G, lv, px, py, pz);
+            const int cy = c.c[1] + (g & 1), cz = c.c[2] + (g >> 1);
 #pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
-            float w = corner_w(c, k);
-            float v0 = w * d0, v1 = w * d1;
-            uint32_t prev = __shfl_up(idx, 1, 16);
-            bool head = r16 == 0 || prev != idx;
-            uint32_t hm = uint32_t(__ballot(head) >> (16 * g)) & 0xFFFFu;
-            uint32_t after = hm >> (r16 + 1);  // bit j: lane r16 + 1 + j starts a run
-#pragma unroll
-            for (int st = 1; st < 16; st <<= 1) {
-              float o0 = __shfl_down(v0, st, 16), o1 = __shfl_down(v1, st, 16);
-              if (r16 + st < 16 && (after & ((1u << st) - 1u)) == 0u) {
-                v0 += o0;
-                v1 += o1;
-              }
-            }
-            if (head && (v0 != 0.0f || v1 != 0.0f)) {
-              if (lv >= kFineLevel0) {
-                g2v hv = {(__bf16)v0, (__bf16)v1};
-                __builtin_amdgcn_global_atomic_fadd_v2bf16(
-                    reinterpret_cast<g2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_ + idx, hv);
-              } else {
-                float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
-                atomicAdd(gp, v0);
-                atomicAdd(gp + 1, v1);
-              }
+            for (int h = 0; h < 2; ++h) {
+              const uint32_t idx = grid_index(a.G, lv, c.c[0] + h, cy, cz);
+              const float w = corner_w(c, 2 * g + h);
+              if (q < 2)
+                scatter_add<true>(a, T_, lv, idx, w * d0, w * d1, r16, g);
+              else if (d0 != 0.0f || d1 != 0.0f)
+                scatter_add<false>(a, T_, lv, idx, w * d0, w * d1, r16, g);
             }
           }
-        } else {
-          if (d0 == 0.0f && d1 == 0.0f) continue;
-          // levels 8..15: one packed BF16 atomic per corner, unmerged
-          g2v* gl = reinterpret_cast<g2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_;
+        }
+      } else {
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+          int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
+          float d0 = q < 2 ? dz[0][2 * (q & 1)] : dz[1][2 * (q & 1)];
+          float d1 = q < 2 ? dz[0][2 * (q & 1) + 1] : dz[1][2 * (q & 1) + 1];
+          if (!valid) d0 = d1 = 0.0f;
+          Cell c = cell_of(a.G, lv, px, py, pz);
+          if (q < 2) {
 #pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
-            float w = corner_w(c, k);
-            g2v v = {(__bf16)(w * d0), (__bf16)(w * d1)};
-            __builtin_amdgcn_global_atomic_fadd_v2bf16(gl + idx, v);
+            for (int k = 0; k < 8; ++k) {
+              uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
+              float w = corner_w(c, k);
+              scatter_add<true>(a, T_, lv, idx, w * d0, w * d1, r16, g);
+            }
+          } else {
+            if (d0 == 0.0f && d1 == 0.0f) continue;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+              uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
+              float w = corner_w(c, k);
+              scatter_add<false>(a, T_, lv, idx, w * d0, w * d1, r16, g);
+            }
           }
         }
       }

@@ -37,9 +37,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 
 METRIC = "ms/frame to relMSE 1e-2 vs FP64 oracle (NRC vs plain PT); Mrays/s"
 HBM_PEAK_GBS = 8000.0
-# VALU issue peak of MI355X: 256 CUs x 4 SIMDs, one wave64 instruction per
-# SIMD every 4 clocks (16 lanes), at 2.4 GHz.
-VALU_PEAK_GINSTR = 256 * 4 * 2.4 / 4.0
+# VALU issue peak of MI355X in wave64 instructions: the published FP32 vector
+# rate, 157.3 TFLOP/s, is 128 FMA lanes per CU per clock (256 CUs, 2.4 GHz),
+# i.e. one wave64 instruction per SIMD every 2 clocks. (One every 4 clocks
+# cannot be the limit: the traversal kernel's counters show 0.35 instructions
+# per SIMD-clock.)
+VALU_PEAK_GINSTR = round(157.3e3 / (2 * 64), 1)
 # Algorithmic bytes of one closest-hit ray: the ray (origin+tmax, direction:
 # 32 B) read once, the hit record (slot, t: 8 B) written once, one 64-byte
 # quantised 4-wide node per node visit and one 48-byte triangle per triangle
@@ -51,6 +54,7 @@ TRAVERSAL_SOURCES = ("aqua-engine_amd/csrc/device/trace.h", "aqua-engine_amd/csr
                      "aqua-engine_amd/csrc/renderer.hip")
 VALU_PROFILE = "profiles/k_intersect_interior1m_1080p_64spp_valu.json"
 TRAFFIC_PROFILE = "profiles/k_intersect_interior1m_1080p_64spp_traffic.json"
+SHADOW_VALU_PROFILE = "profiles/k_shadow_interior1m_1080p_64spp_valu.json"
 
 
 def _load(name, path):
@@ -265,6 +269,7 @@ def nrc_eval_leg(aqua, frames=tuple(range(15, 240, 30)), n_frames=240, oracle_sp
             for r in caches.values():
                 r.render(1, frame=f, camera=cf)
         next_frame = fe
+        print(f"bench: nrc_eval frame {fe}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         ref.render_oracle(0, oracle_spp, frame=fe, camera=cam)
         oracle = ref.oracle_film(cam)
@@ -334,6 +339,7 @@ def large_leg(aqua, meshes=400, tris=5000000, width=3840, height=2160, frames=60
     cams = [aqua.camera_flythrough(scene, f, frames, width=width, height=height) for f in range(frames)]
     imgs = {}
     for name, prec in (("fp8", aqua.NRC_FP8), ("f16", aqua.NRC_F16)):
+        print(f"bench: large, {name} cache", file=sys.stderr, flush=True)
         c = aqua.integrator_default()
         c.type = aqua.INTEGRATOR_NRC
         c.max_depth = 8
@@ -393,6 +399,8 @@ def main():
                     help="skip the extra legs (box room, cache training step, cache evaluation, large scene)")
     ap.add_argument("--only-nrc", action="store_true", help="skip the interior workload, run the extra legs")
     ap.add_argument("--no-large", action="store_true", help="skip the 5.4 M-triangle 3840x2160 leg")
+    ap.add_argument("--legs", default="box,train,eval,large",
+                    help="the extra legs to run, a comma list of box, train, eval, large")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (the reference kernel for HBM counters)")
     ap.add_argument("--profile-frame", action="store_true",
@@ -420,14 +428,21 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
+    legs = set(args.legs.split(","))
+    if args.no_large:
+        legs.discard("large")
+
     def extra_legs(out):
-        print("bench: box room leg", file=sys.stderr, flush=True)
-        out["box_room"] = box_room_leg(aqua)
-        print("bench: cache training leg", file=sys.stderr, flush=True)
-        out["nrc_train"] = nrc_train_leg(aqua)
-        print("bench: cache evaluation leg", file=sys.stderr, flush=True)
-        out["nrc_eval"] = nrc_eval_leg(aqua)
-        if not args.no_large:
+        if "box" in legs:
+            print("bench: box room leg", file=sys.stderr, flush=True)
+            out["box_room"] = box_room_leg(aqua)
+        if "train" in legs:
+            print("bench: cache training leg", file=sys.stderr, flush=True)
+            out["nrc_train"] = nrc_train_leg(aqua)
+        if "eval" in legs:
+            print("bench: cache evaluation leg", file=sys.stderr, flush=True)
+            out["nrc_eval"] = nrc_eval_leg(aqua)
+        if "large" in legs:
             print("bench: large-scene leg", file=sys.stderr, flush=True)
             out["large"] = large_leg(aqua)
 
@@ -485,6 +500,7 @@ def main():
     ms_isect = st.ms_intersect
     launches = st.n_intersect_launches
     isect_rays = st.camera_rays + st.extension_rays
+    ms_shadow, shadow_rays = st.ms_shadow, st.shadow_rays
     # The frame's error against a 256-spp FP64 render of the same view
     # (its own sample set), reduced on the GPU in FP64.
     r.render_oracle(0, 256, frame=777)
@@ -502,7 +518,8 @@ def main():
     traffic = _profile(TRAFFIC_PROFILE)
     roof = {"kernel": "k_intersect (closest hit)", "bound": "valu", "unit": "G VALU instr/s",
             "peak": VALU_PEAK_GINSTR, "achieved": None, "frac": None,
-            "peak_derivation": "256 CUs x 4 SIMDs x 2.4 GHz / 4 clocks per wave64 VALU instruction",
+            "peak_derivation": "157.3 TFLOP/s FP32 vector / (2 FLOP x 64 lanes) = 256 CUs x 4 SIMDs x 2.4 GHz / "
+                               "2 clocks per wave64 VALU instruction",
             "source_digest": traversal_digest()}
     if valu:
         # wave64 VALU instructions per closest-hit ray (PMC SQ_INSTS_VALU of
@@ -510,7 +527,10 @@ def main():
         ginstr = valu["valu_insts_per_ray"] * rays_per_launch / (avg_launch_ms * 1e-3) / 1e9
         roof.update(achieved=round(ginstr, 1), frac=round(ginstr / VALU_PEAK_GINSTR, 4),
                     valu_insts_per_ray=valu["valu_insts_per_ray"],
-                    lane_utilisation=valu["lane_utilisation"], valu_source=VALU_PROFILE)
+                    lane_utilisation=valu["lane_utilisation"], valu_source=VALU_PROFILE,
+                    # clock-independent cross-check: instructions per SIMD per
+                    # GPU-active clock of the profiled launches, over 0.5
+                    frac_per_clock=round(valu["valu_issue_per_simd_cycle"] / 0.5, 4))
     roof["traffic"] = traffic["hbm_bytes_per_launch"] if traffic else None
     roof.update({
         "traffic_source": TRAFFIC_PROFILE if traffic else None,
@@ -532,6 +552,15 @@ def main():
     })
     if traffic:
         roof["hbm_frac"] = round(roof["hbm_achieved"] / HBM_PEAK_GBS, 4)
+    shadow = _profile(SHADOW_VALU_PROFILE)
+    if shadow and ms_shadow > 0:
+        # the any-hit kernel, same bound: its own counters and launch time
+        ginstr = shadow["valu_insts_per_ray"] * shadow_rays / (ms_shadow * 1e-3) / 1e9
+        roof["k_shadow"] = {"achieved": round(ginstr, 1), "frac": round(ginstr / VALU_PEAK_GINSTR, 4),
+                            "frac_per_clock": round(shadow["valu_issue_per_simd_cycle"] / 0.5, 4),
+                            "valu_insts_per_ray": shadow["valu_insts_per_ray"],
+                            "lane_utilisation": shadow["lane_utilisation"], "ms_per_frame": round(ms_shadow, 3),
+                            "rays_per_frame": shadow_rays, "valu_source": SHADOW_VALU_PROFILE}
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
