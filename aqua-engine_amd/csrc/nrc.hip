@@ -42,16 +42,16 @@ constexpr int kLevels = 16;
 // there. An 8-bit mantissa stops adding once a sum is ~256 addends large, so
 // the coarser levels 0-7, whose entries collect many rows each (thousands
 // when a batch clusters in a small region: levels 4-7 lost 30-80 % of their
-// gradient in BF16 there), accumulate FP32 pairs; they also merge equal
-// entries across a lane group first, so their atomics are few.
+// gradient in BF16 there), accumulate FP32 pairs; k_nrc_scatter merges
+// equal entries across a wave of spatially sorted rows first, so their
+// atomics are few.
 constexpr int kFineLevel0 = 8;
-static_assert(kFineLevel0 <= 8, "levels 8..15 (the unmerged scatter) must accumulate in the BF16 pairs");
-// Gather / scatter layout: corner-parallel (the four lanes of a row each take
-// one corner pair of every level) or per level (each lane all eight corners
-// of its four levels). Per level measures faster for queries (2.92 against
+static_assert(kFineLevel0 == 8, "k_nrc_scatter merges levels 0..7 and adds levels 8..15 as BF16 pairs");
+// Gather layout: corner-parallel (the four lanes of a row each take one
+// corner pair of every level) or per level (each lane all eight corners of
+// its four levels). Per level measures faster for queries (2.92 against
 // 3.24 ms for the 6.7 M queries of a 4-spp 1080p box-room frame).
 constexpr bool kCpGather = false;
-constexpr bool kCpScatter = false;
 constexpr int kHidden = 4;
 constexpr int kMlp = 4 * 64 * 64 + 64 * 16;    // 17408 floats
 static_assert(kMlp % 64 == 0, "partial sums run 64 weights per block");
@@ -245,6 +245,7 @@ struct FwdArgs {
   // training
   float* table_grad;        // [level][T][2] FP32 (levels < kFineLevel0)
   uint32_t* table_grad16;   // [level - kFineLevel0][T] two BF16
+  float2* fgrad;            // [level][npad] d loss / d (feature pair) of each row
   uint16_t* HT;             // 5 x [npad][64] layer inputs
   uint16_t* DT;             // 5 x [npad][64] pre-activation grads (last: 16 used)
   float* loss_acc;
@@ -381,41 +382,6 @@ __device__ __forceinline__ void split_b(const float h[4][4], float v0[8], float 
     v0[4 + i] = h[1][i];
     v1[i] = h[2][i];
     v1[4 + i] = h[3][i];
-  }
-}
-
-// Adds (v0, v1) to entry idx of level lv's table gradient. MERGE (levels
-// 0..7, whose entries the 16 spatially sorted rows of a lane group often
-// share): runs of equal entries across the group's 16 lanes are summed first
-// (segmented suffix sum) and only the first lane of a run adds, FP32 pairs
-// below kFineLevel0; every lane of the wave calls it (the merge shuffles).
-// Otherwise one packed BF16 pair per call.
-template <bool MERGE>
-__device__ __forceinline__ void scatter_add(const FwdArgs& a, uint32_t T_, int lv, uint32_t idx, float v0, float v1,
-                                            int r16, int g) {
-  if constexpr (MERGE) {
-    const uint32_t prev = __shfl_up(idx, 1, 16);
-    const bool head = r16 == 0 || prev != idx;
-    const uint32_t hm = uint32_t(__ballot(head) >> (16 * g)) & 0xFFFFu;
-    const uint32_t after = hm >> (r16 + 1);  // bit j: lane r16 + 1 + j starts a run
-#pragma unroll
-    for (int st = 1; st < 16; st <<= 1) {
-      float o0 = __shfl_down(v0, st, 16), o1 = __shfl_down(v1, st, 16);
-      if (r16 + st < 16 && (after & ((1u << st) - 1u)) == 0u) {
-        v0 += o0;
-        v1 += o1;
-      }
-    }
-    if (!head || (v0 == 0.0f && v1 == 0.0f)) return;
-  }
-  if (lv >= kFineLevel0) {
-    g2v hv = {(__bf16)v0, (__bf16)v1};
-    __builtin_amdgcn_global_atomic_fadd_v2bf16(
-        reinterpret_cast<g2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_ + idx, hv);
-  } else {
-    float* gp = a.table_grad + 2 * (size_t(lv) * T_ + idx);
-    atomicAdd(gp, v0);
-    atomicAdd(gp + 1, v1);
   }
 }
 
@@ -589,60 +555,15 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
             dz[mt][j] = (LL == 0 || ((mask[LL] >> (4 * mt + j)) & 1u)) ? clamp16<P>(acc2[j]) : 0.0f;
         }
       }
-      // ---- hash-grid gradient scatter: dz now holds d loss / d input
-      // feature; level q's two features are dz[q < 2 ? 0 : 1][2 (q % 2) + {0,1}]
-      if constexpr (kCpScatter) {
-        // Corner-parallel, as the query gather: the four lanes of a column
-        // share a row; for every level each adds one corner pair (the
-        // gradient of that level comes from its owner lane), so an atomic
-        // instruction covers one level of 16 rows.
+      // ---- d loss / d (feature pair) of this lane's four levels, for
+      // k_nrc_scatter: level q's two features are dz[q < 2 ? 0 : 1][2 (q % 2) + {0,1}]
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float m0 = q < 2 ? dz[0][2 * (q & 1)] : dz[1][2 * (q & 1)];
-          const float m1 = q < 2 ? dz[0][2 * (q & 1) + 1] : dz[1][2 * (q & 1) + 1];
-#pragma unroll 1
-          for (int o = 0; o < 4; ++o) {
-            const int lv = q < 2 ? 2 * o + q : 8 + 2 * o + (q - 2);
-            float d0 = __shfl(m0, o * 16 + r16), d1 = __shfl(m1, o * 16 + r16);
-            if (!valid) d0 = d1 = 0.0f;
-            Cell c = cell_of(a.G, lv, px, py, pz);
-            const int cy = c.c[1] + (g & 1), cz = c.c[2] + (g >> 1);
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-              const uint32_t idx = grid_index(a.G, lv, c.c[0] + h, cy, cz);
-              const float w = corner_w(c, 2 * g + h);
-              if (q < 2)
-                scatter_add<true>(a, T_, lv, idx, w * d0, w * d1, r16, g);
-              else if (d0 != 0.0f || d1 != 0.0f)
-                scatter_add<false>(a, T_, lv, idx, w * d0, w * d1, r16, g);
-            }
-          }
-        }
-      } else {
-#pragma unroll 1
-        for (int q = 0; q < 4; ++q) {
-          int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
-          float d0 = q < 2 ? dz[0][2 * (q & 1)] : dz[1][2 * (q & 1)];
-          float d1 = q < 2 ? dz[0][2 * (q & 1) + 1] : dz[1][2 * (q & 1) + 1];
-          if (!valid) d0 = d1 = 0.0f;
-          Cell c = cell_of(a.G, lv, px, py, pz);
-          if (q < 2) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-              uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
-              float w = corner_w(c, k);
-              scatter_add<true>(a, T_, lv, idx, w * d0, w * d1, r16, g);
-            }
-          } else {
-            if (d0 == 0.0f && d1 == 0.0f) continue;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-              uint32_t idx = grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
-              float w = corner_w(c, k);
-              scatter_add<false>(a, T_, lv, idx, w * d0, w * d1, r16, g);
-            }
-          }
-        }
+      for (int q = 0; q < 4; ++q) {
+        const int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
+        float d0 = q < 2 ? dz[0][2 * (q & 1)] : dz[1][2 * (q & 1)];
+        float d1 = q < 2 ? dz[0][2 * (q & 1) + 1] : dz[1][2 * (q & 1) + 1];
+        if (!valid) d0 = d1 = 0.0f;
+        a.fgrad[size_t(lv) * a.npad + row] = make_float2(d0, d1);
       }
     }
   }
@@ -653,6 +574,95 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
     for (int L = 0; L < 5; ++L) {
       float m = wave_max(amax_lane[L]);
       if (lane == 0) atomicMax(&a.act_amax[L], __float_as_uint(m));
+    }
+  }
+}
+
+// ---- hash-grid gradient scatter -------------------------------------------------
+//
+// One wave per 64 consecutive rows of the batch (in the spatial order the
+// training step sorts it into), after k_nrc_fwd<P, true> has written every
+// row's d loss / d (feature pair) per level to fgrad.
+//
+// The atomics' cost is set by the 64-byte segments an instruction touches,
+// not by how many lanes hit a segment, so each layout keeps a row's corners
+// in one instruction:
+// * levels 0..7 (FP32 pairs): 16 lanes per row (corner k, component c), four
+//   rows per instruction; the two halves of an entry share a segment. Where
+//   the four rows (spatial neighbours) hit the same entry, their values are
+//   summed first and one lane adds.
+// * levels 8..15 (BF16 pairs, one 32-bit atomic per entry): 8 lanes per row,
+//   one corner each, eight rows per instruction (a cell's corners lie in
+//   about 3 segments with the 4 x 4 x 2 block hashing).
+__global__ void __launch_bounds__(kBlock) k_nrc_scatter(FwdArgs a) {
+  const int lane = int(__lane_id());
+  const uint32_t T_ = 1u << a.G.log2T;
+  const uint32_t waves = (a.n + 63u) / 64u;
+  const uint32_t wpb = blockDim.x >> 6;
+  const float* fg = reinterpret_cast<const float*>(a.fgrad);
+  for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < waves; w += gridDim.x * wpb) {
+    const uint32_t row = w * 64u + uint32_t(lane);
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    if (row < a.n) {
+      const float4 r0 = a.rec[4 * size_t(a.perm ? a.perm[row] : row)];
+      px = fminf(fmaxf(r0.x, 0.0f), 1.0f);
+      py = fminf(fmaxf(r0.y, 0.0f), 1.0f);
+      pz = fminf(fmaxf(r0.z, 0.0f), 1.0f);
+    }
+    {
+      const int comp = lane & 1, k = (lane >> 1) & 7;
+#pragma unroll 1
+      for (int r4 = 0; r4 < 16; ++r4) {
+        const int from = 4 * r4 + (lane >> 4);
+        const float qx = __shfl(px, from), qy = __shfl(py, from), qz = __shfl(pz, from);
+        const uint32_t rq = w * 64u + uint32_t(from);
+        const bool vq = rq < a.n;
+#pragma unroll 2
+        for (int lv = 0; lv < kFineLevel0; ++lv) {
+          const Cell c = cell_of(a.G, lv, qx, qy, qz);
+          const uint32_t idx =
+              grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
+          float v = vq ? corner_w(c, k) * fg[2 * (size_t(lv) * a.npad + rq) + comp] : 0.0f;
+          // lanes l, l ^ 16, l ^ 32, l ^ 48 hold the same (corner, half) of
+          // the four rows: fold equal entries pairwise into the lower lane
+          bool live = true;
+#pragma unroll
+          for (int m = 16; m <= 32; m <<= 1) {
+            const uint32_t pi = __shfl_xor(idx, m);
+            const float pv = __shfl_xor(v, m);
+            const bool plive = __shfl_xor(int(live), m) != 0;
+            if (pi == idx && live && plive) {
+              if (lane & m)
+                live = false;
+              else
+                v += pv;
+            }
+          }
+          if (live && v != 0.0f) atomicAdd(a.table_grad + 2 * (size_t(lv) * T_ + idx) + comp, v);
+        }
+      }
+    }
+    {
+      const int k = lane & 7;
+#pragma unroll 1
+      for (int r8 = 0; r8 < 8; ++r8) {
+        const int from = 8 * r8 + (lane >> 3);
+        const float qx = __shfl(px, from), qy = __shfl(py, from), qz = __shfl(pz, from);
+        const uint32_t rq = w * 64u + uint32_t(from);
+        if (rq >= a.n) continue;
+#pragma unroll 2
+        for (int lv = kFineLevel0; lv < kLevels; ++lv) {
+          const float2 d = a.fgrad[size_t(lv) * a.npad + rq];
+          if (d.x == 0.0f && d.y == 0.0f) continue;
+          const Cell c = cell_of(a.G, lv, qx, qy, qz);
+          const uint32_t idx =
+              grid_index(a.G, lv, c.c[0] + (k & 1), c.c[1] + ((k >> 1) & 1), c.c[2] + (k >> 2));
+          const float wk = corner_w(c, k);
+          const g2v hv = {(__bf16)(wk * d.x), (__bf16)(wk * d.y)};
+          __builtin_amdgcn_global_atomic_fadd_v2bf16(
+              reinterpret_cast<g2v*>(a.table_grad16) + size_t(lv - kFineLevel0) * T_ + idx, hv);
+        }
+      }
     }
   }
 }
@@ -958,6 +968,7 @@ struct aqua_nrc {
   Buf<float> wscale, ascale, loss;
   Buf<uint32_t> act_amax;
   Buf<uint16_t> HT, DT;
+  Buf<float2> fgrad;         // [level][npad] feature-pair gradients, forward/backward -> scatter
   Buf<uint32_t> skey, sidx;  // training batch sort: Morton keys, row order (double buffers)
   Buf<uint8_t> stemp;
   uint32_t cap_rows = 0;
@@ -991,6 +1002,7 @@ struct aqua_nrc {
     if (npad & (npad - 1u)) npad = 1u << (32 - __builtin_clz(npad));
     if (int e = HT.alloc(size_t(5) * 64 * npad)) return e;
     if (int e = DT.alloc(size_t(5) * 64 * npad)) return e;
+    if (int e = fgrad.alloc(size_t(kLevels) * npad)) return e;
     uint32_t nchunks = max_chunks(npad);
     if (int e = partial.alloc(size_t(nchunks) * kMlp)) return e;
     if (int e = skey.alloc(size_t(2) * npad)) return e;
@@ -1034,9 +1046,22 @@ struct aqua_nrc {
     a.table_grad16 = tgrad16.p;
     a.HT = HT.p;
     a.DT = DT.p;
+    a.fgrad = fgrad.p;
     a.loss_acc = loss.p;
     a.act_amax = act_amax.p;
     return a;
+  }
+
+  // Training forward + backward of a's rows (activations, feature
+  // gradients), then the scatter of the feature gradients into the table
+  // gradient.
+  void launch_fwd_bwd(const FwdArgs& a, hipStream_t s) const {
+    if (cfg.train_precision == AQUA_NRC_BF16)
+      k_nrc_fwd<AQUA_NRC_BF16, true><<<fwd_grid(a.npad), kBlock, 0, s>>>(a);
+    else
+      k_nrc_fwd<AQUA_NRC_F16, true><<<fwd_grid(a.npad), kBlock, 0, s>>>(a);
+    const int blocks = int(((a.n + 63u) / 64u + 3u) / 4u);
+    k_nrc_scatter<<<std::max(1, std::min(blocks, cus * 8)), kBlock, 0, s>>>(a);
   }
 
   // Blocks of 4 waves, one wave per 16-row tile up to 32 waves per CU; a
@@ -1122,10 +1147,7 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
     k_fill_f32<<<1, 64, 0, s>>>(nrc->loss.p, 1, 0.0f);
     k_fill_u32<<<1, 64, 0, s>>>(nrc->act_amax.p, 5, 0u);
     HIPCHK(hipEventRecord(nrc->ev[1], s));
-    if (c.train_precision == AQUA_NRC_BF16)
-      k_nrc_fwd<AQUA_NRC_BF16, true><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
-    else
-      k_nrc_fwd<AQUA_NRC_F16, true><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+    nrc->launch_fwd_bwd(a, s);
     HIPCHK(hipEventRecord(nrc->ev[2], s));
     dim3 gw(nchunks, 5);
     if (c.train_precision == AQUA_NRC_BF16)
@@ -1462,10 +1484,7 @@ int aqua_nrc_gradients(aqua_nrc* nrc, uint32_t n, const float* records, const fl
   float gscale = float(1.0 / (3.0 * double(n)));
   k_fill_f32<<<1, 64, 0, s>>>(nrc->loss.p, 1, 0.0f);
   k_fill_u32<<<1, 64, 0, s>>>(nrc->act_amax.p, 5, 0u);
-  if (nrc->cfg.train_precision == AQUA_NRC_BF16)
-    k_nrc_fwd<AQUA_NRC_BF16, true><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
-  else
-    k_nrc_fwd<AQUA_NRC_F16, true><<<nrc->fwd_grid(npad), kBlock, 0, s>>>(a);
+  nrc->launch_fwd_bwd(a, s);
   dim3 gw(nchunks, 5);
   if (nrc->cfg.train_precision == AQUA_NRC_BF16)
     k_nrc_wgrad<AQUA_NRC_BF16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),
