@@ -52,6 +52,12 @@ static_assert(kFineLevel0 == 8, "k_nrc_scatter merges levels 0..7 and adds level
 // its four levels). Per level measures faster for queries (2.92 against
 // 3.24 ms for the 6.7 M queries of a 4-spp 1080p box-room frame).
 constexpr bool kCpGather = false;
+// Training stages the forward and backward weight operands in LDS per block
+// (72 KB in 16-bit) instead of reading them through L1 per layer and tile.
+constexpr bool kLdsTrainWeights = true;
+// Training blocks of 4 waves: two fit a CU beside their staged operands
+// (8 and 12 waves per block, one block per CU, measured slower).
+constexpr int kTrainBlock = 256;
 constexpr int kHidden = 4;
 constexpr int kMlp = 4 * 64 * 64 + 64 * 16;    // 17408 floats
 static_assert(kMlp % 64 == 0, "partial sums run 64 weights per block");
@@ -386,7 +392,7 @@ __device__ __forceinline__ void split_b(const float h[4][4], float v0[8], float 
 }
 
 template <int P, bool TRAIN>
-__global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
+__global__ void __launch_bounds__(TRAIN ? kTrainBlock : kBlock) k_nrc_fwd(FwdArgs a) {
   typedef Op<P> O;
   typedef typename O::T T;
   typedef typename O::Packed Pk;
@@ -399,16 +405,21 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
   extern __shared__ __align__(16) unsigned char nrc_lds[];
   const Pk* wl = reinterpret_cast<const Pk*>(nrc_lds);
   int woff = 0;
-  if constexpr (!TRAIN) {
+  constexpr bool kLdsW = !TRAIN || kLdsTrainWeights;
+  if constexpr (kLdsW) {
     Pk* dst = reinterpret_cast<Pk*>(nrc_lds);
     for (int i = int(threadIdx.x); i < kFwdPack; i += int(blockDim.x)) dst[i] = pf[i];
+    if constexpr (TRAIN) {
+      const Pk* pbg = reinterpret_cast<const Pk*>(a.pack_bwd);
+      for (int i = int(threadIdx.x); i < kBwdPack; i += int(blockDim.x)) dst[kFwdPack + i] = pbg[i];
+    }
     __syncthreads();
   }
   auto wfwd = [&](int idx) -> Pk {
-    if constexpr (TRAIN)
-      return pf[idx];
-    else
+    if constexpr (kLdsW)
       return wl[idx + woff];
+    else
+      return pf[idx];
   };
   const int lane = int(__lane_id());
   const int g = lane >> 4, r16 = lane & 15;
@@ -430,10 +441,10 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
     // The weight operands are loop-invariant; without this barrier the
     // compiler hoists all 40 of them out of the tile loop into ~160 VGPRs.
     // Re-reading them per tile hits L1 and keeps occupancy up.
-    if constexpr (TRAIN)
-      asm volatile("" : "+s"(pf));
-    else
+    if constexpr (kLdsW)
       asm volatile("" : "+s"(woff));
+    else
+      asm volatile("" : "+s"(pf));
     const uint32_t row = tile * 16u + uint32_t(r16);
     const bool valid = row < n;
     const uint32_t src = (a.perm && valid) ? a.perm[row] : row;
@@ -459,10 +470,10 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
 #pragma unroll
     for (int L = 0; L <= kHidden; ++L) {
       // one layer's weights live at a time
-      if constexpr (TRAIN)
-        asm volatile("" : "+s"(pf));
-      else
+      if constexpr (kLdsW)
         asm volatile("" : "+s"(woff));
+      else
+        asm volatile("" : "+s"(pf));
       float v0[8], v1[8];
       split_b(cur, v0, v1);
       float sa = 1.0f, deq = 1.0f;
@@ -525,7 +536,17 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
       store4<O>(a.DT + (size_t(kHidden) * a.npad + row) * 64 + 4 * g, dy);
       // ---- backward: dz in the accumulator layout, masks from the forward
       const Pk* pb = reinterpret_cast<const Pk*>(a.pack_bwd);
-      asm volatile("" : "+s"(pb));  // as for pf: keep the loads in the loop
+      auto wbwd = [&](int idx) -> Pk {
+        if constexpr (kLdsW)
+          return wl[kFwdPack + idx + woff];
+        else
+          return pb[idx];
+      };
+      // as for the forward operands: keep the loads in the loop
+      if constexpr (kLdsW)
+        asm volatile("" : "+s"(woff));
+      else
+        asm volatile("" : "+s"(pb));
       float dz[4][4];
       {
         float v[8] = {dy[0], dy[1], dy[2], dy[3], 0.0f, 0.0f, 0.0f, 0.0f};
@@ -533,14 +554,17 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
           f4v acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
-          acc2 = O::mma(O::unpack(pb[kHidden * 8 * 64 + mt * 64 + lane]), bd, acc2);
+          acc2 = O::mma(O::unpack(wbwd(kHidden * 8 * 64 + mt * 64 + lane)), bd, acc2);
 #pragma unroll
           for (int j = 0; j < 4; ++j) dz[mt][j] = ((mask[kHidden] >> (4 * mt + j)) & 1u) ? clamp16<P>(acc2[j]) : 0.0f;
         }
       }
 #pragma unroll
       for (int LL = kHidden - 1; LL >= 0; --LL) {
-        asm volatile("" : "+s"(pb));
+        if constexpr (kLdsW)
+          asm volatile("" : "+s"(woff));
+        else
+          asm volatile("" : "+s"(pb));
         store_row4x4<O>(a.DT + (size_t(LL) * a.npad + row) * 64 + 4 * g, dz);
         float v0b[8], v1b[8];
         split_b(dz, v0b, v1b);
@@ -548,8 +572,8 @@ __global__ void __launch_bounds__(kBlock) k_nrc_fwd(FwdArgs a) {
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
           f4v acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
-          acc2 = O::mma(O::unpack(pb[((LL * 4 + mt) * 2 + 0) * 64 + lane]), bd0, acc2);
-          acc2 = O::mma(O::unpack(pb[((LL * 4 + mt) * 2 + 1) * 64 + lane]), bd1, acc2);
+          acc2 = O::mma(O::unpack(wbwd(((LL * 4 + mt) * 2 + 0) * 64 + lane)), bd0, acc2);
+          acc2 = O::mma(O::unpack(wbwd(((LL * 4 + mt) * 2 + 1) * 64 + lane)), bd1, acc2);
 #pragma unroll
           for (int j = 0; j < 4; ++j)
             dz[mt][j] = (LL == 0 || ((mask[LL] >> (4 * mt + j)) & 1u)) ? clamp16<P>(acc2[j]) : 0.0f;
@@ -941,6 +965,10 @@ struct Buf {
 
 int blocks_for(size_t n, int per = kBlock) { return int((n + per - 1) / per); }
 
+// Events of a cache: 0-7 time the training step, kEvFork / kEvJoin order its
+// two streams.
+constexpr int kEvFork = 8, kEvJoin = 9, kEvCount = 10;
+
 }  // namespace
 
 namespace aqua {
@@ -952,12 +980,14 @@ hipError_t query_sort(void* temp, size_t temp_bytes, const float4* rec, const ui
 struct aqua_nrc {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // the training step's second stream
   aqua_nrc_cfg cfg{};
   Grid grid{};
   size_t entries = 0;  // levels * T
   size_t fine_entries() const { return entries / kLevels * (kLevels - kFineLevel0); }
   uint32_t step = 0;
   int cus = 256;
+  int train_bpc = 1;  // resident training blocks per CU
   Buf<float> tmaster, tgrad, tm, tv;
   Buf<uint32_t> tgrad16;  // gradient of levels >= kFineLevel0, BF16 pairs
   Buf<uint32_t> t16;
@@ -974,7 +1004,7 @@ struct aqua_nrc {
   uint32_t cap_rows = 0;
   Buf<float4> rec_buf, tgt_buf;
   Buf<float> out_buf;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[kEvCount] = {};
   // checkpoint (aqua_nrc_checkpoint): parameters, Adam moments, FP16 table,
   // activation amax and the step count
   Buf<float> ck_tmaster, ck_tm, ck_tv, ck_mmaster, ck_mm, ck_mv;
@@ -983,13 +1013,13 @@ struct aqua_nrc {
   bool ck_valid = false;
 
   ~aqua_nrc() {
-    if (stream) {
-      (void)hipSetDevice(device);
-      (void)hipStreamSynchronize(stream);
-    }
+    if (stream || side) (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (side) (void)hipStreamSynchronize(side);
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+    if (side) (void)hipStreamDestroy(side);
   }
 
   // Grows to the next power of two of rows (at least 2^16), so a batch that
@@ -1052,14 +1082,17 @@ struct aqua_nrc {
     return a;
   }
 
-  // Training forward + backward of a's rows (activations, feature
-  // gradients), then the scatter of the feature gradients into the table
-  // gradient.
-  void launch_fwd_bwd(const FwdArgs& a, hipStream_t s) const {
+  // Training forward + backward of a's rows: activations and feature
+  // gradients.
+  void launch_fwd(const FwdArgs& a, hipStream_t s) const {
+    const size_t lds = kLdsTrainWeights ? size_t(kFwdPack + kBwdPack) * 16 : 0;
     if (cfg.train_precision == AQUA_NRC_BF16)
-      k_nrc_fwd<AQUA_NRC_BF16, true><<<fwd_grid(a.npad), kBlock, 0, s>>>(a);
+      k_nrc_fwd<AQUA_NRC_BF16, true><<<train_grid(a.npad), kTrainBlock, lds, s>>>(a);
     else
-      k_nrc_fwd<AQUA_NRC_F16, true><<<fwd_grid(a.npad), kBlock, 0, s>>>(a);
+      k_nrc_fwd<AQUA_NRC_F16, true><<<train_grid(a.npad), kTrainBlock, lds, s>>>(a);
+  }
+  // The feature gradients into the table gradient.
+  void launch_scatter(const FwdArgs& a, hipStream_t s) const {
     const int blocks = int(((a.n + 63u) / 64u + 3u) / 4u);
     k_nrc_scatter<<<std::max(1, std::min(blocks, cus * 8)), kBlock, 0, s>>>(a);
   }
@@ -1070,6 +1103,13 @@ struct aqua_nrc {
     int tiles = int(npad / 16);
     int waves = std::min(tiles, cus * 32);
     int blocks = (waves + 3) / 4;
+    return std::max(8, (blocks + 7) & ~7);
+  }
+  // Training: resident blocks only (each stages the weight operands once),
+  // a multiple of 8.
+  int train_grid(uint32_t npad) const {
+    const int wpb = kTrainBlock / 64;
+    int blocks = std::min((int(npad / 16) + wpb - 1) / wpb, cus * train_bpc);
     return std::max(8, (blocks + 7) & ~7);
   }
 };
@@ -1147,22 +1187,32 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
     k_fill_f32<<<1, 64, 0, s>>>(nrc->loss.p, 1, 0.0f);
     k_fill_u32<<<1, 64, 0, s>>>(nrc->act_amax.p, 5, 0u);
     HIPCHK(hipEventRecord(nrc->ev[1], s));
-    nrc->launch_fwd_bwd(a, s);
-    HIPCHK(hipEventRecord(nrc->ev[2], s));
+    nrc->launch_fwd(a, s);
+    // the weight gradients and the MLP's Adam step (side stream) run beside
+    // the table-gradient scatter and the table's Adam step (this stream)
+    HIPCHK(hipEventRecord(nrc->ev[kEvFork], s));
+    hipStream_t s2 = nrc->side;
+    HIPCHK(hipStreamWaitEvent(s2, nrc->ev[kEvFork], 0));
+    HIPCHK(hipEventRecord(nrc->ev[6], s2));
     dim3 gw(nchunks, 5);
     if (c.train_precision == AQUA_NRC_BF16)
-      k_nrc_wgrad<AQUA_NRC_BF16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),
-                                                        nrc->partial.p);
+      k_nrc_wgrad<AQUA_NRC_BF16><<<gw, kBlock, 0, s2>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),
+                                                         nrc->partial.p);
     else
-      k_nrc_wgrad<AQUA_NRC_F16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),
+      k_nrc_wgrad<AQUA_NRC_F16><<<gw, kBlock, 0, s2>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),
                                                         nrc->partial.p);
-    HIPCHK(hipEventRecord(nrc->ev[3], s));
+    HIPCHK(hipEventRecord(nrc->ev[7], s2));
     // the weights may still be read on another stream until before_update
+    if (before_update && it == 0) HIPCHK(hipStreamWaitEvent(s2, before_update, 0));
+    k_nrc_adam_mlp<<<kMlp / 64, 256, 0, s2>>>(nrc->partial.p, nchunks, nrc->mmaster.p, nrc->mm.p, nrc->mv.p, A);
+    HIPCHK(hipEventRecord(nrc->ev[kEvJoin], s2));
+    nrc->launch_scatter(a, s);
+    HIPCHK(hipEventRecord(nrc->ev[2], s));
     if (before_update && it == 0) HIPCHK(hipStreamWaitEvent(s, before_update, 0));
-    k_nrc_adam_mlp<<<kMlp / 64, 256, 0, s>>>(nrc->partial.p, nchunks, nrc->mmaster.p, nrc->mm.p, nrc->mv.p, A);
     k_nrc_adam_table<<<blocks_for(nrc->entries), kBlock, 0, s>>>(
         nrc->tmaster.p, nrc->tgrad.p, nrc->tgrad16.p, nrc->entries - nrc->fine_entries(), nrc->tm.p, nrc->tv.p,
         nrc->t16.p, nrc->entries, A);
+    HIPCHK(hipStreamWaitEvent(s, nrc->ev[kEvJoin], 0));
     if (int e = nrc->repack(s)) return e;
     HIPCHK(hipEventRecord(nrc->ev[4], s));
     HIPCHK(hipGetLastError());
@@ -1171,9 +1221,9 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
       float ms;
       HIPCHK(hipEventElapsedTime(&ms, nrc->ev[1], nrc->ev[2]));
       t_fb += ms;
-      HIPCHK(hipEventElapsedTime(&ms, nrc->ev[2], nrc->ev[3]));
+      HIPCHK(hipEventElapsedTime(&ms, nrc->ev[6], nrc->ev[7]));
       t_wg += ms;
-      HIPCHK(hipEventElapsedTime(&ms, nrc->ev[3], nrc->ev[4]));
+      HIPCHK(hipEventElapsedTime(&ms, nrc->ev[2], nrc->ev[4]));
       t_ad += ms;
     }
   }
@@ -1268,7 +1318,14 @@ int aqua_nrc_create(int device, const aqua_nrc_cfg* cfg, aqua_nrc** out) {
   r->device = device;
   r->cfg = *cfg;
   r->cus = prop.multiProcessorCount;
+  {
+    int bpc = 0;
+    const size_t lds = kLdsTrainWeights ? size_t(kFwdPack + kBwdPack) * 16 : 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k_nrc_fwd<AQUA_NRC_F16, true>, kTrainBlock, lds));
+    r->train_bpc = std::max(1, bpc);
+  }
   HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
   for (hipEvent_t& e : r->ev) HIPCHK(hipEventCreate(&e));
   const uint32_t T = 1u << cfg->log2_table;
   r->grid.log2T = cfg->log2_table;
@@ -1484,7 +1541,8 @@ int aqua_nrc_gradients(aqua_nrc* nrc, uint32_t n, const float* records, const fl
   float gscale = float(1.0 / (3.0 * double(n)));
   k_fill_f32<<<1, 64, 0, s>>>(nrc->loss.p, 1, 0.0f);
   k_fill_u32<<<1, 64, 0, s>>>(nrc->act_amax.p, 5, 0u);
-  nrc->launch_fwd_bwd(a, s);
+  nrc->launch_fwd(a, s);
+  nrc->launch_scatter(a, s);
   dim3 gw(nchunks, 5);
   if (nrc->cfg.train_precision == AQUA_NRC_BF16)
     k_nrc_wgrad<AQUA_NRC_BF16><<<gw, kBlock, 0, s>>>(nrc->HT.p, nrc->DT.p, npad, chunk_rows(npad),

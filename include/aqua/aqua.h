@@ -336,9 +336,13 @@ typedef struct aqua_nrc_cfg {
 /* 64 bytes. HIP-event timings of the last training call. */
 typedef struct aqua_nrc_stats {
   double ms_total;          /* all steps                                    */
-  double ms_fwd_bwd;        /* fused encode + forward + backward + scatter  */
-  double ms_wgrad;          /* weight-gradient reduction (MFMA)             */
-  double ms_adam;           /* Adam over the table and the MLP              */
+  double ms_fwd_bwd;        /* encode + forward + backward, then the
+                               hash-table gradient scatter                  */
+  double ms_wgrad;          /* weight gradients (MFMA), on a second stream
+                               beside the scatter                           */
+  double ms_adam;           /* after the scatter: the table's Adam step and
+                               the operand repack (the MLP's Adam step runs
+                               beside, after the weight gradients)          */
   float loss;               /* relative L2 loss of the last step            */
   uint32_t steps;
   uint32_t rows;
