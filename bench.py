@@ -187,10 +187,15 @@ def nrc_train_leg(aqua, rows=1 << 18, steps=20, precision=0):
     ms = st.ms_total / steps
     table_params = 16 * (1 << 19) * 2
     mlp_params = 4 * 64 * 64 + 64 * 16
-    # bytes: records 64 B + target 16 B in, 16 levels x 8 corners x 4 B gathered
-    # and x 8 B scattered per sample; Adam reads param/grad/m/v (16 B) and
-    # writes param/grad/m/v (16 B) + the FP16 copy (2 B) per table parameter.
-    bytes_step = n_use * (80 + 16 * 8 * 4 + 16 * 8 * 8) + table_params * 34 + mlp_params * 34
+    # bytes per sample: record 64 B + target 16 B in; 16 levels x 8 corners x
+    # 4 B gathered; 5 layer inputs and 5 pre-activation gradients (128 B each)
+    # written, then read by the weight gradients; 16 feature-pair gradients
+    # (128 B) written, then read with the position (16 B) by the scatter, which
+    # adds 8 levels x 8 corners x 8 B (FP32 pairs) and 8 x 8 x 4 B (BF16
+    # pairs). Adam reads param/grad/m/v (16 B) and writes param/grad/m/v
+    # (16 B) + the FP16 copy (2 B) per table parameter.
+    per_sample = 80 + 16 * 8 * 4 + 2 * 10 * 128 + 128 + 144 + 8 * 8 * 8 + 8 * 8 * 4
+    bytes_step = n_use * per_sample + table_params * 34 + mlp_params * 34
     flops_step = 3 * 2 * n_use * (4 * 64 * 64 + 64 * 16)
     return {
         "rows": n_use,
@@ -202,7 +207,9 @@ def nrc_train_leg(aqua, rows=1 << 18, steps=20, precision=0):
         "ms_adam": round(st.ms_adam / steps, 4),
         "loss": round(float(st.loss), 5),
         "roofline_hbm": {"achieved": round(bytes_step / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "bytes_per_step": bytes_step},
+                         "unit": "GB/s", "bytes_per_step": bytes_step,
+                         "note": "not the bound: the hash-grid scatter's atomics are (one 64-byte segment "
+                                 "per instruction and segment touched)"},
         "mfma_tflops": round(flops_step / (ms * 1e-3) / 1e12, 3),
     }
 
