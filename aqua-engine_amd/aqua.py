@@ -100,7 +100,7 @@ class FrameStats(C.Structure):
 
 class RendererOpts(C.Structure):
     _fields_ = [("samples_per_pass", C.c_uint32), ("count_traversal", C.c_uint32),
-                ("timing", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+                ("timing", C.c_uint32), ("sort_materials", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 NRC_F16, NRC_BF16, NRC_FP8 = 0, 1, 2
@@ -414,8 +414,9 @@ class Renderer:
         except Exception:
             pass
 
-    def set_opts(self, samples_per_pass: int = 0, count_traversal: bool = False, timing: bool = True):
-        o = RendererOpts(samples_per_pass, int(count_traversal), int(timing))
+    def set_opts(self, samples_per_pass: int = 0, count_traversal: bool = False, timing: bool = True,
+                 sort_materials: bool = False):
+        o = RendererOpts(samples_per_pass, int(count_traversal), int(timing), int(sort_materials))
         _check(lib().aqua_renderer_set_opts(self._h, C.byref(o)))
 
     def render(self, spp: int, frame: int = 0, camera: Optional[Camera] = None) -> FrameStats:

@@ -93,6 +93,7 @@ struct Queues {
   float4* shd;     // shadow direction, destination bits
   float4* shc;     // shadow contribution
   float4* Lbuf;    // radiance of finished paths, by path id
+  const uint32_t* order;  // shading order of the hit queue (material sort), or null
   uint32_t* counters;
   uint32_t* cursors;
   unsigned long long* stats;

@@ -98,6 +98,10 @@ __global__ void k_reset(uint32_t* counters, uint32_t initial_rays, uint32_t* cur
   }
 }
 
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
 __global__ void k_zero_stats(unsigned long long* st) {
   uint32_t i = threadIdx.x;
   if (i < kStWords) st[i] = 0ull;
@@ -350,9 +354,10 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
     v3 L = v3{0.0f, 0.0f, 0.0f};
     uint32_t path_id = 0;
     if (valid) {
-      float4 ro4 = q.ro[c][i], rd4 = q.rd[c][i], tp4 = q.tp[c][i], L4 = q.L[c][i];
-      uint4 rg = q.rng[c][i];
-      uint2 h = q.hit[i];
+      const uint32_t src = q.order ? q.order[i] : i;
+      float4 ro4 = q.ro[c][src], rd4 = q.rd[c][src], tp4 = q.tp[c][src], L4 = q.L[c][src];
+      uint4 rg = q.rng[c][src];
+      uint2 h = q.hit[src];
       v3 o = xyz(ro4), d = xyz(rd4), tp = xyz(tp4);
       float pdf_prev = tp4.w;
       L = xyz(L4);
@@ -540,6 +545,77 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
         q.qidx[qi] = qi;
       }
     }
+  }
+}
+
+// ---- material sort of the hit queue ------------------------------------------
+//
+// Optional (aqua_renderer_opts.sort_materials): before shading, the paths of
+// a bounce are put in order of the material they hit (misses last), so a
+// shading wave runs one material's code and texture. A counting sort:
+// k_mat_hist counts each bin, k_mat_scan turns the counts into bin starts
+// (and clears the counts for the next bounce), k_mat_place hands each path a
+// slot in its bin. The order within a bin is arbitrary; results do not
+// depend on it (paths are independent; the film adds by path id).
+constexpr uint32_t kMatBins = 256;  // materials 254 and up share bin 254; 255 = miss
+
+__device__ __forceinline__ uint32_t mat_bin(const DevBvh& bvh, uint2 h) {
+  if (h.x == kMiss) return kMatBins - 1u;
+  return min(__float_as_uint(bvh.tris[3 * size_t(h.x) + 1].w), kMatBins - 2u);
+}
+
+__global__ void __launch_bounds__(kBlock) k_mat_hist(DevBvh bvh, const uint2* hit, const uint32_t* count,
+                                                     uint32_t* bins) {
+  __shared__ uint32_t h[kMatBins];
+  for (uint32_t i = threadIdx.x; i < kMatBins; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    atomicAdd(&h[mat_bin(bvh, hit[i])], 1u);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kMatBins; i += blockDim.x)
+    if (h[i]) atomicAdd(&bins[i], h[i]);
+}
+
+// One block of kMatBins threads.
+__global__ void __launch_bounds__(kMatBins) k_mat_scan(uint32_t* bins, uint32_t* start) {
+  __shared__ uint32_t s[kMatBins];
+  const uint32_t t = threadIdx.x;
+  const uint32_t v = bins[t];
+  s[t] = v;
+  __syncthreads();
+  for (uint32_t o = 1; o < kMatBins; o <<= 1) {
+    const uint32_t add = t >= o ? s[t - o] : 0u;
+    __syncthreads();
+    s[t] += add;
+    __syncthreads();
+  }
+  start[t] = s[t] - v;
+  bins[t] = 0u;
+}
+
+// Each block takes blockDim.x paths at a time: counts its paths per bin in
+// LDS, claims a range of each bin it uses with one global atomic, then
+// writes its paths' slots there.
+__global__ void __launch_bounds__(kBlock) k_mat_place(DevBvh bvh, const uint2* hit, const uint32_t* count,
+                                                      uint32_t* start, uint32_t* order) {
+  __shared__ uint32_t h[kMatBins], base[kMatBins];
+  const uint32_t n = *count;
+  for (uint32_t first = blockIdx.x * blockDim.x; first < n; first += gridDim.x * blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < kMatBins; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const uint32_t i = first + threadIdx.x;
+    uint32_t bin = 0u, rank = 0u;
+    if (i < n) {
+      bin = mat_bin(bvh, hit[i]);
+      rank = atomicAdd(&h[bin], 1u);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kMatBins; k += blockDim.x)
+      if (h[k]) base[k] = atomicAdd(&start[k], h[k]);
+    __syncthreads();
+    if (i < n) order[base[bin] + rank] = i;
+    __syncthreads();
   }
 }
 
@@ -914,6 +990,21 @@ struct Lane {
   DevBuf<float> qout;
   DevBuf<uint32_t> skey, sidx;  // Morton keys, slot order (double buffers)
   DevBuf<uint8_t> stemp;
+  // material sort: shading order, bin counts (zero between bounces), bin starts
+  size_t order_cap = 0;
+  DevBuf<uint32_t> order, mbins, mstart;
+
+  int ensure_order(size_t n, hipStream_t st) {
+    if (!mbins.p) {
+      if (int e = mbins.ensure(kMatBins)) return e;
+      if (int e = mstart.ensure(kMatBins)) return e;
+      k_zero_u32<<<1, kMatBins, 0, st>>>(mbins.p, kMatBins);
+    }
+    if (n <= order_cap) return AQUA_OK;
+    if (int e = order.ensure(n)) return e;
+    order_cap = n;
+    return AQUA_OK;
+  }
 
   int ensure_nrc(size_t n) {
     if (n <= nrc_cap) return AQUA_OK;
@@ -1080,6 +1171,7 @@ struct aqua_renderer {
     q.shd = l.shd.p;
     q.shc = l.shc.p;
     q.Lbuf = l.Lbuf.p;
+    q.order = nullptr;
     q.counters = l.counters.p;
     q.cursors = l.cursors.p;
     q.stats = stats.p;
@@ -1207,6 +1299,9 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
   if (n_slots) k_clear_taux<<<int((n_slots + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.taux, n_slots);
   k_camera<<<g, kBlock, 0, s>>>(pp, q);
   bool timing = r->opts.timing != 0;
+  const bool sort_mats = r->opts.sort_materials != 0;
+  if (sort_mats)
+    if (int e = lane.ensure_order(n, s)) return e;
   // Without per-kernel timing, the shadow rays of bounce b (stream2) overlap
   // the closest-hit rays of bounce b + 1; shading b + 1 waits for them (they
   // add to the radiance it moves, and it rewrites the shadow queue).
@@ -1250,11 +1345,18 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
     else
       k_intersect<false><<<gp, kBlock, 0, s>>>(sc.bvh, QueueIO{q.ro[c], q.rd[c], q.hit}, cnt, cur, q.stats);
     if (timing) HIPCHK(hipEventRecord(ev[1], s));
+    Queues qs = q;
+    if (sort_mats) {
+      k_mat_hist<<<std::min(g, 1024), kBlock, 0, s>>>(sc.bvh, q.hit, cnt, lane.mbins.p);
+      k_mat_scan<<<1, kMatBins, 0, s>>>(lane.mbins.p, lane.mstart.p);
+      k_mat_place<<<g, kBlock, 0, s>>>(sc.bvh, q.hit, cnt, lane.mstart.p, lane.order.p);
+      qs.order = lane.order.p;
+    }
     if (b > 0 && shadow_on_s2(b - 1)) HIPCHK(hipStreamWaitEvent(s, lane.join_ev[b - 1], 0));
     if (pp.nrc)
-      k_shade<true><<<g, kBlock, 0, s>>>(sc, pp, b, q);
+      k_shade<true><<<g, kBlock, 0, s>>>(sc, pp, b, qs);
     else
-      k_shade<false><<<g, kBlock, 0, s>>>(sc, pp, b, q);
+      k_shade<false><<<g, kBlock, 0, s>>>(sc, pp, b, qs);
     if (timing) HIPCHK(hipEventRecord(ev[2], s));
     if (qfork && b == pp.query_depth) {
       HIPCHK(hipEventRecord(lane.q_fork, s));

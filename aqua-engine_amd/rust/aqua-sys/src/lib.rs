@@ -163,7 +163,8 @@ pub mod ffi {
         pub samples_per_pass: u32,
         pub count_traversal: u32,
         pub timing: u32,
-        pub reserved: [u32; 5],
+        pub sort_materials: u32,
+        pub reserved: [u32; 4],
     }
 
     #[repr(C)]

@@ -15,6 +15,8 @@ kernel, and the CPU restatement's rate on a bounded sample of the same frame
 on all host cores. Unless --no-nrc, further legs follow:
   box_room       the 36-triangle box room, 512x512, 16 spp, depth 5, with its
                  relMSE against a 4096-spp FP64 render;
+  material_sort  shading in material order vs path order (textured room
+                 subset, interior), 1080p, 16 spp;
   nrc_train      one cache training step on 2^18 tracer samples;
   nrc_eval       ms/frame to relMSE <= 1e-2 against a 4096-spp FP64 render,
                  plain tracer vs cache (FP16 and BF16, query at vertex 2; FP16
@@ -158,6 +160,52 @@ def box_room_leg(aqua, oracle_spp=4096, steps=10):
     return {"scene": "box room, 36 triangles, 512x512, 16 spp, max depth 5",
             "ms_per_frame": round(wall * 1e3 / steps, 3), "mrays_per_s": round(rays / wall / 1e6, 1),
             "relmse_vs_fp64": round(e, 6), "oracle_spp": oracle_spp}
+
+
+def material_sort_leg(aqua, spp=16, steps=5, width=1920, height=1080):
+    """Shading in material order (aqua_renderer_opts.sort_materials) against
+    path order, plain tracer, depth 8: the textured living-room subset
+    (16 materials, 8 images; the fixture under tests/golden) at its own
+    camera, and the 150-mesh interior. ms/frame (kernels overlapped) and the
+    shading time per frame (per-kernel events; with the sort on it includes
+    the sort's kernels), both ways."""
+    import numpy as np
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        scenes = {
+            "room_subset": aqua.Scene.load(os.path.join(ROOT, "tests", "golden", "room_subset", "scene.json"),
+                                           emission_from_mtl=True),
+            "interior": aqua.Scene.load(aqua.procgen(aqua.PROCGEN_INTERIOR, d, seed=0, n_meshes=150,
+                                                     target_tris=1000000, width=width, height=height)),
+        }
+    for name, scene in scenes.items():
+        base = scene.camera()
+        cam = aqua.Camera()
+        cam.cam_to_world[:] = base.cam_to_world[:]
+        cam.tan_half_fov = base.tan_half_fov
+        cam.width, cam.height = width, height
+        cam.aspect = np.float32(np.float32(width) / np.float32(height))
+        cfg = aqua.integrator_default()
+        cfg.spp, cfg.max_depth = spp, 8
+        r = aqua.Renderer(scene, cfg)
+        res = {"materials": scene.info().n_materials}
+        films = {}
+        for sort in (False, True):
+            r.set_opts(timing=False, sort_materials=sort)
+            r.render(spp, frame=0, camera=cam)
+            t0 = time.perf_counter()
+            for k in range(steps):
+                r.render(spp, frame=k, camera=cam)
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            films[sort] = r.film(cam)
+            r.set_opts(timing=True, sort_materials=sort)
+            st = r.render(spp, frame=steps - 1, camera=cam)
+            res["sorted" if sort else "path_order"] = {"ms_per_frame": round(ms, 3),
+                                                       "ms_shade": round(st.ms_shade, 3)}
+        res["same_film"] = bool(np.array_equal(films[False], films[True]))
+        out[name] = res
+    out["config"] = f"{width}x{height}, {spp} spp, max depth 8"
+    return out
 
 
 def nrc_train_leg(aqua, rows=1 << 18, steps=20, precision=0):
@@ -406,8 +454,8 @@ def main():
                     help="skip the extra legs (box room, cache training step, cache evaluation, large scene)")
     ap.add_argument("--only-nrc", action="store_true", help="skip the interior workload, run the extra legs")
     ap.add_argument("--no-large", action="store_true", help="skip the 5.4 M-triangle 3840x2160 leg")
-    ap.add_argument("--legs", default="box,train,eval,large",
-                    help="the extra legs to run, a comma list of box, train, eval, large")
+    ap.add_argument("--legs", default="box,sort,train,eval,large",
+                    help="the extra legs to run, a comma list of box, sort, train, eval, large")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (the reference kernel for HBM counters)")
     ap.add_argument("--profile-frame", action="store_true",
@@ -443,6 +491,9 @@ def main():
         if "box" in legs:
             print("bench: box room leg", file=sys.stderr, flush=True)
             out["box_room"] = box_room_leg(aqua)
+        if "sort" in legs:
+            print("bench: material sort leg", file=sys.stderr, flush=True)
+            out["material_sort"] = material_sort_leg(aqua)
         if "train" in legs:
             print("bench: cache training leg", file=sys.stderr, flush=True)
             out["nrc_train"] = nrc_train_leg(aqua)

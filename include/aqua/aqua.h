@@ -244,7 +244,9 @@ typedef struct aqua_renderer_opts {
   uint32_t samples_per_pass;   /* 0 = choose (about 128 M paths in flight)    */
   uint32_t count_traversal;    /* 1 = count nodes / triangles (slower)        */
   uint32_t timing;             /* 1 = per-kernel HIP-event timing             */
-  uint32_t reserved[5];
+  uint32_t sort_materials;     /* 1 = order each bounce's hits by material
+                                  before shading (same film)                 */
+  uint32_t reserved[4];
 } aqua_renderer_opts;
 
 int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg, int device,

@@ -119,6 +119,38 @@ def test_room_subset_film_parity(gpu, room_subset, fov_deg):
     assert img_c.mean() > 0.01
 
 
+def test_material_sort_keeps_the_film(gpu, room_subset, box_room):
+    # ordering each bounce's hits by material before shading changes which
+    # wave shades a path, never what it computes: the film of the textured
+    # room (16 materials, 8 images) matches the oracle with the sort on, in
+    # one pass and in several overlapped ones, and the cache's first frame
+    # (queries, training records) is unchanged too
+    sc, orc = room_subset
+    cam = small_camera(sc.camera(), 160, 90)
+    r = aqua.Renderer(sc, cfg(spp=6, max_depth=6))
+    r.set_opts(timing=False, sort_materials=True)
+    st = r.render(6, frame=2, camera=cam)
+    img_c, ost = orc.render(aqua.camera_to_array(cam), 6, 6, frame=2)
+    np.testing.assert_array_equal(r.film(cam), img_c)
+    assert (st.camera_rays, st.extension_rays, st.shadow_rays) == \
+        (ost.camera_rays, ost.extension_rays, ost.shadow_rays)
+    r.set_opts(samples_per_pass=2, timing=False, sort_materials=True)
+    r.render(6, frame=2, camera=cam)
+    np.testing.assert_array_equal(r.film(cam), img_c)
+    c = cfg(spp=2, max_depth=6)
+    c.type = aqua.INTEGRATOR_NRC
+    c.nrc_train_paths = 1024
+    films = []
+    for sort in (False, True):
+        rn = aqua.Renderer(box_room, c)
+        rn.set_opts(timing=False, sort_materials=sort)
+        big = small_camera(box_room.camera(), 96, 64)
+        st = rn.render(2, frame=0, camera=big)
+        assert st.nrc_queries > 0
+        films.append(rn.film(big))
+    np.testing.assert_array_equal(films[0], films[1])
+
+
 def _srgb_code(x):
     x = np.clip(np.asarray(x, np.float64), 0.0, 1.0)
     e = np.where(x <= 0.0031308, 12.92 * x, 1.055 * x ** (1.0 / 2.4) - 0.055)
