@@ -50,10 +50,12 @@ VALU_PEAK_GINSTR = round(157.3e3 / (2 * 64), 1)
 # quantised 4-wide node per node visit and one 48-byte triangle per triangle
 # test.
 RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 32, 8, 64, 48
-# The sources the traversal kernel's instruction stream comes from: a
-# committed counter profile counts only while their digest matches.
-TRAVERSAL_SOURCES = ("aqua-engine_amd/csrc/device/trace.h", "aqua-engine_amd/csrc/device/common.h",
-                     "aqua-engine_amd/csrc/renderer.hip")
+# The sources the traversal kernels' instruction streams come from (the
+# kernels, their work queue, the node / triangle tests, the vector math and
+# the queue and BVH structures): a committed counter profile counts only
+# while their digest matches.
+TRAVERSAL_SOURCES = ("aqua-engine_amd/csrc/device/traverse.h", "aqua-engine_amd/csrc/device/trace.h",
+                     "aqua-engine_amd/csrc/device/common.h", "aqua-engine_amd/csrc/device/kernels.h")
 VALU_PROFILE = "profiles/k_intersect_interior1m_1080p_64spp_valu.json"
 TRAFFIC_PROFILE = "profiles/k_intersect_interior1m_1080p_64spp_traffic.json"
 SHADOW_VALU_PROFILE = "profiles/k_shadow_interior1m_1080p_64spp_valu.json"
