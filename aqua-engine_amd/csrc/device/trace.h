@@ -182,9 +182,11 @@ AQ_DEV float ubyte(uint32_t w, int k) { return float((w >> (8 * k)) & 0xFFu); }
 // Four slab tests of one 64-byte quantised node (four 16-byte loads issued
 // together, straight-line code). A child plane is origin + q * 2^e, so its
 // distance along the ray is q * (2^e / d) + (origin - o) / d: one FMA per
-// plane after six per-node products. Empty children get +inf. ref[k] is
-// the packed child reference (index << 3) | triangle count; leaf[] the
-// node's leaf run (first slot, per-child offset | count bytes).
+// plane after six per-node products. ref[k] is the packed child reference
+// (index << 3) | triangle count; leaf[] the node's leaf run (first slot,
+// per-child offset | count bytes). An empty child is stored as a leaf of no
+// triangles (reference 1, leaf byte 0), so it needs no test here: whatever
+// its distance, it adds no triangle and is never pushed.
 AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float delta, float tn[4],
                        uint32_t ref[4], uint32_t leaf[2]) {
   const float4 h = np[0];
@@ -215,7 +217,7 @@ AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float del
                          __builtin_fmaf(ubyte(ny, k), ay, by), __builtin_fmaf(ubyte(xy, k), ay, by),
                          __builtin_fmaf(ubyte(nz, k), az, bz), __builtin_fmaf(ubyte(xz, k), az, bz), tbest,
                          delta);
-    tn[k] = ref[k] != 0xFFFFFFFFu ? t : __builtin_inff();
+    tn[k] = t;
   }
 }
 
@@ -232,8 +234,9 @@ AQ_DEV uint32_t leaf_mask(const float tn[4], uint32_t bits) {
   uint32_t m = 0u;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const uint32_t b = (bits >> (8 * k)) & 0xFFu;
-    const uint32_t mk = ((1u << (b & 7u)) - 1u) << (b >> 3);
+    // ((1 << count) - 1) << offset in one v_bfm_b32
+    uint32_t mk;
+    asm("v_bfm_b32 %0, %1, %2" : "=v"(mk) : "v"((bits >> (8 * k)) & 7u), "v"((bits >> (8 * k + 3)) & 31u));
     m |= tn[k] != __builtin_inff() ? mk : 0u;
   }
   return m;
@@ -290,22 +293,23 @@ AQ_DEV void closest_node(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCoun
   c.lbase = leaf[0];
   c.lmask = leaf_mask(tn, leaf[1]);
   // key = entry distance bits with the child index in the low two bits
-  // (entry distances are >= 0)
+  // (entry distances are >= 0); a culled inner child keys as +inf | k, a leaf
+  // or empty child as all ones: keys below kKeyInf are the hit inner children
   uint32_t key[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
-    key[k] = (tn[k] != __builtin_inff() && (ref[k] & 7u) == 0u) ? ((__float_as_uint(tn[k]) & ~3u) | uint32_t(k))
-                                                                 : 0xFFFFFFFFu;
+    key[k] = (ref[k] & 7u) == 0u ? ((__float_as_uint(tn[k]) & ~3u) | uint32_t(k)) : 0xFFFFFFFFu;
   cswap(key[0], key[1]);
   cswap(key[2], key[3]);
   cswap(key[0], key[2]);
   cswap(key[1], key[3]);
   cswap(key[1], key[2]);
+  constexpr uint32_t kKeyInf = 0x7F800000u;
 #pragma unroll
   for (int j = 3; j >= 1; --j)
-    if (key[j] != 0xFFFFFFFFu) st_push(st, pick4(ref[0], ref[1], ref[2], ref[3], key[j] & 3u) >> 3);
+    if (key[j] < kKeyInf) st_push(st, pick4(ref[0], ref[1], ref[2], ref[3], key[j] & 3u) >> 3);
   if (COUNT && st.gsp > 0) cnt->deep++;
-  if (key[0] != 0xFFFFFFFFu)
+  if (key[0] < kKeyInf)
     c.node = pick4(ref[0], ref[1], ref[2], ref[3], key[0] & 3u) >> 3;
   else
     c.node = st_empty(st) ? kNoNode : st_pop(st);

@@ -63,14 +63,17 @@ struct Bvh4 {
 //   w4..9  child planes as bytes (byte k = child k): lo.x, hi.x, lo.y, hi.y,
 //          lo.z, hi.z, each rounded outward (lo down, hi up)
 //   w10..13 child references (index << 3) | triangle count (count 0 = inner
-//          node, count <= 7); 0xFFFFFFFF = no child
+//          node, count <= 7); an empty child is kNoChild
 //   w14    first triangle slot of the node's leaf children (one run of
 //          slots; 0 when there are none)
 //   w15    byte k = (offset of child k's triangles in that run << 3) | count
 //          for a leaf child, 0 for an inner or empty child
 // Decoded boxes enclose the original ones, so traversal stays conservative.
 constexpr uint32_t kNodeWords = 16;
-constexpr uint32_t kNoChild = 0xFFFFFFFFu;
+// An empty child: not an inner node (count bits 1) and, with its w15 byte 0,
+// a leaf of no triangles, so traversal treats it like any culled leaf and
+// needs no test of its own (its planes are all 0).
+constexpr uint32_t kNoChild = 1u;
 // false when a node's leaf children do not form one run that fits w15.
 bool quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out);
 

@@ -430,7 +430,8 @@ int aqua_scene_bvh(const aqua_scene* scene, uint32_t* n_nodes, float* nodes, uin
  * scales, words 4-9 the child planes lo.x, hi.x, lo.y, hi.y, lo.z, hi.z as
  * one byte per child (byte k = child k; plane = origin + q * 2^(e - 127),
  * rounded outward), words 10-13 the child references (index << 3) | count
- * (0xFFFFFFFF = empty), word 14 the first triangle slot of the node's leaf
+ * (an empty child is 1 with leaf byte 0: a leaf of no triangles), word 14
+ * the first triangle slot of the node's leaf
  * children (which take one run of slots), word 15 one byte per child: offset
  * into that run << 3 | triangle count for a leaf child, 0 otherwise. Same
  * calling pattern as above. */

@@ -396,7 +396,10 @@ def test_device_nodes_enclose_the_exact_boxes(box_room, tmp_path):
         idx = nodes[:, 24:28].view(np.int32)
         cnt = nodes[:, 28:32].view(np.int32)
         used = idx >= 0
-        np.testing.assert_array_equal(ref == 0xFFFFFFFF, ~used)
+        # an empty child is stored as a leaf of no triangles: reference 1 and
+        # leaf byte 0 (a real leaf has a nonzero count in its byte)
+        byte15 = np.stack([(words[:, 15] >> (8 * k)) & 0xFF for k in range(4)], 1)
+        np.testing.assert_array_equal((ref == 1) & (byte15 == 0), ~used)
         np.testing.assert_array_equal(ref[used] >> 3, idx[used].astype(np.uint32))
         np.testing.assert_array_equal(ref[used] & 7, cnt[used].astype(np.uint32))
         # the leaf run: word 14 = first leaf slot, word 15 = per child
