@@ -47,11 +47,6 @@ constexpr int kLevels = 16;
 // atomics are few.
 constexpr int kFineLevel0 = 8;
 static_assert(kFineLevel0 == 8, "k_nrc_scatter merges levels 0..7 and adds levels 8..15 as BF16 pairs");
-// Gather layout: corner-parallel (the four lanes of a row each take one
-// corner pair of every level) or per level (each lane all eight corners of
-// its four levels). Per level measures faster for queries (2.92 against
-// 3.24 ms for the 6.7 M queries of a 4-spp 1080p box-room frame).
-constexpr bool kCpGather = false;
 // Training stages the forward and backward weight operands in LDS per block
 // (72 KB in 16-bit) instead of reading them through L1 per layer and tile.
 constexpr bool kLdsTrainWeights = true;
@@ -269,54 +264,11 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 // Encodes this lane group's 16 input features of one row: f[ks*8 + i] is
 // feature kappa(g, ks, i) (hash levels 2g, 2g+1, 8+2g, 9+2g; SH 4g..4g+3;
-// extras 4g..4g+3).
-//
-// CP (queries): corner-parallel gather. The four lanes of a column (lane %
-// 16 = the row) hold the same row; for every level each fetches one of the
-// cell's four corner pairs (x, x + 1 at y + (g & 1), z + (g >> 1)), weighs
-// it, and the four partial sums are added across the lane groups. A gather
-// instruction then covers one level of 16 rows (about 2.3 lines a row)
-// instead of 64 unrelated cells, which is what an L1 that serves about a
-// line per clock needs. Lane group g keeps the features of its own levels.
-// Otherwise (training, whose scatter owns levels per lane) each lane sums
-// all eight corners of its four levels.
-template <bool CP>
+// extras 4g..4g+3). Each lane sums all eight corners of its four levels.
 __device__ __forceinline__ void encode16(const FwdArgs& a, uint32_t T_, int g, bool valid, float px, float py,
                                          float pz, float4 r0, float4 r1, float4 r2, float4 r3, float f[16]) {
-  if constexpr (CP) {
-    // feature pair q of owner group o is level (q < 2 ? 2 o + q : 8 + 2 o + q - 2);
-    // q unrolled (static indices into f), o a loop (few live registers)
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
 #pragma unroll 1
-    for (int o = 0; o < 4; ++o) {
-      const int lv = q < 2 ? 2 * o + q : 8 + 2 * o + (q - 2);
-      Cell c = cell_of(a.G, lv, px, py, pz);
-      const uint32_t* tl = a.table16 + size_t(lv) * T_;
-      const bool pair = ((a.G.dense_mask >> lv) & 1u) || (c.c[0] & 3) != 3;
-      const int cy = c.c[1] + (g & 1), cz = c.c[2] + (g >> 1);
-      u2a u = *reinterpret_cast<const u2a*>(tl + grid_index(a.G, lv, c.c[0], cy, cz));
-      if (!pair) u.y = tl[grid_index(a.G, lv, c.c[0] + 1, cy, cz)];
-      float e0, e1;
-      unpack_entry(u.x, &e0, &e1);
-      float w = corner_w(c, 2 * g);
-      float s0 = w * e0, s1 = w * e1;
-      unpack_entry(u.y, &e0, &e1);
-      w = corner_w(c, 2 * g + 1);
-      s0 = __builtin_fmaf(w, e0, s0);
-      s1 = __builtin_fmaf(w, e1, s1);
-      s0 += __shfl_xor(s0, 16);
-      s1 += __shfl_xor(s1, 16);
-      s0 += __shfl_xor(s0, 32);
-      s1 += __shfl_xor(s1, 32);
-      if (o == g) {
-        f[2 * q] = valid ? s0 : 0.0f;
-        f[2 * q + 1] = valid ? s1 : 0.0f;
-      }
-    }
-  }
-#pragma unroll 1
-  for (int q = 0; q < 4 && !CP; ++q) {
+  for (int q = 0; q < 4; ++q) {
     int lv = q < 2 ? 2 * g + q : 8 + 2 * g + (q - 2);
     Cell c = cell_of(a.G, lv, px, py, pz);
     float s0 = 0.0f, s1 = 0.0f;
@@ -460,7 +412,7 @@ __global__ void __launch_bounds__(TRAIN ? kTrainBlock : kBlock) k_nrc_fwd(FwdArg
     float cur[4][4];  // current layer input, feature 16 mt + 4 g + j
     {
       float f[16];
-      encode16<kCpGather && !TRAIN>(a, T_, g, valid, px, py, pz, r0, r1, r2, r3, f);
+      encode16(a, T_, g, valid, px, py, pz, r0, r1, r2, r3, f);
 #pragma unroll
       for (int i = 0; i < 16; ++i) cur[2 * (i >> 3) + ((i & 7) >> 2)][i & 3] = f[i];
     }
