@@ -20,7 +20,8 @@ on all host cores. Unless --no-nrc, further legs follow:
   nrc_train      one cache training step on 2^18 tracer samples;
   nrc_eval       ms/frame to relMSE <= 1e-2 against a 4096-spp FP64 render,
                  plain tracer vs cache (FP16 and BF16, query at vertex 2; FP16
-                 at vertex 1 beside it) on the box-room fly-through at 1080p;
+                 at vertex 1 beside it) on 24 frames of the box-room
+                 fly-through at 1080p;
   large          the 400-mesh / 5.4 M-triangle interior at 3840x2160, 1 spp
                  with an FP8 (e4m3) and an FP16 cache over a 60-frame
                  fly-through, error against a 64-spp FP64 render.
@@ -283,12 +284,14 @@ def _ladder(render, oracle, max_spp, before=None):
     return res
 
 
-def nrc_eval_leg(aqua, frames=tuple(range(15, 240, 30)), n_frames=240, oracle_spp=4096, width=1920,
+def nrc_eval_leg(aqua, frames=tuple(range(5, 240, 10)), n_frames=240, oracle_spp=4096, width=1920,
                  height=1080, max_spp=256):
     """ms/frame to relMSE <= 1e-2 against an FP64 render of `oracle_spp` samples,
     plain path tracer vs cache, on the box room fly-through: max depth 8, 2^16
     training suffix paths per frame, online training at 1 spp through every
-    frame before an evaluated one.
+    frame before an evaluated one. Every tenth frame of the 240 is evaluated
+    (24 frames; the FP64 render takes ~3.7 s a frame, so all 240 would take a
+    quarter of an hour); the figures are means over them.
 
     Each evaluated frame uses three disjoint sample sets: the oracle renders
     frame index f, the plain tracer f + 100000, the cache f + 200000. The
