@@ -17,6 +17,7 @@ on all host cores. Unless --no-nrc, further legs follow:
                  relMSE against a 4096-spp FP64 render;
   material_sort  shading in material order vs path order (textured room
                  subset, interior), 1080p, 16 spp;
+  nrc_infer      4-spp cache frames at 1080p: frame time and the cache's own work;
   nrc_train      one cache training step on 2^18 tracer samples;
   nrc_eval       ms/frame to relMSE <= 1e-2 against a 4096-spp FP64 render,
                  plain tracer vs cache (FP16 and BF16, query at vertex 2; FP16
@@ -284,6 +285,42 @@ def _ladder(render, oracle, max_spp, before=None):
     return res
 
 
+def nrc_infer_leg(aqua, warm_frames=40, frames=5, spp=4, width=1920, height=1080):
+    """Cache frames alone: the box room fly-through at 1080p with an FP16 cache
+    (query at vertex 2, 2^16 training paths), trained online over
+    `warm_frames` 1-spp frames, then `frames` frames of `spp` samples. ms per
+    frame with kernels overlapped, and with per-kernel events the cache's own
+    work in a frame (query sort, inference, apply, the frame's training step)."""
+    import numpy as np
+    with tempfile.TemporaryDirectory() as d:
+        scene = aqua.Scene.load(aqua.procgen(aqua.PROCGEN_BOX_ROOM, d, width=width, height=height))
+    c = aqua.integrator_default()
+    c.type = aqua.INTEGRATOR_NRC
+    c.max_depth = 8
+    c.learning_rate = 1e-2
+    c.nrc_precision = aqua.NRC_F16
+    c.nrc_train_paths = 1 << 16
+    r = aqua.Renderer(scene, c)
+    r.set_opts(timing=False)
+    for f in range(warm_frames):
+        r.render(1, frame=f, camera=aqua.camera_flythrough(scene, f, 240, width=width, height=height))
+    ms, nq = [], 0
+    for f in range(warm_frames, warm_frames + frames):
+        st = r.render(spp, frame=f, camera=aqua.camera_flythrough(scene, f, 240, width=width, height=height))
+        ms.append(st.ms_frame)
+        nq = int(st.nrc_queries)
+    r.set_opts(timing=True)
+    mc, mf = [], []
+    for f in range(warm_frames + frames, warm_frames + 2 * frames):
+        st = r.render(spp, frame=f, camera=aqua.camera_flythrough(scene, f, 240, width=width, height=height))
+        mc.append(st.ms_nrc)
+        mf.append(st.ms_frame)
+    return {"scene": f"box room, {width}x{height}, {spp} spp, max depth 8, FP16 cache after {warm_frames} frames",
+            "queries_per_frame": nq, "ms_per_frame": round(float(np.mean(ms)), 3),
+            "ms_per_frame_kernels_serial": round(float(np.mean(mf)), 3),
+            "ms_cache_work": round(float(np.mean(mc)), 3)}
+
+
 def nrc_eval_leg(aqua, frames=tuple(range(5, 240, 10)), n_frames=240, oracle_spp=4096, width=1920,
                  height=1080, max_spp=256):
     """ms/frame to relMSE <= 1e-2 against an FP64 render of `oracle_spp` samples,
@@ -459,8 +496,8 @@ def main():
                     help="skip the extra legs (box room, cache training step, cache evaluation, large scene)")
     ap.add_argument("--only-nrc", action="store_true", help="skip the interior workload, run the extra legs")
     ap.add_argument("--no-large", action="store_true", help="skip the 5.4 M-triangle 3840x2160 leg")
-    ap.add_argument("--legs", default="box,sort,train,eval,large",
-                    help="the extra legs to run, a comma list of box, sort, train, eval, large")
+    ap.add_argument("--legs", default="box,sort,infer,train,eval,large",
+                    help="the extra legs to run, a comma list of box, sort, infer, train, eval, large")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (the reference kernel for HBM counters)")
     ap.add_argument("--profile-frame", action="store_true",
@@ -499,6 +536,9 @@ def main():
         if "sort" in legs:
             print("bench: material sort leg", file=sys.stderr, flush=True)
             out["material_sort"] = material_sort_leg(aqua)
+        if "infer" in legs:
+            print("bench: cache inference leg", file=sys.stderr, flush=True)
+            out["nrc_infer"] = nrc_infer_leg(aqua)
         if "train" in legs:
             print("bench: cache training leg", file=sys.stderr, flush=True)
             out["nrc_train"] = nrc_train_leg(aqua)
