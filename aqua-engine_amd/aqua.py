@@ -179,6 +179,7 @@ _SIGS = {
     "aqua_nrc_gradients": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, C.POINTER(C.c_float)]),
     "aqua_renderer_nrc": (_P, [_P]),
     "aqua_renderer_set_nrc_enabled": (C.c_int, [_P, C.c_int]),
+    "aqua_renderer_set_nrc_precision": (C.c_int, [_P, C.c_uint32]),
     "aqua_renderer_nrc_checkpoint": (C.c_int, [_P, C.c_int]),
     "aqua_nrc_checkpoint": (C.c_int, [_P, C.c_int]),
     "aqua_renderer_nrc_batch": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(_P), C.POINTER(_P)]),
@@ -478,6 +479,10 @@ class Renderer:
 
     def set_nrc_enabled(self, enabled: bool) -> None:
         _check(lib().aqua_renderer_set_nrc_enabled(self._h, int(enabled)))
+
+    def set_nrc_precision(self, precision: int) -> None:
+        """Inference precision of the cache from the next frame on (same weights)."""
+        _check(lib().aqua_renderer_set_nrc_precision(self._h, precision))
 
     def nrc_checkpoint(self, restore: bool = False) -> None:
         """Saves (restore False) or puts back (True) the cache's training state."""

@@ -1888,6 +1888,14 @@ int aqua_renderer_set_nrc_enabled(aqua_renderer* r, int enabled) {
   return AQUA_OK;
 }
 
+int aqua_renderer_set_nrc_precision(aqua_renderer* r, uint32_t precision) {
+  if (!r) return aqua::fail(AQUA_ERR_INVALID_ARG, "null renderer");
+  if (!r->nrc) return aqua::fail(AQUA_ERR_INVALID_ARG, "the renderer has no cache (type is not nrc)");
+  if (precision > AQUA_NRC_FP8) return aqua::fail(AQUA_ERR_INVALID_ARG, "unknown nrc_precision %u", precision);
+  r->cfg.nrc_precision = precision;
+  return AQUA_OK;
+}
+
 int aqua_renderer_nrc_checkpoint(aqua_renderer* r, int restore) {
   if (!r) return aqua::fail(AQUA_ERR_INVALID_ARG, "null renderer");
   if (!r->nrc) return aqua::fail(AQUA_ERR_INVALID_ARG, "the renderer has no cache (type is not nrc)");

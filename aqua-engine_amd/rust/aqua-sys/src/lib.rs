@@ -251,6 +251,7 @@ pub mod ffi {
         pub fn aqua_nrc_train_device(nrc: *mut aqua_nrc, n: u32, d_records: *const c_void, d_targets: *const c_void, steps: u32, stats: *mut aqua_nrc_stats) -> c_int;
         pub fn aqua_renderer_nrc(r: *mut aqua_renderer) -> *mut aqua_nrc;
         pub fn aqua_renderer_set_nrc_enabled(r: *mut aqua_renderer, enabled: c_int) -> c_int;
+        pub fn aqua_renderer_set_nrc_precision(r: *mut aqua_renderer, precision: u32) -> c_int;
         pub fn aqua_renderer_nrc_checkpoint(r: *mut aqua_renderer, restore: c_int) -> c_int;
         pub fn aqua_renderer_nrc_batch(r: *mut aqua_renderer, n: *mut u32, d_records: *mut *const c_void, d_targets: *mut *const c_void) -> c_int;
         // ---- misc

@@ -392,6 +392,12 @@ aqua_nrc* aqua_renderer_nrc(aqua_renderer* r);
 /* 0 disables the cache (frames render as the plain path tracer, which is
  * also the fallback when training diverges); 1 re-enables it. */
 int aqua_renderer_set_nrc_enabled(aqua_renderer* r, int enabled);
+/* The precision the cache's queries run in from the next frame on:
+ * AQUA_NRC_F16, AQUA_NRC_BF16 or AQUA_NRC_FP8, from the same weights (the
+ * training precision, fixed when the renderer was created, does not change).
+ * With aqua_renderer_nrc_checkpoint it renders one cache state in two
+ * precisions. */
+int aqua_renderer_set_nrc_precision(aqua_renderer* r, uint32_t precision);
 /* aqua_nrc_checkpoint of the renderer's cache, plus the renderer's own cache
  * schedule (training steps taken, cache on / off), between frames. */
 int aqua_renderer_nrc_checkpoint(aqua_renderer* r, int restore);

@@ -193,3 +193,35 @@ def test_cache_checkpoint_restores_the_training_state(gpu, box_room):
     r.nrc_checkpoint(restore=True)
     r.render(2, frame=100, camera=cam)
     np.testing.assert_array_equal(r.film(cam), at_ckpt)
+
+
+def test_cache_query_precision_switches_on_one_state(gpu, box_room):
+    # The query precision is a property of inference only: the same saved
+    # state renders a frame in FP8 and in BF16 from one set of weights, the
+    # two frames differ by FP8's rounding alone, and switching back and
+    # restoring gives the FP8 frame again bit for bit.
+    cam = small_camera(box_room.camera(), 96, 64)
+    c = nrc_cfg(max_depth=8, train_paths=2048, precision=aqua.NRC_FP8)
+    r = aqua.Renderer(box_room, c)
+    for f in range(8):
+        r.render(2, frame=f, camera=cam)
+    r.nrc_checkpoint()
+    r.render(2, frame=100, camera=cam)
+    fp8 = r.film(cam).astype(np.float64)
+    r.nrc_checkpoint(restore=True)
+    r.set_nrc_precision(aqua.NRC_BF16)
+    st = r.render(2, frame=100, camera=cam)
+    assert st.nrc_queries > 0
+    bf16 = r.film(cam).astype(np.float64)
+    assert not np.array_equal(fp8, bf16)
+    rel = np.mean((fp8 - bf16) ** 2 / (bf16 ** 2 + 1e-2))
+    assert rel < 0.05, rel
+    r.nrc_checkpoint(restore=True)
+    r.set_nrc_precision(aqua.NRC_FP8)
+    r.render(2, frame=100, camera=cam)
+    np.testing.assert_array_equal(r.film(cam), fp8.astype(np.float32))
+    with pytest.raises(aqua.AquaError):
+        r.set_nrc_precision(7)
+    pt = aqua.Renderer(box_room, aqua.integrator_default())
+    with pytest.raises(aqua.AquaError):
+        pt.set_nrc_precision(aqua.NRC_F16)
