@@ -80,15 +80,23 @@ AQ_DEV bool tri_hit(const RayPre& r, v3 v0, v3 v1, v3 v2, float* t_out) {
 }
 
 // Slab test of one box from its near and far plane distances along the ray
-// per axis; returns the entry distance, or +inf when culled. The t values are
-// finite (|1/d| is clamped), so the hardware min / max (v_min3 / v_max3)
-// apply. Culling only has to be conservative: the far bound is widened by
-// kFarScale and delta, which also covers the rounding of the plane distances.
-AQ_DEV float slab_enter(float nx, float fx, float ny, float fy, float nz, float fz, float tbest, float delta) {
-  float tn = __builtin_fmaxf(__builtin_fmaxf(nx, ny), __builtin_fmaxf(nz, 0.0f));
-  float tf = __builtin_fminf(__builtin_fminf(fx, fy), __builtin_fminf(fz, tbest));
-  return tn <= __builtin_fmaf(tf, kFarScale, delta) ? tn : __builtin_inff();
+// per axis: *tn = the entry distance, true when the box is not culled. The t
+// values are finite (|1/d| is clamped), so the hardware min / max (v_min3 /
+// v_max3) apply. Culling only has to be conservative: the far bound is
+// widened by kFarScale and delta, which also covers the rounding of the
+// plane distances.
+AQ_DEV bool slab_test(float nx, float fx, float ny, float fy, float nz, float fz, float tbest, float delta,
+                      float* tn) {
+  const float e = __builtin_fmaxf(__builtin_fmaxf(nx, ny), __builtin_fmaxf(nz, 0.0f));
+  const float f = __builtin_fminf(__builtin_fminf(fx, fy), __builtin_fminf(fz, tbest));
+  *tn = e;
+  return e <= __builtin_fmaf(f, kFarScale, delta);
 }
+
+// A child reference with this bit set is a leaf: the rest of the word is the
+// mask of its triangle slots over the node's leaf run (an empty child is a
+// leaf of no triangles). Without it, the word is an inner node's index.
+constexpr uint32_t kLeafFlag = 0x80000000u;
 
 // The first kTopNodes nodes (the builder puts the top three levels there)
 // are staged in LDS by every traversal block: every ray starts at the root,
@@ -150,8 +158,9 @@ AQ_DEV void st_init(TStack& s, uint32_t* lds_base, uint32_t* spill_base) {
   s.cnt = 0;
   s.gsp = 0;
 }
+static_assert((kLdsStack & (kLdsStack - 1)) == 0, "the ring index wraps with a mask");
 AQ_DEV void st_push(TStack& s, uint32_t v) {
-  int slot = s.top + 1 == kLdsStack ? 0 : s.top + 1;
+  const int slot = (s.top + 1) & (kLdsStack - 1);
   if (s.cnt == kLdsStack)
     s.spill[size_t(s.gsp++) * s.sstride] = s.lds[slot * kStackThreads];
   else
@@ -163,7 +172,7 @@ AQ_DEV bool st_empty(const TStack& s) { return s.cnt == 0 && s.gsp == 0; }
 AQ_DEV uint32_t st_pop(TStack& s) {
   if (s.cnt > 0) {
     uint32_t v = s.lds[s.top * kStackThreads];
-    s.top = s.top == 0 ? kLdsStack - 1 : s.top - 1;
+    s.top = (s.top - 1) & (kLdsStack - 1);
     --s.cnt;
     return v;
   }
@@ -179,31 +188,35 @@ AQ_DEV uint32_t pick4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t k
 // Byte k of a word as a float (v_cvt_f32_ubyteK).
 AQ_DEV float ubyte(uint32_t w, int k) { return float((w >> (8 * k)) & 0xFFu); }
 
+// 2^e x v for the signed exponent byte at bit `sh` of w (v_bfe_i32 and
+// v_ldexp_f32; exact but for overflow and underflow, like the product).
+AQ_DEV float scale_by(float v, uint32_t w, int sh) {
+  return __builtin_amdgcn_ldexpf(v, int(int8_t(uint8_t(w >> sh))));
+}
+
 // Four slab tests of one 64-byte quantised node (four 16-byte loads issued
 // together, straight-line code). A child plane is origin + q * 2^e, so its
 // distance along the ray is q * (2^e / d) + (origin - o) / d: one FMA per
-// plane after six per-node products. ref[k] is the packed child reference
-// (index << 3) | triangle count; leaf[] the node's leaf run (first slot,
-// per-child offset | count bytes). An empty child is stored as a leaf of no
-// triangles (reference 1, leaf byte 0), so it needs no test here: whatever
-// its distance, it adds no triangle and is never pushed.
-AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float delta, float tn[4],
-                       uint32_t ref[4], uint32_t leaf[2]) {
+// plane after six per-node products. hit[k] tells whether child k's box
+// survives culling and tn[k] is its entry distance; ref[k] is the child
+// reference (an inner node's index, or kLeafFlag | the leaf's slot mask over
+// the run starting at *lbase). An empty child is a leaf of no triangles, so
+// it needs no test here: whatever its distance, it adds no triangle and is
+// never pushed.
+AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float delta, float tn[4], bool hit[4],
+                       uint32_t ref[4], uint32_t* lbase) {
   const float4 h = np[0];
   const uint4 q0 = *reinterpret_cast<const uint4*>(np + 1);  // lo.x hi.x lo.y hi.y
   const uint4 q1 = *reinterpret_cast<const uint4*>(np + 2);  // lo.z hi.z ref0 ref1
-  const uint4 q2 = *reinterpret_cast<const uint4*>(np + 3);  // ref2 ref3 leaf run
+  const uint4 q2 = *reinterpret_cast<const uint4*>(np + 3);  // ref2 ref3 leaf run -
   const uint32_t e = __float_as_uint(h.w);
-  const float sx = __uint_as_float((e & 0xFFu) << 23), sy = __uint_as_float(((e >> 8) & 0xFFu) << 23),
-              sz = __uint_as_float(((e >> 16) & 0xFFu) << 23);
-  const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
+  const float ax = scale_by(r.inv.x, e, 0), ay = scale_by(r.inv.y, e, 8), az = scale_by(r.inv.z, e, 16);
   const float bx = (h.x - r.o.x) * r.inv.x, by = (h.y - r.o.y) * r.inv.y, bz = (h.z - r.o.z) * r.inv.z;
   ref[0] = q1.z;
   ref[1] = q1.w;
   ref[2] = q2.x;
   ref[3] = q2.y;
-  leaf[0] = q2.z;
-  leaf[1] = q2.w;
+  *lbase = q2.z;
   // Near and far planes per axis from the sign of the direction: the plane
   // distance fma(q, a, b) is monotone in q, and a has the sign of 1/d, so
   // this picks exactly min / max of the two distances.
@@ -212,13 +225,11 @@ AQ_DEV void node_test4(const RayPre& r, const float4* np, float tbest, float del
   const uint32_t ny = fy ? q0.w : q0.z, xy = fy ? q0.z : q0.w;
   const uint32_t nz = fz ? q1.y : q1.x, xz = fz ? q1.x : q1.y;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    float t = slab_enter(__builtin_fmaf(ubyte(nx, k), ax, bx), __builtin_fmaf(ubyte(xx, k), ax, bx),
-                         __builtin_fmaf(ubyte(ny, k), ay, by), __builtin_fmaf(ubyte(xy, k), ay, by),
-                         __builtin_fmaf(ubyte(nz, k), az, bz), __builtin_fmaf(ubyte(xz, k), az, bz), tbest,
-                         delta);
-    tn[k] = t;
-  }
+  for (int k = 0; k < 4; ++k)
+    hit[k] = slab_test(__builtin_fmaf(ubyte(nx, k), ax, bx), __builtin_fmaf(ubyte(xx, k), ax, bx),
+                       __builtin_fmaf(ubyte(ny, k), ay, by), __builtin_fmaf(ubyte(xy, k), ay, by),
+                       __builtin_fmaf(ubyte(nz, k), az, bz), __builtin_fmaf(ubyte(xz, k), az, bz), tbest, delta,
+                       &tn[k]);
 }
 
 AQ_DEV void cswap(uint32_t& a, uint32_t& b) {
@@ -227,19 +238,12 @@ AQ_DEV void cswap(uint32_t& a, uint32_t& b) {
   b = hi;
 }
 
-// The hit leaf children of a node as a bit mask over the node's run of leaf
-// slots (bit i = slot leaf[0] + i): leaf[1] holds one byte per child,
-// (offset in the run << 3) | triangle count, 0 for inner and empty children.
-AQ_DEV uint32_t leaf_mask(const float tn[4], uint32_t bits) {
+// The triangles of a node's hit leaf children as one mask over its leaf run.
+AQ_DEV uint32_t leaf_mask(const bool hit[4], const uint32_t ref[4]) {
   uint32_t m = 0u;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    // ((1 << count) - 1) << offset in one v_bfm_b32
-    uint32_t mk;
-    asm("v_bfm_b32 %0, %1, %2" : "=v"(mk) : "v"((bits >> (8 * k)) & 7u), "v"((bits >> (8 * k + 3)) & 31u));
-    m |= tn[k] != __builtin_inff() ? mk : 0u;
-  }
-  return m;
+  for (int k = 0; k < 4; ++k) m |= hit[k] && (ref[k] & kLeafFlag) ? ref[k] : 0u;
+  return m & ~kLeafFlag;
 }
 
 // ---- resumable traversal: the state of one ray between steps, so a
@@ -286,19 +290,19 @@ AQ_DEV bool closest_done(const ClosestRay& c) { return c.node == kNoNode && c.lm
 template <bool COUNT>
 AQ_DEV void closest_node(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCount* cnt) {
   float tn[4];
+  bool hit[4];
   uint32_t ref[4];
-  uint32_t leaf[2];
-  node_test4(c.r, (c.node < bvh.n_top ? bvh.top : bvh.nodes) + 4 * c.node, c.tbest, bvh.delta, tn, ref, leaf);
+  node_test4(c.r, (c.node < bvh.n_top ? bvh.top : bvh.nodes) + 4 * c.node, c.tbest, bvh.delta, tn, hit, ref,
+             &c.lbase);
   if (COUNT) cnt->nodes++;
-  c.lbase = leaf[0];
-  c.lmask = leaf_mask(tn, leaf[1]);
+  c.lmask = leaf_mask(hit, ref);
   // key = entry distance bits with the child index in the low two bits
-  // (entry distances are >= 0); a culled inner child keys as +inf | k, a leaf
-  // or empty child as all ones: keys below kKeyInf are the hit inner children
+  // (entry distances are finite and >= 0) for a hit inner child, all ones
+  // otherwise: keys below kKeyInf are the hit inner children
   uint32_t key[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
-    key[k] = (ref[k] & 7u) == 0u ? ((__float_as_uint(tn[k]) & ~3u) | uint32_t(k)) : 0xFFFFFFFFu;
+    key[k] = hit[k] && !(ref[k] & kLeafFlag) ? ((__float_as_uint(tn[k]) & ~3u) | uint32_t(k)) : 0xFFFFFFFFu;
   cswap(key[0], key[1]);
   cswap(key[2], key[3]);
   cswap(key[0], key[2]);
@@ -307,10 +311,10 @@ AQ_DEV void closest_node(const DevBvh& bvh, TStack& st, ClosestRay& c, TraceCoun
   constexpr uint32_t kKeyInf = 0x7F800000u;
 #pragma unroll
   for (int j = 3; j >= 1; --j)
-    if (key[j] < kKeyInf) st_push(st, pick4(ref[0], ref[1], ref[2], ref[3], key[j] & 3u) >> 3);
+    if (key[j] < kKeyInf) st_push(st, pick4(ref[0], ref[1], ref[2], ref[3], key[j] & 3u));
   if (COUNT && st.gsp > 0) cnt->deep++;
   if (key[0] < kKeyInf)
-    c.node = pick4(ref[0], ref[1], ref[2], ref[3], key[0] & 3u) >> 3;
+    c.node = pick4(ref[0], ref[1], ref[2], ref[3], key[0] & 3u);
   else
     c.node = st_empty(st) ? kNoNode : st_pop(st);
 }
@@ -387,20 +391,20 @@ AQ_DEV bool any_done(const AnyRay& a) { return a.node == kNoNode && a.lmask == 0
 template <bool COUNT>
 AQ_DEV void any_node(const DevBvh& bvh, TStack& st, AnyRay& a, TraceCount* cnt) {
   float tn[4];
+  bool hit[4];
   uint32_t ref[4];
-  uint32_t leaf[2];
-  node_test4(a.r, (a.node < bvh.n_top ? bvh.top : bvh.nodes) + 4 * a.node, a.tmax, bvh.delta, tn, ref, leaf);
+  node_test4(a.r, (a.node < bvh.n_top ? bvh.top : bvh.nodes) + 4 * a.node, a.tmax, bvh.delta, tn, hit, ref,
+             &a.lbase);
   if (COUNT) cnt->nodes++;
-  a.lbase = leaf[0];
-  a.lmask = leaf_mask(tn, leaf[1]);
+  a.lmask = leaf_mask(hit, ref);
   uint32_t next = kNoNode;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    if (tn[k] == __builtin_inff() || (ref[k] & 7u) != 0u) continue;
+    if (!hit[k] || (ref[k] & kLeafFlag)) continue;
     if (next == kNoNode)
-      next = ref[k] >> 3;
+      next = ref[k];
     else
-      st_push(st, ref[k] >> 3);
+      st_push(st, ref[k]);
   }
   if (COUNT && st.gsp > 0) cnt->deep++;
   a.node = next != kNoNode ? next : (st_empty(st) ? kNoNode : st_pop(st));

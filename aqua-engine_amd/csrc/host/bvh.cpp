@@ -500,7 +500,7 @@ bool quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out) {
       ex[a] = std::max(-100, std::min(100, e));
     }
     for (int a = 0; a < 3; ++a) memcpy(&w[a], &origin[a], 4);
-    w[3] = uint32_t(ex[0] + 127) | (uint32_t(ex[1] + 127) << 8) | (uint32_t(ex[2] + 127) << 16);
+    w[3] = (uint32_t(ex[0]) & 0xFFu) | ((uint32_t(ex[1]) & 0xFFu) << 8) | ((uint32_t(ex[2]) & 0xFFu) << 16);
     for (int k = 0; k < 4; ++k) {
       if (n.child[k] < 0) {
         w[10 + k] = kNoChild;
@@ -515,10 +515,11 @@ bool quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out) {
         w[4 + 2 * a] |= bl << (8 * k);
         w[5 + 2 * a] |= bh << (8 * k);
       }
-      w[10 + k] = (uint32_t(n.child[k]) << 3) | uint32_t(n.count[k]);
+      if (n.count[k] == 0) w[10 + k] = uint32_t(n.child[k]);  // inner node
     }
     // the leaf run: collapse_bvh4 gives a node's leaf children consecutive
-    // slots in child order, at most 4 x 7 of them
+    // slots in child order, at most 4 x 7 of them; a leaf child's reference
+    // is the flag plus its slots as a mask over the run
     uint32_t base = 0xFFFFFFFFu;
     for (int k = 0; k < 4; ++k)
       if (n.child[k] >= 0 && n.count[k] > 0) base = std::min(base, uint32_t(n.child[k]));
@@ -527,8 +528,8 @@ bool quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out) {
       for (int k = 0; k < 4; ++k)
         if (n.child[k] >= 0 && n.count[k] > 0) {
           const uint32_t off = uint32_t(n.child[k]) - base;
-          if (n.count[k] > 7 || off + uint32_t(n.count[k]) > 32) return false;
-          w[15] |= ((off << 3) | uint32_t(n.count[k])) << (8 * k);
+          if (n.count[k] > 7 || off + uint32_t(n.count[k]) > kLeafMaskBits) return false;
+          w[10 + k] = kLeafFlag | (((1u << n.count[k]) - 1u) << off);
         }
     }
   }

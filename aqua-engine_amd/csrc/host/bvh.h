@@ -58,22 +58,24 @@ struct Bvh4 {
 
 // Device node format: 64 bytes = 16 words per 4-wide node.
 //   w0..2  origin (float): the low corner of the union of the child boxes
-//   w3     per-axis power-of-two scales as biased exponents (bits 0-7 x,
-//          8-15 y, 16-23 z): plane = origin + q * 2^(e - 127)
+//   w3     per-axis power-of-two scales as signed exponent bytes (bits 0-7
+//          x, 8-15 y, 16-23 z, two's complement): plane = origin + q * 2^e
 //   w4..9  child planes as bytes (byte k = child k): lo.x, hi.x, lo.y, hi.y,
 //          lo.z, hi.z, each rounded outward (lo down, hi up)
-//   w10..13 child references (index << 3) | triangle count (count 0 = inner
-//          node, count <= 7); an empty child is kNoChild
+//   w10..13 child references: an inner child is its node index (< 2^31); a
+//          leaf child is kLeafFlag | the mask of its triangle slots over the
+//          node's leaf run (bit i = slot w14 + i); an empty child is kNoChild
 //   w14    first triangle slot of the node's leaf children (one run of
 //          slots; 0 when there are none)
-//   w15    byte k = (offset of child k's triangles in that run << 3) | count
-//          for a leaf child, 0 for an inner or empty child
+//   w15    0
 // Decoded boxes enclose the original ones, so traversal stays conservative.
 constexpr uint32_t kNodeWords = 16;
-// An empty child: not an inner node (count bits 1) and, with its w15 byte 0,
-// a leaf of no triangles, so traversal treats it like any culled leaf and
-// needs no test of its own (its planes are all 0).
-constexpr uint32_t kNoChild = 1u;
+constexpr uint32_t kLeafFlag = 0x80000000u;
+constexpr uint32_t kLeafMaskBits = 31;
+// An empty child: a leaf of no triangles (flag, empty mask), so traversal
+// treats it like any culled leaf and needs no test of its own (its planes
+// are all 0).
+constexpr uint32_t kNoChild = kLeafFlag;
 // false when a node's leaf children do not form one run that fits w15.
 bool quantize_bvh4(const Bvh4& in, std::vector<uint32_t>* out);
 

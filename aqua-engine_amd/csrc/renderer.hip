@@ -1382,9 +1382,10 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   aqua::build_bvh(scene->verts.data(), n, bo, &bvh);
   aqua::Bvh4 bvh4;
   aqua::collapse_bvh4(bvh, &bvh4);
-  // The traversal packs a child reference as (index << 3) | count.
-  if (n >= (1u << 29) || bvh4.nodes.size() >= (size_t(1) << 29) || bo.max_leaf > 7)
-    return aqua::fail(AQUA_ERR_UNSUPPORTED, "scene too large for 29-bit child references");
+  // Inner child references are node indices below the leaf flag (2^31), and
+  // the traversal addresses triangle slot i at 3 i float4s in 32 bits.
+  if (n >= (1u << 29) || bvh4.nodes.size() >= (size_t(1) << 31) || bo.max_leaf > 7)
+    return aqua::fail(AQUA_ERR_UNSUPPORTED, "scene too large: %u triangles (at most 2^29)", n);
   // At most three entries are pushed per visited node.
   if (3 * (bvh4.max_inner_depth + 1) > uint32_t(kLdsStack + kSpillStack))
     return aqua::fail(AQUA_ERR_UNSUPPORTED, "BVH depth %u exceeds the traversal stack",

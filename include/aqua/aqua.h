@@ -432,14 +432,14 @@ int aqua_scene_bvh(const aqua_scene* scene, uint32_t* n_nodes, float* nodes, uin
                    uint32_t* max_depth);
 /* Diagnostics: the same tree as the renderer stores it on the device, 16
  * words (64 bytes) per node: words 0-2 the node origin (float), word 3 the
- * biased exponents e_x | e_y << 8 | e_z << 16 of the per-axis power-of-two
- * scales, words 4-9 the child planes lo.x, hi.x, lo.y, hi.y, lo.z, hi.z as
- * one byte per child (byte k = child k; plane = origin + q * 2^(e - 127),
- * rounded outward), words 10-13 the child references (index << 3) | count
- * (an empty child is 1 with leaf byte 0: a leaf of no triangles), word 14
- * the first triangle slot of the node's leaf
- * children (which take one run of slots), word 15 one byte per child: offset
- * into that run << 3 | triangle count for a leaf child, 0 otherwise. Same
+ * exponents e_x | e_y << 8 | e_z << 16 of the per-axis power-of-two scales as
+ * signed bytes, words 4-9 the child planes lo.x, hi.x, lo.y, hi.y, lo.z, hi.z
+ * as one byte per child (byte k = child k; plane = origin + q * 2^e, rounded
+ * outward), words 10-13 the child references (an inner child: its node
+ * index, below 2^31; a leaf child: 0x80000000 | the mask of its triangle
+ * slots over the node's leaf run, bit i = slot word14 + i; an empty child:
+ * 0x80000000, a leaf of no triangles), word 14 the first triangle slot of
+ * the node's leaf children (which take one run of slots), word 15 zero. Same
  * calling pattern as above. */
 int aqua_scene_bvh_device(const aqua_scene* scene, uint32_t* n_nodes, uint32_t* words);
 

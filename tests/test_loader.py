@@ -374,12 +374,18 @@ def test_scene_bvh_invariants(box_room, tmp_path):
 def _decode_device_nodes(words):
     """Float64 decode of the 64-byte device nodes: (lo[n,4,3], hi[n,4,3], ref[n,4])."""
     origin = words[:, 0:3].view(np.float32).astype(np.float64)
-    e = np.stack([(words[:, 3] >> (8 * a)) & 0xFF for a in range(3)], 1).astype(np.int64) - 127
+    e = _device_exponents(words)
     scale = np.ldexp(1.0, e)                                                     # [n, 3]
     q = np.stack([(words[:, 4:10] >> (8 * k)) & 0xFF for k in range(4)], 1)     # [n, 4, 6]
     lo = origin[:, None, :] + q[:, :, 0::2] * scale[:, None, :]
     hi = origin[:, None, :] + q[:, :, 1::2] * scale[:, None, :]
     return lo, hi, words[:, 10:14]
+
+
+def _device_exponents(words):
+    """Word 3: signed exponent bytes (two's complement), x in bits 0-7."""
+    return np.stack([((words[:, 3] >> (8 * a)) & 0xFF).astype(np.uint8).view(np.int8) for a in range(3)],
+                    1).astype(np.int64)
 
 
 def test_device_nodes_enclose_the_exact_boxes(box_room, tmp_path):
@@ -396,26 +402,33 @@ def test_device_nodes_enclose_the_exact_boxes(box_room, tmp_path):
         idx = nodes[:, 24:28].view(np.int32)
         cnt = nodes[:, 28:32].view(np.int32)
         used = idx >= 0
-        # an empty child is stored as a leaf of no triangles: reference 1 and
-        # leaf byte 0 (a real leaf has a nonzero count in its byte)
-        byte15 = np.stack([(words[:, 15] >> (8 * k)) & 0xFF for k in range(4)], 1)
-        np.testing.assert_array_equal((ref == 1) & (byte15 == 0), ~used)
-        np.testing.assert_array_equal(ref[used] >> 3, idx[used].astype(np.uint32))
-        np.testing.assert_array_equal(ref[used] & 7, cnt[used].astype(np.uint32))
-        # the leaf run: word 14 = first leaf slot, word 15 = per child
-        # (offset << 3) | count for leaves, 0 otherwise
+        flag = np.uint32(0x80000000)
+        # an inner child is its node index; an empty child is a leaf of no
+        # triangles (the leaf flag alone)
+        inner = used & (cnt == 0)
+        np.testing.assert_array_equal(ref[inner], idx[inner].astype(np.uint32))
+        np.testing.assert_array_equal(ref[~used], flag)
+        assert np.all(ref[inner] < flag)
+        # a leaf child is the flag plus its slots as a mask over the node's
+        # leaf run, which starts at word 14 (word 15 is 0)
         leaf = used & (cnt > 0)
         run = np.where(leaf, idx, np.iinfo(np.int32).max).min(1)
         has = leaf.any(1)
         np.testing.assert_array_equal(words[has, 14], run[has].astype(np.uint32))
-        np.testing.assert_array_equal(words[~has, 14:16], 0)
-        byte = np.stack([(words[:, 15] >> (8 * k)) & 0xFF for k in range(4)], 1)
-        want = np.where(leaf, ((idx - run[:, None]) << 3) | cnt, 0)
-        np.testing.assert_array_equal(byte, want.astype(np.uint32))
+        np.testing.assert_array_equal(words[~has, 14], 0)
+        np.testing.assert_array_equal(words[:, 15], 0)
+        off = np.where(leaf, idx - run[:, None], 0).astype(np.int64)
+        want = flag | (((1 << cnt.astype(np.int64)) - 1) << off).astype(np.uint32)
+        np.testing.assert_array_equal(ref[leaf], want[leaf])
+        # the masks of one node's leaves are disjoint and cover its run
+        masks = np.where(leaf, ref & ~flag, 0)
+        np.testing.assert_array_equal(np.bitwise_or.reduce(masks, 1),
+                                      ((1 << np.where(leaf, cnt, 0).sum(1).astype(np.int64)) - 1).astype(np.uint32))
+        assert np.all(np.sum([bin(int(v)).count("1") for v in masks.ravel()]) == np.where(leaf, cnt, 0).sum())
         exact_lo = np.stack([nodes[:, 0:4], nodes[:, 8:12], nodes[:, 16:20]], 2).astype(np.float64)
         exact_hi = np.stack([nodes[:, 4:8], nodes[:, 12:16], nodes[:, 20:24]], 2).astype(np.float64)
         assert np.all(lo[used] <= exact_lo[used]) and np.all(hi[used] >= exact_hi[used])
         # and tight: at most one quantisation step (extent / 255) of slack
-        e = np.stack([(words[:, 3] >> (8 * a)) & 0xFF for a in range(3)], 1).astype(np.int64) - 127
+        e = _device_exponents(words)
         step = np.broadcast_to(np.ldexp(1.0, e)[:, None, :], lo.shape)
         assert np.all((exact_lo - lo)[used] <= step[used]) and np.all((hi - exact_hi)[used] <= step[used])
