@@ -173,6 +173,7 @@ _SIGS = {
     "aqua_nrc_get_params": (C.c_int, [_P, _P, _P]),
     "aqua_nrc_set_params": (C.c_int, [_P, _P, _P]),
     "aqua_nrc_fp8_scales": (C.c_int, [_P, _P, _P]),
+    "aqua_nrc_query_order": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "aqua_nrc_infer": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P]),
     "aqua_nrc_train": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(NrcStats)]),
     "aqua_nrc_train_device": (C.c_int, [_P, C.c_uint32, _P, _P, C.c_uint32, C.POINTER(NrcStats)]),
@@ -560,6 +561,13 @@ class Nrc:
         a = np.zeros(5, np.float32)
         _check(lib().aqua_nrc_fp8_scales(self._h, _ptr(w), _ptr(a)))
         return w, a
+
+    def query_order(self, records) -> np.ndarray:
+        """The order the cache evaluates the records in (stable Morton sort)."""
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, 16)
+        out = np.zeros(len(rec), np.uint32)
+        _check(lib().aqua_nrc_query_order(self._h, len(rec), _ptr(rec), _ptr(out)))
+        return out
 
     def infer(self, records, precision=NRC_F16) -> np.ndarray:
         rec = np.ascontiguousarray(records, np.float32).reshape(-1, 16)

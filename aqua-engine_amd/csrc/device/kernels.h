@@ -101,7 +101,6 @@ struct Queues {
   float4* qrec;    // 4 per query: (pos, roughness), (dir, -), (normal, -), (albedo, -)
   float4* qaux;    // (throughput, path id bits)
   uint32_t* qkey;  // Morton key of each query's position (sort input)
-  uint32_t* qidx;  // its slot (sort input)
   float4* trec;    // 4 per training slot, as qrec
   float4* taux;    // 2 per training slot: (throughput, valid), (radiance after emission, -)
 };

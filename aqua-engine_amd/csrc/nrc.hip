@@ -1116,7 +1116,7 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
   {
     size_t need = 0;
     HIPCHK(aqua::query_sort_storage(n, &need));
-    if (int e = nrc->stemp.alloc(need)) return e;  // rocPRIM's need is not monotone in n
+    if (int e = nrc->stemp.alloc(need)) return e;
     HIPCHK(aqua::query_sort(nrc->stemp.p, nrc->stemp.n, d_rec, nullptr, n, nrc->skey.p, nrc->sidx.p, s,
                             &a.perm));
   }
@@ -1462,6 +1462,24 @@ int aqua_nrc_infer(aqua_nrc* nrc, uint32_t n, const float* records, uint32_t pre
                                      nullptr))
     return e;
   HIPCHK(hipMemcpyAsync(rgb, nrc->out_buf.p, size_t(n) * 12, hipMemcpyDeviceToHost, nrc->stream));
+  HIPCHK(hipStreamSynchronize(nrc->stream));
+  return AQUA_OK;
+}
+
+int aqua_nrc_query_order(aqua_nrc* nrc, uint32_t n, const float* records, uint32_t* order) {
+  if (!nrc || (n && (!records || !order))) return aqua::fail(AQUA_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return AQUA_OK;
+  HIPCHK(hipSetDevice(nrc->device));
+  if (int e = upload_records(nrc, n, records, nullptr)) return e;
+  if (int e = nrc->skey.alloc(2 * size_t(n))) return e;
+  if (int e = nrc->sidx.alloc(2 * size_t(n))) return e;
+  size_t need = 0;
+  HIPCHK(aqua::query_sort_storage(n, &need));
+  if (int e = nrc->stemp.alloc(need)) return e;
+  const uint32_t* perm = nullptr;
+  HIPCHK(aqua::query_sort(nrc->stemp.p, nrc->stemp.n, nrc->rec_buf.p, nullptr, n, nrc->skey.p, nrc->sidx.p,
+                          nrc->stream, &perm));
+  HIPCHK(hipMemcpyAsync(order, perm, size_t(n) * 4, hipMemcpyDeviceToHost, nrc->stream));
   HIPCHK(hipStreamSynchronize(nrc->stream));
   return AQUA_OK;
 }

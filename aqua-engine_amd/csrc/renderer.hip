@@ -362,7 +362,6 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
         q.qrec[4 * size_t(qi) + 3] = qr3;
         q.qaux[qi] = qa;
         q.qkey[qi] = morton30(qr0.x, qr0.y, qr0.z);
-        q.qidx[qi] = qi;
       }
     }
   }
@@ -755,7 +754,7 @@ struct Lane {
   size_t nrc_cap = 0;
   DevBuf<float4> qrec, qaux;
   DevBuf<float> qout;
-  DevBuf<uint32_t> skey, sidx;  // Morton keys, slot order (double buffers)
+  DevBuf<uint32_t> skey, sidx;  // Morton keys, slot order (sort double buffers)
   DevBuf<uint8_t> stemp;
   // material sort: shading order, bin counts (zero between bounces), bin starts
   size_t order_cap = 0;
@@ -945,7 +944,6 @@ struct aqua_renderer {
     q.qrec = l.qrec.p;
     q.qaux = l.qaux.p;
     q.qkey = l.skey.p;
-    q.qidx = l.sidx.p;
     q.trec = trec.p;
     q.taux = taux.p;
     return q;
@@ -1084,8 +1082,8 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
     HIPCHK(aqua::query_sort_storage(n, &need));
     if (2 * size_t(n) > lane.skey.n || 2 * size_t(n) > lane.sidx.n)
       return aqua::fail(AQUA_ERR_HIP, "query sort buffers too small for %u queries", n);
-    if (int e = lane.stemp.ensure(need)) return e;  // rocPRIM's need is not monotone in n
-    // k_shade wrote the queries' keys; the sort fills the padding slots
+    if (int e = lane.stemp.ensure(need)) return e;
+    // k_shade wrote the queries' keys; the sort orders the first *count slots
     HIPCHK(aqua::query_sort(lane.stemp.p, lane.stemp.n, nullptr, q.counters + kCntQuery, n, lane.skey.p,
                             lane.sidx.p, qs, &perm));
     if (int e = aqua::nrc_infer_device(r->nrc, n, lane.qrec.p, lane.qout.p, r->cfg.nrc_precision, qs,

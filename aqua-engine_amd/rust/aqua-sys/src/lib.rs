@@ -244,6 +244,7 @@ pub mod ffi {
         pub fn aqua_nrc_get_params(nrc: *mut aqua_nrc, table: *mut f32, mlp: *mut f32) -> c_int;
         pub fn aqua_nrc_set_params(nrc: *mut aqua_nrc, table: *const f32, mlp: *const f32) -> c_int;
         pub fn aqua_nrc_fp8_scales(nrc: *mut aqua_nrc, wscale: *mut f32, ascale: *mut f32) -> c_int;
+        pub fn aqua_nrc_query_order(nrc: *mut aqua_nrc, n: u32, records: *const f32, order: *mut u32) -> c_int;
         pub fn aqua_nrc_checkpoint(nrc: *mut aqua_nrc, restore: c_int) -> c_int;
         pub fn aqua_nrc_infer(nrc: *mut aqua_nrc, n: u32, records: *const f32, precision: u32, rgb: *mut f32) -> c_int;
         pub fn aqua_nrc_train(nrc: *mut aqua_nrc, n: u32, records: *const f32, targets: *const f32, steps: u32, stats: *mut aqua_nrc_stats) -> c_int;

@@ -367,6 +367,14 @@ int aqua_nrc_set_params(aqua_nrc* nrc, const float* table, const float* mlp);
  * 448 / amax |layer input| seen by the last training forward pass (1 before
  * any training). Diagnostics for checking the FP8 path. */
 int aqua_nrc_fp8_scales(aqua_nrc* nrc, float* wscale, float* ascale);
+/* The order the cache evaluates n records in (aqua_nrc_infer, and the rows of
+ * a training step): order[j] = the record evaluated j-th, the records sorted
+ * stably by the Morton code of their position (10 bits per axis of the
+ * position clamped to [0, 1], x lowest) with its low 6 bits dropped, i.e. by
+ * cells of 1/256 of the unit cube per axis, ties in record order. No
+ * counterpart in the reference (whose renderer source is empty);
+ * diagnostics for checking the device sort. */
+int aqua_nrc_query_order(aqua_nrc* nrc, uint32_t n, const float* records, uint32_t* order);
 /* restore == 0: saves the cache's whole training state on the device
  * (parameters, Adam moments, step count, FP8 activation ranges); restore ==
  * 1: puts the saved state back, so later queries and training steps run

@@ -20,7 +20,11 @@ so this restates the published methods the GPU kernels implement:
 * the relative L2 loss of Mueller, Rousselle, Novak, Keller 2021 ("Real-time
   Neural Radiance Caching for Path Tracing"):
   mean over samples and channels of (y - t)^2 / (sg(y)^2 + 0.01);
-* Adam (Kingma & Ba 2015) with bias correction.
+* Adam (Kingma & Ba 2015) with bias correction;
+* the order the cache evaluates records in: a stable sort by the Morton code
+  (Z-order, x lowest) of the position's 10-bit cells, its low 6 bits
+  dropped. This order has no counterpart in the reference; it is defined here
+  and pinned by a bit-by-bit interleave in the tests.
 
 Feature order of the 64-wide input (the GPU kernels use the same order):
   [0, 32)  hash features, level-major (level l -> 2l, 2l + 1)
@@ -274,6 +278,29 @@ def loss_and_grads(rec, targets, table_np, mlp_np, dtype=torch.float64 if torch 
     loss.backward()
     g_mlp = np.concatenate([W.grad.numpy().reshape(-1) for W in Ws])
     return float(loss.detach()), table.grad.numpy(), g_mlp
+
+
+def morton30(pos):
+    """Morton codes of positions [n, 3]: 10-bit cells per axis
+    (min(floor(clamp(p, 0, 1) * 1024), 1023) in float32), interleaved with x in
+    bit 0, y in bit 1, z in bit 2 of each group of three."""
+    p = np.asarray(pos, np.float32)
+    cell = np.minimum((np.clip(p, 0.0, 1.0).astype(np.float32) * np.float32(1024.0)).astype(np.uint32), 1023)
+    key = np.zeros(len(p), np.uint64)
+    for axis in range(3):
+        v = cell[:, axis].astype(np.uint64)
+        v = (v | (v << 16)) & 0x030000FF
+        v = (v | (v << 8)) & 0x0300F00F
+        v = (v | (v << 4)) & 0x030C30C3
+        v = (v | (v << 2)) & 0x09249249
+        key |= v << axis
+    return key.astype(np.uint32)
+
+
+def query_order(rec):
+    """Record order of a cache pass: stable argsort of morton30 >> 6."""
+    rec = np.asarray(rec, np.float32).reshape(-1, 16)
+    return np.argsort(morton30(rec[:, 0:3]) >> 6, kind="stable").astype(np.uint32)
 
 
 def adam_step(params, grads, m, v, step, lr, beta1=0.9, beta2=0.99, eps=1e-15):

@@ -78,6 +78,29 @@ def test_infer_matches_oracle(gpu, prec):
     assert rel_l2(nrc.infer(rec, other), want) < TOL[other]
 
 
+def test_query_order_matches_oracle(gpu):
+    # the device's stable radix sort of Morton cells, bit for bit: sizes
+    # around the sort's 4096-key tiles, many equal cells, positions outside
+    # [0, 1], and a call after a larger one (reused buffers)
+    nrc, _, _ = make()
+    rng = np.random.default_rng(11)
+    cases = []
+    for n in (1, 7, 2048, 4095, 4096, 4097, 8193, 300_000):
+        cases.append(records(n, n))
+    tied = records(70_000, 3)
+    tied[:, 0:3] = rng.integers(0, 4, (70_000, 3)) / 4.0
+    cases.append(tied)
+    wide = records(9000, 4)
+    wide[:, 0:3] = rng.uniform(-0.5, 1.5, (9000, 3))
+    cases.append(wide)
+    cases.append(records(3000, 5))
+    for rec in cases:
+        want = ref.query_order(rec)
+        got = nrc.query_order(rec)
+        print(f"query order n={len(rec)}", flush=True)
+        np.testing.assert_array_equal(got, want)
+
+
 def test_infer_ragged_sizes(gpu):
     nrc, table, mlp = make()
     rec = records(97, 2)

@@ -232,3 +232,28 @@ def test_fp8_forward_is_exact_when_quantisation_is_exact():
     # a scale that pushes the weights past 448 saturates them: results change
     sat = ref.forward_fp8(X, Ws, [4096.0] * 5, ascale).numpy()
     assert not np.allclose(sat, got, rtol=1e-2)
+
+
+def test_morton_codes_match_a_bit_by_bit_interleave():
+    # the magic-number spread against the definition: bit b of the x / y / z
+    # cell goes to bit 3b / 3b + 1 / 3b + 2
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-0.2, 1.2, (4000, 3)).astype(np.float32)
+    pos[:6] = [[0, 0, 0], [1, 1, 1], [1 / 1024, 0, 0], [0, 1 / 1024, 0], [0, 0, 1 / 1024], [0.5, 0.25, 0.125]]
+    got = ref.morton30(pos)
+    cells = [[min(int(np.float32(min(max(c, 0.0), 1.0)) * np.float32(1024.0)), 1023) for c in p] for p in pos]
+    want = [sum(((c[a] >> b) & 1) << (3 * b + a) for b in range(10) for a in range(3)) for c in cells]
+    np.testing.assert_array_equal(got, np.array(want, np.uint32))
+    assert list(got[:6]) == [0, (1 << 30) - 1, 1, 2, 4, (1 << 27) | (1 << 25) | (1 << 23)]
+
+
+def test_query_order_is_a_stable_sort_of_cells():
+    rng = np.random.default_rng(6)
+    rec = np.zeros((3000, 16), np.float32)
+    rec[:, 0:3] = rng.integers(0, 8, (3000, 3)) / 8.0 + 1e-4  # many equal cells
+    order = ref.query_order(rec)
+    assert sorted(order.tolist()) == list(range(3000))
+    key = ref.morton30(rec[order, 0:3]) >> 6
+    assert np.all(np.diff(key.astype(np.int64)) >= 0)
+    same = np.diff(key.astype(np.int64)) == 0
+    assert np.all(np.diff(order.astype(np.int64))[same] > 0)  # ties keep record order
