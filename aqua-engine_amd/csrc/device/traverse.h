@@ -148,7 +148,7 @@ struct RayIO {
 };
 
 template <bool COUNT, class IO>
-__global__ void __launch_bounds__(kStackThreads, 6) k_intersect(DevBvh bvh, IO io, const uint32_t* count, uint32_t* cursor,
+__global__ void __launch_bounds__(kStackThreads, 7) k_intersect(DevBvh bvh, IO io, const uint32_t* count, uint32_t* cursor,
                                                       unsigned long long* stats) {
   __shared__ float4 lds_top[4 * kTopNodes];
   stage_top(bvh, lds_top);
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(kStackThreads, 6) k_intersect(DevBvh bvh, IO i
 // else writes that radiance meanwhile, so the sum is formed when the ray is
 // taken (its loads overlap the traversal) and only stored when it ends.
 template <bool COUNT>
-__global__ void __launch_bounds__(kStackThreads, 6) k_shadow(DevBvh bvh, Queues q, uint32_t b) {
+__global__ void __launch_bounds__(kStackThreads, 7) k_shadow(DevBvh bvh, Queues q, uint32_t b) {
   __shared__ float4 lds_top[4 * kTopNodes];
   stage_top(bvh, lds_top);
   __shared__ uint32_t lds_stack[kLdsStack * kStackThreads];
