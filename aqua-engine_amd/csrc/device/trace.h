@@ -452,18 +452,4 @@ AQ_DEV void any_wave_step(const DevBvh& bvh, TStack& st, AnyRay& a, bool active,
   }
 }
 
-// Any hit in (0, tmax), one ray per thread.
-template <bool COUNT>
-AQ_DEV bool trace_any(const DevBvh& bvh, TStack& st, v3 o, v3 d, float tmax, TraceCount* cnt) {
-  AnyRay a;
-  any_begin(a, o, d, tmax);
-  while (!any_done(a)) {
-    if (a.lmask != 0u)
-      any_leaf<COUNT>(bvh, st, a, cnt);
-    else
-      any_node<COUNT>(bvh, st, a, cnt);
-  }
-  return a.occluded;
-}
-
 }  // namespace aqd

@@ -1,7 +1,7 @@
 // Persistent traversal kernels: the work queue that feeds their waves
 // (WaveWork, LaneFeed), closest-hit traversal of a ray queue (k_intersect;
-// path queues or caller rays) and any-hit traversal of a shadow-ray queue
-// (k_shadow). Included by renderer.hip only; kept apart from the shading
+// path queues or caller rays) and any-hit traversal (k_shadow; shadow-ray
+// queues or caller rays). Included by renderer.hip only; kept apart from the shading
 // kernels so that the sources of the traversal kernels' instruction stream
 // are these device headers alone.
 #pragma once
@@ -191,37 +191,72 @@ __global__ void __launch_bounds__(kStackThreads, 7) k_intersect(DevBvh bvh, IO i
   }
 }
 
-// Shadow rays of bounce b: an unoccluded ray adds its contribution to the
-// radiance of its path (the next bounce's slot, or the pass buffer for a path
-// that ended). Each path has at most one shadow ray per bounce and nothing
-// else writes that radiance meanwhile, so the sum is formed when the ray is
-// taken (its loads overlap the traversal) and only stored when it ends.
-template <bool COUNT>
-__global__ void __launch_bounds__(kStackThreads, 7) k_shadow(DevBvh bvh, Queues q, uint32_t b) {
+// Where the any-hit kernel reads its rays and writes its answers, with the
+// state a lane keeps for its ray meanwhile. Shadow queues: an unoccluded ray
+// adds its contribution to the radiance of its path (the next bounce's slot,
+// or the pass buffer for a path that ended). Each path has at most one shadow
+// ray per bounce and nothing else writes that radiance meanwhile, so the sum
+// is formed when the ray is taken (its loads overlap the traversal) and only
+// stored when it ends.
+struct ShadowIO {
+  Queues q;
+  uint32_t nx;  // radiance buffer of the next bounce's paths
+  struct Lane {
+    float4* Lp;
+    float4 sum;
+  };
+  __device__ __forceinline__ void load(uint32_t r, float4* o, float4* d, Lane* l) const {
+    *o = q.sho[r];
+    *d = q.shd[r];
+    const float4 cc = q.shc[r];
+    const uint32_t dst = __float_as_uint(d->w);
+    l->Lp = (dst & 0x80000000u) ? &q.Lbuf[dst & 0x7FFFFFFFu] : &q.L[nx][dst];
+    const float4 L = *l->Lp;
+    l->sum = make_float4(L.x + cc.x, L.y + cc.y, L.z + cc.z, 0.0f);
+  }
+  __device__ __forceinline__ void store(uint32_t, const Lane& l, bool occluded) const {
+    if (!occluded) *l.Lp = l.sum;
+  }
+};
+// Caller rays (origin and tmax, direction): 1 where the open segment
+// (0, tmax) hits a triangle, else 0 -- what aqua_trace_occluded returns.
+struct OcclusionIO {
+  const float4* __restrict__ ro;
+  const float4* __restrict__ rd;
+  uint8_t* occluded;
+  struct Lane {};
+  __device__ __forceinline__ void load(uint32_t r, float4* o, float4* d, Lane*) const {
+    *o = ro[r];
+    *d = rd[r];
+  }
+  __device__ __forceinline__ void store(uint32_t r, const Lane&, bool occ) const { occluded[r] = occ ? 1u : 0u; }
+};
+
+// Any-hit traversal of the rays [0, *count) that IO supplies, persistent like
+// k_intersect (cursor: the launch's 8 range cursors, zeroed).
+template <bool COUNT, class IO>
+__global__ void __launch_bounds__(kStackThreads, 7) k_shadow(DevBvh bvh, IO io, const uint32_t* count,
+                                                             uint32_t* cursor, unsigned long long* stats) {
   __shared__ float4 lds_top[4 * kTopNodes];
   stage_top(bvh, lds_top);
   __shared__ uint32_t lds_stack[kLdsStack * kStackThreads];
   TStack st;
   st_init(st, lds_stack, bvh.spill);
-  const uint32_t nx = (b & 1u) ^ 1u;
-  const uint32_t n = q.counters[cnt_shadow(b)];
+  const uint32_t n = *count;
   TraceCount tc{0u, 0u, 0u};
-  LaneFeed feed = feed_init(n, q.cursors + cursor_set(b, 1));
+  LaneFeed feed = feed_init(n, cursor);
   AnyRay ar;
+  typename IO::Lane ls{};
   bool active = false;
-  float4* Lp = nullptr;
-  float4 lsum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  uint32_t ray = 0;
   while (true) {
     if (feed.more) {
       uint32_t r = feed_take(feed, !active);
       if (r != kMiss) {
-        float4 o = q.sho[r], d = q.shd[r];
-        float4 cc = q.shc[r];
+        float4 o, d;
+        io.load(r, &o, &d, &ls);
         any_begin(ar, xyz(o), xyz(d), o.w);
-        const uint32_t dst = __float_as_uint(d.w);
-        Lp = (dst & 0x80000000u) ? &q.Lbuf[dst & 0x7FFFFFFFu] : &q.L[nx][dst];
-        const float4 l = *Lp;
-        lsum = make_float4(l.x + cc.x, l.y + cc.y, l.z + cc.z, 0.0f);
+        ray = r;
         active = true;
       }
     }
@@ -231,16 +266,16 @@ __global__ void __launch_bounds__(kStackThreads, 7) k_shadow(DevBvh bvh, Queues 
     while (live > keep) {
       any_wave_step<COUNT>(bvh, st, ar, active, &tc);
       if (active && any_done(ar)) {
-        if (!ar.occluded) *Lp = lsum;
+        io.store(ray, ls, ar.occluded);
         active = false;
       }
       live = uint32_t(__popcll(__ballot(active)));
     }
   }
   if (COUNT) {
-    atomicAdd(&q.stats[kStShNodes], (unsigned long long)tc.nodes);
-    atomicAdd(&q.stats[kStShTris], (unsigned long long)tc.tris);
-    atomicAdd(&q.stats[kStSpill], (unsigned long long)tc.deep);
+    atomicAdd(&stats[kStShNodes], (unsigned long long)tc.nodes);
+    atomicAdd(&stats[kStShTris], (unsigned long long)tc.tris);
+    atomicAdd(&stats[kStSpill], (unsigned long long)tc.deep);
   }
 }
 

@@ -552,20 +552,6 @@ __global__ void k_hits_out(const uint2* hit, const float4* tris, uint32_t n, uin
   ts[i] = __uint_as_float(h.y);
 }
 
-__global__ void __launch_bounds__(kBlock) k_occluded(DevBvh bvh, const float4* ro, const float4* rd,
-                                                     uint32_t n, uint8_t* out) {
-  __shared__ float4 lds_top[4 * kTopNodes];
-  stage_top(bvh, lds_top);
-  __shared__ uint32_t lds_stack[kLdsStack * kBlock];
-  TStack st;
-  st_init(st, lds_stack, bvh.spill);
-  TraceCount tc;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float4 o = ro[i], d = rd[i];
-    out[i] = trace_any<false>(bvh, st, xyz(o), xyz(d), o.w, &tc) ? 1 : 0;
-  }
-}
-
 // relMSE / relRMSE partial sums in FP64, one triple per block.
 __global__ void k_err_partial(const float4* film, float inv_spp, const double* ref, double inv_ref_spp,
                               uint32_t npix, double* partial) {
@@ -1138,9 +1124,11 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
         ss = s2;
       }
       if (count)
-        k_shadow<true><<<gp, kBlock, 0, ss>>>(shadow_bvh, q, b);
+        k_shadow<true><<<gp, kBlock, 0, ss>>>(shadow_bvh, ShadowIO{q, (b & 1u) ^ 1u}, q.counters + cnt_shadow(b),
+                                              q.cursors + cursor_set(b, 1), q.stats);
       else
-        k_shadow<false><<<gp, kBlock, 0, ss>>>(shadow_bvh, q, b);
+        k_shadow<false><<<gp, kBlock, 0, ss>>>(shadow_bvh, ShadowIO{q, (b & 1u) ^ 1u}, q.counters + cnt_shadow(b),
+                                               q.cursors + cursor_set(b, 1), q.stats);
       if (shadow_on_s2(b)) HIPCHK(hipEventRecord(lane.join_ev[b], s2));
     }
     if (timing) HIPCHK(hipEventRecord(ev[3], s));
@@ -1356,7 +1344,7 @@ int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg
   {
     int a = 0, b = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_intersect<false, QueueIO>, kBlock, 0));
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kBlock, 0));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false, ShadowIO>, kBlock, 0));
     int per_cu = std::max(1, std::min(a, b));
     r->persistent_blocks = std::max(8, (std::max(1, cus) * per_cu) & ~7);
   }
@@ -1867,11 +1855,14 @@ int aqua_trace_occluded(aqua_renderer* r, uint32_t n, const float* rays, uint8_t
   if (n == 0) return AQUA_OK;
   HIPCHK(hipSetDevice(r->device));
   if (int e = upload_rays(r, n, rays)) return e;
+  Queues q = r->queues();
   DevScene sc = r->dev_scene();
   DevBuf<uint8_t> out;
   if (int e = out.ensure(n)) return e;
   hipStream_t s = r->stream;
-  k_occluded<<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, r->lane[0].ro[0].p, r->lane[0].rd[0].p, n, out.p);
+  k_reset<<<8, kBlock, 0, s>>>(q.counters, n, q.cursors, kCursorSet);
+  k_shadow<false><<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, OcclusionIO{q.ro[0], q.rd[0], out.p},
+                                                     q.counters + cnt_rays(0), q.cursors, nullptr);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(occluded, out.p, n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
