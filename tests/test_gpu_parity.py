@@ -110,6 +110,30 @@ def test_trace_closest_random_rays_vs_brute_force(gpu, box_room, box_room_oracle
     np.testing.assert_array_equal(occ_g, occ_c)
 
 
+def test_trace_occluded_sizes_and_mixed_segments(gpu, box_room, box_room_oracle):
+    # the persistent any-hit kernel on ray counts around its 64-ray chunks,
+    # half of the segments ending just past their closest hit (occluded), half
+    # just short of it (clear)
+    rng = np.random.default_rng(8)
+    r = aqua.Renderer(box_room, cfg())
+    for n in (1, 63, 65, 4097):
+        o = rng.uniform([-0.99, 0.01, -0.99], [0.99, 1.99, 0.99], (n, 3))
+        d = rng.standard_normal((n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, 0:3] = o
+        rays[:, 3] = 3.0e38
+        rays[:, 4:7] = d
+        ids_b, t_b = box_room_oracle.trace_closest(rays, brute=True)
+        past = rng.random(n) < 0.5
+        seg = rays.copy()
+        seg[:, 3] = np.where(ids_b != aqua.MISS, t_b * np.where(past, np.float32(1.001), np.float32(0.999)),
+                             np.float32(3.0e38))
+        occ_g = r.trace_occluded(seg)
+        np.testing.assert_array_equal(occ_g, box_room_oracle.trace_occluded(seg, brute=True))
+        np.testing.assert_array_equal(occ_g, ((ids_b != aqua.MISS) & past).astype(np.uint8))
+
+
 def _render_match(scene, orc_scene, cam, spp, max_depth, seed=0, frame=0):
     r = aqua.Renderer(scene, cfg(spp, max_depth, seed))
     st = r.render(spp, frame=frame, camera=cam)
