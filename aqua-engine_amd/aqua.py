@@ -116,7 +116,7 @@ class NrcCfg(C.Structure):
 class NrcStats(C.Structure):
     _fields_ = [("ms_total", C.c_double), ("ms_fwd_bwd", C.c_double), ("ms_wgrad", C.c_double),
                 ("ms_adam", C.c_double), ("loss", C.c_float), ("steps", C.c_uint32),
-                ("rows", C.c_uint32), ("reserved", C.c_uint32 * 5)]
+                ("rows", C.c_uint32), ("ms_scatter", C.c_float), ("reserved", C.c_uint32 * 4)]
 
 
 assert C.sizeof(NrcCfg) == 64 and C.sizeof(NrcStats) == 64

@@ -1122,7 +1122,7 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
   }
   uint32_t npad = a.npad;
   uint32_t nchunks = n_chunks(npad);
-  double t_fb = 0, t_wg = 0, t_ad = 0;
+  double t_fb = 0, t_wg = 0, t_ad = 0, t_sc = 0;
   float last_loss = 0.0f;
   HIPCHK(hipEventRecord(nrc->ev[0], s));
   for (uint32_t it = 0; it < steps; ++it) {
@@ -1158,6 +1158,7 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
     if (before_update && it == 0) HIPCHK(hipStreamWaitEvent(s2, before_update, 0));
     k_nrc_adam_mlp<<<kMlp / 64, 256, 0, s2>>>(nrc->partial.p, nchunks, nrc->mmaster.p, nrc->mm.p, nrc->mv.p, A);
     HIPCHK(hipEventRecord(nrc->ev[kEvJoin], s2));
+    HIPCHK(hipEventRecord(nrc->ev[3], s));
     nrc->launch_scatter(a, s);
     HIPCHK(hipEventRecord(nrc->ev[2], s));
     if (before_update && it == 0) HIPCHK(hipStreamWaitEvent(s, before_update, 0));
@@ -1173,6 +1174,8 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
       float ms;
       HIPCHK(hipEventElapsedTime(&ms, nrc->ev[1], nrc->ev[2]));
       t_fb += ms;
+      HIPCHK(hipEventElapsedTime(&ms, nrc->ev[3], nrc->ev[2]));
+      t_sc += ms;
       HIPCHK(hipEventElapsedTime(&ms, nrc->ev[6], nrc->ev[7]));
       t_wg += ms;
       HIPCHK(hipEventElapsedTime(&ms, nrc->ev[2], nrc->ev[4]));
@@ -1190,6 +1193,7 @@ int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float
     stats->ms_fwd_bwd = t_fb;
     stats->ms_wgrad = t_wg;
     stats->ms_adam = t_ad;
+    stats->ms_scatter = float(t_sc);
     stats->loss = last_loss / float(3.0 * double(n));
     stats->steps = steps;
     stats->rows = n;

@@ -194,7 +194,8 @@ pub mod ffi {
         pub loss: f32,
         pub steps: u32,
         pub rows: u32,
-        pub reserved: [u32; 5],
+        pub ms_scatter: f32,
+        pub reserved: [u32; 4],
     }
 
     extern "C" {

@@ -61,6 +61,10 @@ TRAVERSAL_SOURCES = ("aqua-engine_amd/csrc/device/traverse.h", "aqua-engine_amd/
 VALU_PROFILE = "profiles/k_intersect_interior1m_1080p_64spp_valu.json"
 TRAFFIC_PROFILE = "profiles/k_intersect_interior1m_1080p_64spp_traffic.json"
 SHADOW_VALU_PROFILE = "profiles/k_shadow_interior1m_1080p_64spp_valu.json"
+# the hash-grid gradient scatter's atomic requests per training step (PMC),
+# valid while the cache sources are the ones it was counted on
+CACHE_SOURCES = ("aqua-engine_amd/csrc/nrc.hip",)
+SCATTER_PROFILE = "profiles/k_nrc_scatter_boxroom_1080p_2e18rows_atomics.json"
 
 
 def _load(name, path):
@@ -71,23 +75,67 @@ def _load(name, path):
     return mod
 
 
-def traversal_digest():
+def _digest(sources):
     h = hashlib.sha256()
-    for rel in TRAVERSAL_SOURCES:
+    for rel in sources:
         with open(os.path.join(ROOT, rel), "rb") as f:
             h.update(f.read())
     return h.hexdigest()[:16]
 
 
-def _profile(rel):
+def traversal_digest():
+    return _digest(TRAVERSAL_SOURCES)
+
+
+def cache_digest():
+    return _digest(CACHE_SOURCES)
+
+
+def _profile(rel, digest=traversal_digest):
     """A committed counter summary, or None when absent or measured on other
-    traversal sources than these."""
+    sources than these (the traversal sources unless `digest` says otherwise)."""
     try:
         with open(os.path.join(ROOT, rel)) as f:
             p = json.load(f)
     except (OSError, ValueError):
         return None
-    return p if p.get("source_digest") == traversal_digest() else None
+    return p if p.get("source_digest") == digest() else None
+
+
+def atomic_ceiling_here(n=1 << 24, table=1 << 23, reps=10):
+    """G float atomics per second with every lane on its own random word of a
+    32 MiB table (torch index_add_: one global_atomic_add_f32 per element, so
+    one 64-byte segment per lane): the rate the hash-grid scatter is held to.
+    Its streamed reads (index 8 B + value 4 B per atomic) are ~1/20 of the
+    time at HBM speed. Needs a process in which torch is the first to use the
+    GPU (torch carries its own HIP runtime, which finds no device once the
+    library's runtime holds it): see atomic_ceiling."""
+    import torch
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    t = torch.zeros(table, dtype=torch.float32, device=dev)
+    idx = torch.randint(0, table, (n,), device=dev, generator=g)
+    src = torch.ones(n, dtype=torch.float32, device=dev)
+    for _ in range(2):
+        t.index_add_(0, idx, src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        t.index_add_(0, idx, src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return n / (ms * 1e-3) / 1e9
+
+
+def atomic_ceiling():
+    """atomic_ceiling_here in a child process (bench.py --atomic-ceiling)."""
+    import subprocess
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--atomic-ceiling"], capture_output=True,
+                         text=True, timeout=300, check=True)
+    return float(out.stdout.strip().splitlines()[-1])
 
 
 def _cpu_model():
@@ -212,7 +260,7 @@ def material_sort_leg(aqua, spp=16, steps=5, width=1920, height=1080):
     return out
 
 
-def nrc_train_leg(aqua, rows=1 << 18, steps=20, precision=0):
+def nrc_train_leg(aqua, rows=1 << 18, steps=20, precision=0, ceiling=None):
     """Cache training step alone on samples the path tracer produced: box room,
     1920x1080, training suffix paths only, then `steps` Adam steps on 2^18 of
     those samples resident in HBM."""
@@ -257,13 +305,36 @@ def nrc_train_leg(aqua, rows=1 << 18, steps=20, precision=0):
         "ms_fwd_bwd": round(st.ms_fwd_bwd / steps, 4),
         "ms_wgrad": round(st.ms_wgrad / steps, 4),
         "ms_adam": round(st.ms_adam / steps, 4),
+        "ms_scatter": round(st.ms_scatter / steps, 4),
         "loss": round(float(st.loss), 5),
         "roofline_hbm": {"achieved": round(bytes_step / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "bytes_per_step": bytes_step,
                          "note": "not the bound: the hash-grid scatter's atomics are (one 64-byte segment "
                                  "per instruction and segment touched)"},
         "mfma_tflops": round(flops_step / (ms * 1e-3) / 1e12, 3),
+        "roofline_atomics": scatter_roofline(st.ms_scatter / steps, n_use, ceiling),
     }
+
+
+def scatter_roofline(ms_scatter, rows, ceiling):
+    """The scatter against the atomic rate: its 64-byte atomic segments per
+    step (L1-to-L2 atomic requests, counted once with rocprofv3 on this batch)
+    over its HIP-event time, against the rate measured live for one lane per
+    random word (atomic_ceiling)."""
+    if ceiling is None:
+        return {"note": "atomic rate not measured in this run"}
+    out = {"unit": "G 64-byte atomic segments/s", "peak": round(ceiling, 2),
+           "peak_source": "measured here: torch index_add_ of 2^24 ones into a 32 MiB FP32 table at random "
+                          "indices (one atomic per lane and segment)"}
+    p = _profile(SCATTER_PROFILE, cache_digest)
+    if p is None or p.get("rows") != rows:
+        out["note"] = "no counted segments for these cache sources or this batch"
+        return out
+    seg = p["atomic_requests_per_step"]
+    out.update({"segments_per_step": seg, "segments_source": SCATTER_PROFILE,
+                "achieved": round(seg / (ms_scatter * 1e-3) / 1e9, 2)})
+    out["frac"] = round(out["achieved"] / ceiling, 3)
+    return out
 
 
 def _ladder(render, oracle, max_spp, before=None):
@@ -515,10 +586,18 @@ def main():
                     help="the extra legs to run, a comma list of box, sort, infer, train, eval, large")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (the reference kernel for HBM counters)")
+    ap.add_argument("--atomic-ceiling", action="store_true",
+                    help="print the atomic rate of one lane per random word (G/s) and exit")
+    ap.add_argument("--no-atomic-ceiling", action="store_true",
+                    help="do not measure that rate for the training leg (runs under a profiler, whose "
+                         "preloaded library has the GPU before the child process could start)")
     ap.add_argument("--profile-frame", action="store_true",
                     help="render frame 0 of the interior workload once (kernels overlapped, as in the timed "
                          "frames), print its ray counts as JSON and exit: the run counter profiles use")
     args = ap.parse_args()
+    if args.atomic_ceiling:
+        print(atomic_ceiling_here(), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the path tracer would shard by interleaved pixel tiles, one process per GPU)", file=sys.stderr)
@@ -535,14 +614,19 @@ def main():
         torch.cuda.synchronize()
         del x, y
 
+    legs = set(args.legs.split(","))
+    if args.no_large:
+        legs.discard("large")
+    # the atomic rate for the training leg's scatter, measured in a child
+    # process before this one touches the GPU
+    ceiling = None
+    if "train" in legs and not args.no_nrc and not args.profile_frame and not args.no_atomic_ceiling:
+        ceiling = atomic_ceiling()
+
     aqua = _load("aqua_amd", os.path.join(ROOT, "aqua-engine_amd", "aqua.py"))
     if aqua.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
-
-    legs = set(args.legs.split(","))
-    if args.no_large:
-        legs.discard("large")
 
     def extra_legs(out):
         if "box" in legs:
@@ -556,7 +640,7 @@ def main():
             out["nrc_infer"] = nrc_infer_leg(aqua)
         if "train" in legs:
             print("bench: cache training leg", file=sys.stderr, flush=True)
-            out["nrc_train"] = nrc_train_leg(aqua)
+            out["nrc_train"] = nrc_train_leg(aqua, ceiling=ceiling)
         if "eval" in legs:
             print("bench: cache evaluation leg", file=sys.stderr, flush=True)
             out["nrc_eval"] = nrc_eval_leg(aqua)

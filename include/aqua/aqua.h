@@ -348,7 +348,8 @@ typedef struct aqua_nrc_stats {
   float loss;               /* relative L2 loss of the last step            */
   uint32_t steps;
   uint32_t rows;
-  uint32_t reserved[5];
+  float ms_scatter;         /* the scatter alone (part of ms_fwd_bwd)       */
+  uint32_t reserved[4];
 } aqua_nrc_stats;
 
 void aqua_nrc_default_cfg(aqua_nrc_cfg* out);

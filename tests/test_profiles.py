@@ -34,6 +34,14 @@ def test_traversal_profiles_match_the_sources(bench):
         assert bench._profile(rel) is not None
 
 
+def test_scatter_profile_matches_the_cache_sources(bench):
+    with open(os.path.join(ROOT, bench.SCATTER_PROFILE)) as f:
+        p = json.load(f)
+    assert p["source_digest"] == bench.cache_digest(), "the scatter was counted on other cache sources: count again"
+    assert p["rows"] == 1 << 18 and p["atomic_requests_per_step"] > 0
+    assert bench._profile(bench.SCATTER_PROFILE, bench.cache_digest) is not None
+
+
 def test_profile_issue_rates_are_below_peak(bench):
     # a fraction above 1 would mean the peak (one wave64 VALU instruction per
     # SIMD every 2 clocks) is not the bound
