@@ -131,11 +131,16 @@ def atomic_ceiling_here(n=1 << 24, table=1 << 23, reps=10):
 
 
 def atomic_ceiling():
-    """atomic_ceiling_here in a child process (bench.py --atomic-ceiling)."""
+    """atomic_ceiling_here in a child process (bench.py --atomic-ceiling), or
+    None when that process fails (no GPU for torch, say)."""
     import subprocess
-    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--atomic-ceiling"], capture_output=True,
-                         text=True, timeout=300, check=True)
-    return float(out.stdout.strip().splitlines()[-1])
+    try:
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--atomic-ceiling"], capture_output=True,
+                             text=True, timeout=300, check=True)
+        return float(out.stdout.strip().splitlines()[-1])
+    except (subprocess.SubprocessError, ValueError, IndexError) as e:
+        print(f"bench: atomic rate not measured ({type(e).__name__})", file=sys.stderr, flush=True)
+        return None
 
 
 def _cpu_model():
