@@ -1295,7 +1295,10 @@ int nrc_pass_params(aqua_renderer* r, uint32_t n, uint32_t frame, uint32_t sampl
   uint32_t offset = uint32_t((uint64_t(frame) * 2654435761ull + uint64_t(sample0) * 40503ull) % stride);
   uint32_t n_train = sample0 == 0 ? std::min(want, (n - offset + stride - 1) / stride) : 0u;
   pp->nrc = 1;
-  pp->query_depth = r->cfg.nrc_query_vertex >= 1 ? r->cfg.nrc_query_vertex - 1 : 0;
+  // visualize_cache shows the cache itself: paths end in a query at their
+  // first hit, so the image is the emission there plus the cache's
+  // prediction (training goes on as configured)
+  pp->query_depth = r->cfg.visualize_cache ? 0u : (r->cfg.nrc_query_vertex >= 1 ? r->cfg.nrc_query_vertex - 1 : 0u);
   pp->train_stride = stride;
   pp->train_offset = offset;
   pp->n_train = n_train;

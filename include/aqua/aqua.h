@@ -196,7 +196,11 @@ typedef struct aqua_integrator_cfg {
   uint32_t batch_size;      /* "batch_size"                    (integrator.json:4) */
   uint32_t training_iters;  /* "training_iters"                (integrator.json:6) */
   float learning_rate;      /* "learning_rate"                 (integrator.json:7) */
-  uint32_t visualize_cache; /* "visualize_cache"               (integrator.json:8) */
+  uint32_t visualize_cache; /* "visualize_cache"               (integrator.json:8):
+                               with "nrc", 1 ends every non-training path in a
+                               cache query at its first hit, so the image is the
+                               emission there plus the cache's prediction; the
+                               cache trains as configured. No effect on "pt".   */
   uint32_t seed;            /* "seed" (extension, default 0): PCG32 initstate     */
   uint32_t rr_depth;        /* "rr_depth" (extension, default 3)                  */
   /* "nrc" only (extensions): */

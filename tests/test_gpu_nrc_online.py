@@ -225,3 +225,21 @@ def test_cache_query_precision_switches_on_one_state(gpu, box_room):
     pt = aqua.Renderer(box_room, aqua.integrator_default())
     with pytest.raises(aqua.AquaError):
         pt.set_nrc_precision(aqua.NRC_F16)
+
+
+def test_visualize_cache_queries_at_the_first_hit(gpu, box_room):
+    # visualize_cache renders the cache itself: the first frame (initial
+    # weights, training after the film) is bit for bit the frame of a cache
+    # queried at vertex 1, and not the default vertex-2 frame
+    cam = small_camera(box_room.camera(), 64, 48)
+    films = []
+    for vis, qv in ((1, 2), (0, 1), (0, 2)):
+        c = nrc_cfg(max_depth=5, query_vertex=qv)
+        c.visualize_cache = vis
+        r = aqua.Renderer(box_room, c)
+        st = r.render(2, frame=0, camera=cam)
+        assert st.nrc_queries > 0
+        films.append(r.film(cam))
+    np.testing.assert_array_equal(films[0], films[1])
+    assert not np.array_equal(films[0], films[2])
+    assert np.all(np.isfinite(films[0]))
