@@ -1221,7 +1221,11 @@ int nrc_train_frame(aqua_renderer* r, uint32_t nt, aqua_frame_stats* st, hipEven
   const bool timing = r->opts.timing != 0;
   r->batch_n = nt;
   if (timing) HIPCHK(hipEventRecord(r->nev[2], s));
-  const uint32_t steps = r->nrc_steps < 128 ? 4u : 1u;
+  uint32_t steps = r->nrc_steps < 128 ? 4u : 1u;
+  if (r->cfg.training_iters > 0)  // the frame takes no step past training_iters
+    steps = uint32_t(std::min<uint64_t>(steps, r->cfg.training_iters > r->nrc_steps
+                                                   ? r->cfg.training_iters - r->nrc_steps : 0u));
+  if (steps == 0) return AQUA_OK;
   uint32_t per = (nt + steps - 1) / steps;
   uint32_t last = steps;
   for (uint32_t k = 0; k < steps && per > 0; ++k) {
@@ -1293,7 +1297,10 @@ int nrc_pass_params(aqua_renderer* r, uint32_t n, uint32_t frame, uint32_t sampl
   uint32_t want = std::max(1u, r->cfg.nrc_train_paths);
   uint32_t stride = std::max(1u, n / want);
   uint32_t offset = uint32_t((uint64_t(frame) * 2654435761ull + uint64_t(sample0) * 40503ull) % stride);
-  uint32_t n_train = sample0 == 0 ? std::min(want, (n - offset + stride - 1) / stride) : 0u;
+  // once the cache has taken training_iters Adam steps (0: no limit) it only
+  // answers queries: no training paths, no more steps
+  const bool trained = r->cfg.training_iters > 0 && r->nrc_steps >= r->cfg.training_iters;
+  uint32_t n_train = sample0 == 0 && !trained ? std::min(want, (n - offset + stride - 1) / stride) : 0u;
   pp->nrc = 1;
   // visualize_cache shows the cache itself: paths end in a query at their
   // first hit, so the image is the emission there plus the cache's

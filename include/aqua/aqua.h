@@ -194,7 +194,10 @@ typedef struct aqua_integrator_cfg {
   uint32_t spp;             /* "spp"                           (integrator.json:3) */
   uint32_t max_depth;       /* "max_depth": bounces            (integrator.json:5) */
   uint32_t batch_size;      /* "batch_size"                    (integrator.json:4) */
-  uint32_t training_iters;  /* "training_iters"                (integrator.json:6) */
+  uint32_t training_iters;  /* "training_iters"                (integrator.json:6):
+                               the online cache's Adam steps in all; after them
+                               frames trace no training paths and only query
+                               (0: no limit)                                       */
   float learning_rate;      /* "learning_rate"                 (integrator.json:7) */
   uint32_t visualize_cache; /* "visualize_cache"               (integrator.json:8):
                                with "nrc", 1 ends every non-training path in a

@@ -243,3 +243,21 @@ def test_visualize_cache_queries_at_the_first_hit(gpu, box_room):
     np.testing.assert_array_equal(films[0], films[1])
     assert not np.array_equal(films[0], films[2])
     assert np.all(np.isfinite(films[0]))
+
+
+def test_training_stops_after_training_iters(gpu, box_room):
+    # training_iters caps the cache's Adam steps: the warm-up takes four per
+    # frame, so a cap of 6 ends training within the second frame (two steps
+    # there); later frames trace no training paths and leave the cache as it
+    # was
+    cam = small_camera(box_room.camera(), 48, 32)
+    c = nrc_cfg(max_depth=5)
+    c.training_iters = 6
+    r = aqua.Renderer(box_room, c)
+    got = [r.render(1, frame=f, camera=cam).nrc_train_records for f in range(4)]
+    assert got[0] > 0 and got[1] > 0 and got[2] == 0 and got[3] == 0, got
+    r.render(1, frame=50, camera=cam)
+    before = r.film(cam)
+    r.render(1, frame=7, camera=cam)
+    r.render(1, frame=50, camera=cam)
+    np.testing.assert_array_equal(r.film(cam), before)
