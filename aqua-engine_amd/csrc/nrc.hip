@@ -89,14 +89,19 @@ __device__ __forceinline__ int kappa(int g, int ks, int i) { return 16 * (2 * ks
 // (the primes of the spatial hash of Mueller et al. 2022) and keep the
 // vertex's place in its block in the low 5 bits, so a block is one aligned
 // 128-byte run of the table: a cell's 8 corners touch 2.3 lines on average
-// instead of 4, and nearby queries share lines.
+// instead of 4, and nearby queries share lines. Within the block x takes bits
+// 0-1, y's low bit bit 2, z bit 3 and y's high bit bit 4, so each 64-byte half
+// is a 4 x 2 x 2 brick: a cell's corners lie in 2.8 such segments on average
+// (3.1 with 4 x 4 x 1 halves), which is what the gradient atomics pay for.
 __device__ __forceinline__ uint32_t grid_index(const Grid& G, int lv, int cx, int cy, int cz) {
   if ((G.dense_mask >> lv) & 1u) {
     uint32_t n = uint32_t(G.res[lv]) + 1u;
     return uint32_t(cx) + n * (uint32_t(cy) + n * uint32_t(cz));
   }
   const uint32_t b = (uint32_t(cx >> 2) * 1u) ^ (uint32_t(cy >> 2) * 2654435761u) ^ (uint32_t(cz >> 1) * 805459861u);
-  return ((b << 5) | (uint32_t(cx) & 3u) | ((uint32_t(cy) & 3u) << 2) | ((uint32_t(cz) & 1u) << 4)) & G.tmask;
+  return ((b << 5) | (uint32_t(cx) & 3u) | ((uint32_t(cy) & 1u) << 2) | ((uint32_t(cz) & 1u) << 3) |
+          ((uint32_t(cy) & 2u) << 3)) &
+         G.tmask;
 }
 
 struct Cell {
@@ -569,7 +574,7 @@ __global__ void __launch_bounds__(TRAIN ? kTrainBlock : kBlock) k_nrc_fwd(FwdArg
 //   summed first and one lane adds.
 // * levels 8..15 (BF16 pairs, one 32-bit atomic per entry): 8 lanes per row,
 //   one corner each, eight rows per instruction (a cell's corners lie in
-//   about 3 segments with the 4 x 4 x 2 block hashing).
+//   about 2.8 segments with the block layout of grid_index).
 __global__ void __launch_bounds__(kBlock) k_nrc_scatter(FwdArgs a) {
   const int lane = int(__lane_id());
   const uint32_t T_ = 1u << a.G.log2T;

@@ -10,9 +10,10 @@ so this restates the published methods the GPU kernels implement:
   N_l = floor(N_min * b^l), trilinear interpolation of the 8 cell corners,
   dense indexing when (N_l + 1)^3 <= T, otherwise the spatial hash with
   its primes applied to 4 x 4 x 2 blocks of vertices, the vertex's place in
-  its block in the low 5 bits:
+  its block in the low 5 bits (x, y's low bit, z, y's high bit: each 64-byte
+  half of a block a 4 x 2 x 2 brick):
   ((((x >> 2) * 1) ^ ((y >> 2) * 2654435761) ^ ((z >> 1) * 805459861)) << 5
-   | (x & 3) | (y & 3) << 2 | (z & 1) << 4) mod T;
+   | (x & 3) | (y & 1) << 2 | (z & 1) << 3 | (y & 2) << 3) mod T;
 * real spherical harmonics up to degree 4 (16 coefficients, l = 0..3) of the
   direction (the standard graphics table, e.g. Sloan 2008);
 * an MLP of 4 hidden ReLU layers of width 64, no biases (a constant-1 input
@@ -71,8 +72,8 @@ def corner_index(l: int, c: np.ndarray, log2_table=LOG2_TABLE) -> np.ndarray:
     x, y, z = c[..., 0], c[..., 1], c[..., 2]
     b = ((x >> np.uint64(2)) * np.uint64(PRIMES[0])) ^ ((y >> np.uint64(2)) * np.uint64(PRIMES[1])) ^ \
         ((z >> np.uint64(1)) * np.uint64(PRIMES[2]))
-    h = ((b & np.uint64(0xFFFFFFFF)) << np.uint64(5)) | (x & np.uint64(3)) | ((y & np.uint64(3)) << np.uint64(2)) | \
-        ((z & np.uint64(1)) << np.uint64(4))
+    h = ((b & np.uint64(0xFFFFFFFF)) << np.uint64(5)) | (x & np.uint64(3)) | ((y & np.uint64(1)) << np.uint64(2)) | \
+        ((z & np.uint64(1)) << np.uint64(3)) | ((y & np.uint64(2)) << np.uint64(3))
     return (h & np.uint64(0xFFFFFFFF) & np.uint64(T - 1)).astype(np.int64)
 
 

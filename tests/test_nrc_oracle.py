@@ -63,14 +63,17 @@ def test_hash_level_indices_in_range():
 
 def test_hash_levels_keep_vertex_blocks_together():
     # hashed levels: the 32 vertices of a 4 x 4 x 2 block take one aligned
-    # 32-entry run, in x-fastest order; different blocks hash apart
+    # 32-entry run, each half (16 entries, 64 bytes) a 4 x 2 x 2 brick: x in
+    # bits 0-1, y's low bit in bit 2, z in bit 3, y's high bit in bit 4;
+    # different blocks hash apart
     l = next(l for l in range(ref.N_LEVELS) if not ref.level_is_dense(l))
     rng = np.random.default_rng(3)
     base = rng.integers(0, ref.level_res(l) // 4, (200, 3)) * np.array([4, 4, 2])
     off = np.stack(np.meshgrid(np.arange(4), np.arange(4), np.arange(2), indexing="ij"), -1).reshape(-1, 3)
     idx = ref.corner_index(l, base[:, None, :] + off[None, :, :])
     assert np.all(idx // 32 == idx[:, :1] // 32)
-    np.testing.assert_array_equal(idx % 32, np.broadcast_to(off[:, 0] + 4 * off[:, 1] + 16 * off[:, 2], idx.shape))
+    np.testing.assert_array_equal(idx % 32, np.broadcast_to(off[:, 0] + 4 * (off[:, 1] & 1) + 8 * off[:, 2] +
+                                                            16 * (off[:, 1] >> 1), idx.shape))
     # the block part is the spatial hash of the block coordinates
     bc = (base // np.array([4, 4, 2])).astype(np.uint64)
     h = (bc[:, 0] * np.uint64(1)) ^ (bc[:, 1] * np.uint64(2654435761)) ^ (bc[:, 2] * np.uint64(805459861))
