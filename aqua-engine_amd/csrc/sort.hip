@@ -262,7 +262,7 @@ hipError_t query_sort_storage(uint32_t n, size_t* bytes) {
 hipError_t query_sort(void* temp, size_t temp_bytes, const float4* rec, const uint32_t* d_count, uint32_t n,
                       uint32_t* keys, uint32_t* idx, hipStream_t s, const uint32_t** perm) {
   size_t need = 0;
-  query_sort_storage(n, &need);
+  if (hipError_t e = query_sort_storage(n, &need); e != hipSuccess) return e;
   if (n == 0 || temp_bytes < need || (!rec && !d_count)) return hipErrorInvalidValue;
   if (rec) {
     const uint32_t blocks = std::min<uint32_t>((n + 255u) / 256u, 8192u);
