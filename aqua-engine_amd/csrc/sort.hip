@@ -46,7 +46,7 @@ constexpr uint32_t kScanChunk = kScanThreads * 16;  // counts per chunk of the s
 
 namespace {
 
-uint32_t sort_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
+uint32_t sort_tiles(uint32_t n) { return uint32_t((uint64_t(n) + kTile - 1) / kTile); }
 
 __device__ __forceinline__ uint32_t real_count(const uint32_t* count, uint32_t n) {
   return count ? min(*count, n) : n;
