@@ -28,6 +28,7 @@ ERR_IO, ERR_PARSE, ERR_MISSING_MESH, ERR_NO_DEVICE = 1, 2, 3, 4
 ERR_OOM, ERR_HIP, ERR_INVALID_ARG, ERR_NONFINITE, ERR_UNSUPPORTED = 5, 6, 7, 8, 9
 
 INTEGRATOR_PT, INTEGRATOR_NRC = 0, 1
+SAMPLER_INDEPENDENT, SAMPLER_STRATIFIED = 0, 1
 PROCGEN_BOX_ROOM, PROCGEN_INTERIOR = 1, 2
 MISS = 0xFFFFFFFF
 
@@ -79,7 +80,7 @@ class IntegratorCfg(C.Structure):
                 ("learning_rate", C.c_float), ("visualize_cache", C.c_uint32),
                 ("seed", C.c_uint32), ("rr_depth", C.c_uint32), ("nrc_precision", C.c_uint32),
                 ("nrc_train_paths", C.c_uint32), ("nrc_query_vertex", C.c_uint32),
-                ("reserved", C.c_uint32 * 4)]
+                ("sampler", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class FrameStats(C.Structure):

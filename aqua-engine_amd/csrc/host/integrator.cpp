@@ -26,6 +26,7 @@ void integrator_default(aqua_integrator_cfg* c) {
   c->nrc_precision = AQUA_NRC_F16;
   c->nrc_train_paths = 1u << 16;
   c->nrc_query_vertex = 2;
+  c->sampler = AQUA_SAMPLER_INDEPENDENT;
 }
 
 int integrator_load(const char* path, aqua_integrator_cfg* out) {
@@ -82,6 +83,15 @@ int integrator_load(const char* path, aqua_integrator_cfg* out) {
       c.nrc_precision = AQUA_NRC_FP8;
     else
       return fail(AQUA_ERR_PARSE, "%s: unknown nrc_precision '%s'", path, v->str.c_str());
+  }
+  if (const Json* v = j.get("sampler")) {
+    if (v->kind != Json::String) return fail(AQUA_ERR_PARSE, "%s: 'sampler' must be a string", path);
+    if (v->str == "independent")
+      c.sampler = AQUA_SAMPLER_INDEPENDENT;
+    else if (v->str == "stratified")
+      c.sampler = AQUA_SAMPLER_STRATIFIED;
+    else
+      return fail(AQUA_ERR_PARSE, "%s: unknown sampler '%s'", path, v->str.c_str());
   }
   if (const Json* v = j.get("visualize_cache")) {
     if (v->kind != Json::Bool) return fail(AQUA_ERR_PARSE, "%s: 'visualize_cache' must be a bool", path);

@@ -16,6 +16,7 @@
 
 #include "aqua/aqua.h"
 #include "device/kernels.h"
+#include "device/sampler.h"
 #include "device/traverse.h"
 #include "host/bvh.h"
 #include "host/internal.h"
@@ -123,7 +124,7 @@ __device__ __forceinline__ v3 camera_dir(const CamParams& c, uint32_t px, uint32
 
 // Camera rays + fresh path state for samples [sample0, sample0 + S) of
 // every pixel; path id = s_local * npix + pixel.
-__global__ void k_camera(PassParams pp, Queues q) {
+__global__ void k_camera(PassParams pp, uint32_t sampler, Queues q) {
   uint32_t n = pp.npix * pp.spp_in_pass;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t s_local = i / pp.npix;
@@ -135,6 +136,7 @@ __global__ void k_camera(PassParams pp, Queues q) {
     float u[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) u[k] = u01(pcg_next(rng));
+    if (sampler) stratified2(pp.seed, pp.frame, pixel, 0u, sample, &u[0], &u[1]);
     v3 d = camera_dir(pp.cam, px, py, u[0], u[1]);
     q.ro[0][i] = make_float4(pp.cam.m[3], pp.cam.m[7], pp.cam.m[11], kFltMax);
     q.rd[0][i] = make_float4(d.x, d.y, d.z, 0.0f);
@@ -161,7 +163,8 @@ __device__ __forceinline__ uint32_t pick_light(const float* cdf, uint32_t n, flo
 // NRC: the cache is on (pp.nrc): query records and training slots; the plain
 // tracer's instance carries none of their registers.
 template <bool NRC>
-__global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, uint32_t b, Queues q) {
+__global__ void __launch_bounds__(kBlock)
+    k_shade(DevScene sc, PassParams pp, uint32_t sampler, uint32_t b, Queues q) {
   const uint32_t c = b & 1u, nx = c ^ 1u;
   const uint32_t n = q.counters[cnt_rays(b)];
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -245,6 +248,12 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene sc, PassParams pp, ui
           float u[8];
 #pragma unroll
           for (int k = 0; k < 8; ++k) u[k] = u01(pcg_next(rng));
+          if (sampler && depth == 0u) {
+            const uint32_t sample = pp.sample0 + path_id / pp.npix;
+            stratified2(pp.seed, pp.frame, pixel, 1u, sample, &u[1], &u[2]);
+            stratified2(pp.seed, pp.frame, pixel, 2u, sample, &u[4], &u[5]);
+            stratified2(pp.seed, pp.frame, pixel, 3u, sample, &u[0], &u[3]);
+          }
           v3 ns = cg < 0.0f ? neg(ng) : ng;
           Bsdf bs = make_bsdf(m);
           // ---- next-event estimation ----
@@ -1048,7 +1057,7 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
   HIPCHK(hipStreamWaitEvent(s, lane.film_done, 0));
   k_reset<<<64, kBlock, 0, s>>>(q.counters, n, q.cursors, cursor_set(pp.max_depth + 1, 0));
   if (n_slots) k_clear_taux<<<int((n_slots + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.taux, n_slots);
-  k_camera<<<g, kBlock, 0, s>>>(pp, q);
+  k_camera<<<g, kBlock, 0, s>>>(pp, r->cfg.sampler, q);
   bool timing = r->opts.timing != 0;
   const bool sort_mats = r->opts.sort_materials != 0;
   if (sort_mats)
@@ -1105,9 +1114,9 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
     }
     if (b > 0 && shadow_on_s2(b - 1)) HIPCHK(hipStreamWaitEvent(s, lane.join_ev[b - 1], 0));
     if (pp.nrc)
-      k_shade<true><<<g, kBlock, 0, s>>>(sc, pp, b, qs);
+      k_shade<true><<<g, kBlock, 0, s>>>(sc, pp, r->cfg.sampler, b, qs);
     else
-      k_shade<false><<<g, kBlock, 0, s>>>(sc, pp, b, qs);
+      k_shade<false><<<g, kBlock, 0, s>>>(sc, pp, r->cfg.sampler, b, qs);
     if (timing) HIPCHK(hipEventRecord(ev[2], s));
     if (qfork && b == pp.query_depth) {
       HIPCHK(hipEventRecord(lane.q_fork, s));
@@ -1340,6 +1349,8 @@ int aqua_device_count(int* count) {
 int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg, int device,
                          aqua_renderer** out) {
   if (!scene || !cfg || !out) return aqua::fail(AQUA_ERR_INVALID_ARG, "aqua_renderer_create: null argument");
+  if (cfg->sampler > AQUA_SAMPLER_STRATIFIED)
+    return aqua::fail(AQUA_ERR_INVALID_ARG, "unknown sampler %u", cfg->sampler);
   *out = nullptr;
   if (cfg->max_depth >= kMaxBounces)
     return aqua::fail(AQUA_ERR_INVALID_ARG, "max_depth %u exceeds %u", cfg->max_depth, kMaxBounces - 1);
@@ -1784,7 +1795,7 @@ int aqua_primary_hits(aqua_renderer* r, const aqua_camera* cam_in, uint32_t fram
   if (int e = ts.ensure(n)) return e;
   hipStream_t s = r->stream;
   k_reset<<<8, kBlock, 0, s>>>(q.counters, n, q.cursors, kCursorSet);
-  k_camera<<<grid_for(n), kBlock, 0, s>>>(pp, q);
+  k_camera<<<grid_for(n), kBlock, 0, s>>>(pp, r->cfg.sampler, q);
   k_intersect<false><<<r->trav_grid(n), kBlock, 0, s>>>(sc.bvh, QueueIO{q.ro[0], q.rd[0], q.hit}, q.counters,
                                                          q.cursors, q.stats);
   k_hits_out<<<int((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.hit, sc.bvh.tris, n, ids.p, ts.p);

@@ -31,6 +31,8 @@ pub mod ffi {
     pub const AQUA_PROCGEN_INTERIOR: u32 = 2;
     pub const AQUA_INTEGRATOR_PT: u32 = 0;
     pub const AQUA_INTEGRATOR_NRC: u32 = 1;
+    pub const AQUA_SAMPLER_INDEPENDENT: u32 = 0;
+    pub const AQUA_SAMPLER_STRATIFIED: u32 = 1;
     pub const AQUA_NRC_F16: u32 = 0;
     pub const AQUA_NRC_BF16: u32 = 1;
     pub const AQUA_NRC_FP8: u32 = 2;
@@ -130,7 +132,8 @@ pub mod ffi {
         pub nrc_precision: u32,
         pub nrc_train_paths: u32,
         pub nrc_query_vertex: u32,
-        pub reserved: [u32; 4],
+        pub sampler: u32,
+        pub reserved: [u32; 3],
     }
 
     #[repr(C)]

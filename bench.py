@@ -22,12 +22,14 @@ on all host cores. Unless --no-nrc, further legs follow:
   nrc_eval       ms/frame to relMSE <= 1e-2 against a 4096-spp FP64 render,
                  plain tracer vs cache (FP16 and BF16, query at vertex 2; FP16
                  at vertex 1 beside it) on 24 frames of the box-room
-                 fly-through at 1080p;
+                 fly-through at 1080p, with independent sample draws and
+                 again with stratified camera / first-hit dimensions;
   large          the 400-mesh / 5.4 M-triangle interior at 3840x2160, 1 spp
                  with an FP8 (e4m3) and an FP16 cache over a 60-frame
                  fly-through, error against a 64-spp FP64 render.
 """
 import argparse
+import gc
 import hashlib
 import importlib.util
 import json
@@ -398,7 +400,7 @@ def nrc_infer_leg(aqua, warm_frames=40, frames=5, spp=4, width=1920, height=1080
 
 
 def nrc_eval_leg(aqua, frames=tuple(range(5, 240, 10)), n_frames=240, oracle_spp=4096, width=1920,
-                 height=1080, max_spp=256):
+                 height=1080, max_spp=256, sampler=0, oracles=None):
     """ms/frame to relMSE <= 1e-2 against an FP64 render of `oracle_spp` samples,
     plain path tracer vs cache, on the box room fly-through: max depth 8, 2^16
     training suffix paths per frame, online training at 1 spp through every
@@ -413,11 +415,17 @@ def nrc_eval_leg(aqua, frames=tuple(range(5, 240, 10)), n_frames=240, oracle_spp
     those earlier frames trained -- and pays for its own training step, as
     in online use. The caches query at path vertex 2 in FP16 and in BF16 (the
     configuration the ratio is quoted for); an FP16 cache querying at vertex
-    1 is reported beside them."""
+    1 is reported beside them.
+
+    `sampler` is the tracer's and the caches' sample generator (independent
+    PCG32 draws, or stratified camera / first-hit dimensions). The FP64
+    renders are filled into `oracles` (frame -> film) on the first call and
+    reused by a later one."""
     with tempfile.TemporaryDirectory() as d:
         scene = aqua.Scene.load(aqua.procgen(aqua.PROCGEN_BOX_ROOM, d, width=width, height=height))
     pt = aqua.integrator_default()
     pt.max_depth = 8
+    pt.sampler = sampler
     ref = aqua.Renderer(scene, pt)
     ref.set_opts(timing=False)  # ms/frame from the frame's own events; kernels overlapped
     caches = {}
@@ -430,11 +438,13 @@ def nrc_eval_leg(aqua, frames=tuple(range(5, 240, 10)), n_frames=240, oracle_spp
         c.nrc_precision = prec
         c.nrc_train_paths = 1 << 16
         c.nrc_query_vertex = qv
+        c.sampler = sampler
         caches[key] = aqua.Renderer(scene, c)
         caches[key].set_opts(timing=False)
     out = []
     next_frame = 0
     t_oracle = 0.0
+    oracles = {} if oracles is None else oracles
     for fe in frames:
         cam = aqua.camera_flythrough(scene, fe, n_frames, width=width, height=height)
         for f in range(next_frame, fe):
@@ -443,10 +453,12 @@ def nrc_eval_leg(aqua, frames=tuple(range(5, 240, 10)), n_frames=240, oracle_spp
                 r.render(1, frame=f, camera=cf)
         next_frame = fe
         print(f"bench: nrc_eval frame {fe}", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        ref.render_oracle(0, oracle_spp, frame=fe, camera=cam)
-        oracle = ref.oracle_film(cam)
-        t_oracle += time.perf_counter() - t0
+        if fe not in oracles:
+            t0 = time.perf_counter()
+            ref.render_oracle(0, oracle_spp, frame=fe, camera=cam)
+            oracles[fe] = ref.oracle_film(cam)
+            t_oracle += time.perf_counter() - t0
+        oracle = oracles[fe]
         row = {"frame": fe}
 
         def pt_render(spp):
@@ -466,8 +478,10 @@ def nrc_eval_leg(aqua, frames=tuple(range(5, 240, 10)), n_frames=240, oracle_spp
     res = {
         "scene": "box room (36 tris), fly-through of %d frames, %dx%d, max depth 8" % (n_frames, width, height),
         "frames_evaluated": list(frames),
+        "sampler": ("stratified camera / first-hit dimensions (shuffled Owen-scrambled Sobol)" if sampler
+                    else "independent PCG32 draws"),
         "oracle_spp": oracle_spp,
-        "oracle_s_per_frame": round(t_oracle / len(frames), 2),
+        "oracle_s_per_frame": round(t_oracle / len(frames), 2) if t_oracle > 0 else "reused",
         "sample_sets": "oracle frame f, plain tracer f + 100000, cache f + 200000; cache state restored per rung",
         "frames": out,
         "pt_ms_to_1e-2": round(pt_ms, 3),
@@ -648,7 +662,12 @@ def main():
             out["nrc_train"] = nrc_train_leg(aqua, ceiling=ceiling)
         if "eval" in legs:
             print("bench: cache evaluation leg", file=sys.stderr, flush=True)
-            out["nrc_eval"] = nrc_eval_leg(aqua)
+            oracles = {}
+            out["nrc_eval"] = nrc_eval_leg(aqua, oracles=oracles)
+            gc.collect()  # the first set of renderers gives its GPU queues back
+            print("bench: cache evaluation leg, stratified sampler", file=sys.stderr, flush=True)
+            out["nrc_eval_stratified"] = nrc_eval_leg(aqua, sampler=aqua.SAMPLER_STRATIFIED, oracles=oracles)
+            del oracles
         if "large" in legs:
             print("bench: large-scene leg", file=sys.stderr, flush=True)
             out["large"] = large_leg(aqua)

@@ -187,6 +187,7 @@ int aqua_procgen_write(const aqua_procgen_params* params, const char* out_dir, c
 /* ---- integrator (scenes/integrator.json:2-8) --------------------------- */
 
 enum aqua_integrator_type { AQUA_INTEGRATOR_PT = 0, AQUA_INTEGRATOR_NRC = 1 };
+enum aqua_sampler { AQUA_SAMPLER_INDEPENDENT = 0, AQUA_SAMPLER_STRATIFIED = 1 };
 
 /* 64 bytes. */
 typedef struct aqua_integrator_cfg {
@@ -213,7 +214,13 @@ typedef struct aqua_integrator_cfg {
                                per frame (default 65536)                             */
   uint32_t nrc_query_vertex;/* "nrc_query_vertex": path vertex (1 = first hit) at
                                which other paths end in a cache query (default 2)    */
-  uint32_t reserved[4];
+  uint32_t sampler;         /* "sampler" (extension): "independent" (0, default): every
+                               draw from the pixel's PCG32 stream; "stratified" (1):
+                               the camera ray's 2 and the first hit's 6 draws from
+                               shuffled Owen-scrambled 2-D Sobol points over the
+                               sample index (Burley 2020), seeded by seed, pixel and
+                               frame; later vertices keep the PCG32 stream          */
+  uint32_t reserved[3];
 } aqua_integrator_cfg;
 
 void aqua_integrator_default(aqua_integrator_cfg* out);

@@ -119,6 +119,71 @@ void orc_pcg32_seq(uint64_t initstate, uint64_t initseq, uint64_t advance, uint3
 }
 
 /* ------------------------------------------------------------------------ */
+/* Stratified sampler (integrator "sampler": "stratified"): the camera ray's  */
+/* two dimensions and the first hit's six come from shuffled, Owen-scrambled  */
+/* 2-D Sobol points over the sample index (Burley 2020, "Practical Hash-based */
+/* Owen Scrambling", JCGT 9(4): nested uniform scramble by the Laine-Karras   */
+/* hash on reversed bits, the index shuffled by the same scramble); the rest  */
+/* stay on the PCG32 stream. Every power-of-two prefix of a pixel's samples   */
+/* is then a (0, m, 2)-net in each pattern.                                   */
+/* ------------------------------------------------------------------------ */
+
+static inline uint32_t rev32(uint32_t x) {
+  x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+  x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
+  x = ((x >> 4) & 0x0F0F0F0Fu) | ((x & 0x0F0F0F0Fu) << 4);
+  x = ((x >> 8) & 0x00FF00FFu) | ((x & 0x00FF00FFu) << 8);
+  return (x >> 16) | (x << 16);
+}
+
+/* Laine & Karras 2011 hash as given by Burley 2020 (listing 2). */
+static inline uint32_t lk_hash(uint32_t x, uint32_t seed) {
+  x += seed;
+  x ^= x * 0x6c50b47cu;
+  x ^= x * 0xb82f1e52u;
+  x ^= x * 0xc7afe638u;
+  x ^= x * 0x8d22f6e6u;
+  return x;
+}
+
+/* Nested uniform (Owen) scramble of the bits of x, most significant first. */
+static inline uint32_t owen32(uint32_t x, uint32_t seed) { return rev32(lk_hash(rev32(x), seed)); }
+
+/* 32-bit integer mix ("lowbias32", C. Wellons, hash-prospector, public domain). */
+static inline uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+/* Sobol's second dimension (Sobol 1967; every direction number m_k = 1): the
+ * k-th column is the previous one xor itself shifted right by one. The first
+ * dimension is the bit reversal of the index (van der Corput). */
+static inline uint32_t sobol_dim1(uint32_t i) {
+  uint32_t r = 0u, v = 0x80000000u;
+  for (; i; i >>= 1, v ^= v >> 1)
+    if (i & 1u) r ^= v;
+  return r;
+}
+
+/* Point `index` of pattern `pair` of pixel `pixel` in frame `frame`. */
+static void strat2(uint32_t seed, uint32_t frame, uint32_t pixel, uint32_t pair, uint32_t index, float* a,
+                   float* b) {
+  const uint32_t h = mix32(mix32(mix32(mix32(seed) ^ pixel) ^ frame) ^ pair);
+  const uint32_t i = owen32(index, h);
+  *a = unit_float(owen32(rev32(i), mix32(h ^ 0xa511e9b3u)));
+  *b = unit_float(owen32(sobol_dim1(i), mix32(h ^ 0x63d83595u)));
+}
+
+void orc_strat2_seq(uint32_t seed, uint32_t frame, uint32_t pixel, uint32_t pair, uint32_t first, uint32_t n,
+                    float* out) {
+  for (uint32_t k = 0; k < n; ++k) strat2(seed, frame, pixel, pair, first + k, &out[2 * k], &out[2 * k + 1]);
+}
+
+/* ------------------------------------------------------------------------ */
 /* sin / cos of 2 pi u: exact quadrant split of 4u, fold to [0, 1/2] with the */
 /* exact complement, Taylor series of sin(pi/2 g) and cos(pi/2 g) in Horner   */
 /* form (coefficients (-1)^k (pi/2)^n / n!).                                  */
@@ -1039,7 +1104,10 @@ typedef struct {
 
 /* One path: pbrt-style unidirectional path tracing with next-event
  * estimation and the power heuristic (Veach 1997, ch. 9), PCG32 stream
- * `pixel` advanced to ((frame << 20) | sample) * dims, 8 draws per vertex.
+ * `pixel` advanced to ((frame << 20) | sample) * dims, 8 draws per vertex
+ * (with the stratified sampler the camera's two and the first hit's six
+ * draws that are used are replaced by strat2 points; the stream still
+ * advances past them).
  * Each vertex: emission (MIS-weighted against the light pdf unless it is the
  * camera ray's hit), then below max_depth: light pick / point / BSDF eval /
  * shadow ray, BSDF sample, Russian roulette from depth rr_depth on with
@@ -1051,6 +1119,7 @@ static V trace_path(const orc_scene* s, const Cam* cam, const orc_cfg* cfg, uint
   pcg_advance(&rng, (((uint64_t)frame << 20) | (uint64_t)sample) * (uint64_t)dims);
   float u[8];
   for (int k = 0; k < 8; ++k) u[k] = unit_float(pcg_next(&rng));
+  if (cfg->sampler) strat2(cfg->seed, frame, pixel, 0u, sample, &u[0], &u[1]);
   uint32_t px = pixel % cam->width, py = pixel / cam->width;
   V o = v(cam->m[3], cam->m[7], cam->m[11]);
   V d = camera_dir(cam, px, py, u[0], u[1]);
@@ -1092,6 +1161,11 @@ static V trace_path(const orc_scene* s, const Cam* cam, const orc_cfg* cfg, uint
     }
     if (depth >= cfg->max_depth) break;
     for (int k = 0; k < 8; ++k) u[k] = unit_float(pcg_next(&rng));
+    if (cfg->sampler && depth == 0u) {
+      strat2(cfg->seed, frame, pixel, 1u, sample, &u[1], &u[2]); /* light point */
+      strat2(cfg->seed, frame, pixel, 2u, sample, &u[4], &u[5]); /* BSDF direction */
+      strat2(cfg->seed, frame, pixel, 3u, sample, &u[0], &u[3]); /* light pick, lobe */
+    }
     V ns = cg < 0.0f ? vneg(ng) : ng;
     Bsdf bs = bsdf_make(m);
     if (s->n_lights > 0u) {
@@ -1325,6 +1399,7 @@ static void* hit_worker(void* arg) {
       pcg_advance(&rng, (((uint64_t)J->frame << 20) | (uint64_t)J->sample) * (uint64_t)dims);
       float u0 = unit_float(pcg_next(&rng));
       float u1 = unit_float(pcg_next(&rng));
+      if (J->cfg->sampler) strat2(J->cfg->seed, J->frame, pixel, 0u, J->sample, &u0, &u1);
       V o = v(cam->m[3], cam->m[7], cam->m[11]);
       V d = camera_dir(cam, x, y, u0, u1);
       float tt;

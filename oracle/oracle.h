@@ -15,6 +15,8 @@
  *
  * How each part is pinned (tests/test_oracle_*.py):
  *   pcg32 / advance   published pcg32-demo vector; advance == n x next
+ *   strat2            (0, m, 2)-net property of every power-of-two prefix;
+ *                     Sobol dimension values from the direction numbers
  *   sincos_2pi        numpy float64 sin/cos, <= 2 ulp
  *   camera            closed forms (centre ray = -Z axis, corners at fov/2)
  *   closest / any hit brute force over all triangles; float64 Moller-Trumbore
@@ -57,12 +59,18 @@ typedef struct orc_cfg {
   uint32_t rr_depth;
   uint32_t seed;
   uint32_t threads;   /* 0 = all online CPUs */
+  uint32_t sampler;   /* 0 = PCG32 only, 1 = stratified camera / first-hit dims */
 } orc_cfg;
 
 typedef struct orc_stats {
   uint64_t camera_rays, extension_rays, shadow_rays, nonfinite;
   uint64_t nodes, tris;  /* closest-hit traversal work on the oracle's own BVH */
 } orc_stats;
+
+/* Points first .. first + n - 1 of the stratified sampler's 2-D pattern
+ * `pair` of one pixel and frame: out[2 k], out[2 k + 1]. */
+void orc_strat2_seq(uint32_t seed, uint32_t frame, uint32_t pixel, uint32_t pair, uint32_t first, uint32_t n,
+                    float* out);
 
 /* Film over pixels [y0, y1) x [0, width): rgb receives the mean radiance
  * (float sum over samples 0..spp-1 in order, times 1/spp). */

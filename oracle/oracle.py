@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 class Cfg(C.Structure):
     _fields_ = [("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("seed", C.c_uint32),
-                ("threads", C.c_uint32)]
+                ("threads", C.c_uint32), ("sampler", C.c_uint32)]
 
 
 class Stats(C.Structure):
@@ -45,6 +45,7 @@ _SIGS = {
     "orc_trace_closest": (C.c_int, [_P, C.c_uint32, _P, C.c_int, _P, _P]),
     "orc_trace_occluded": (C.c_int, [_P, C.c_uint32, _P, C.c_int, _P]),
     "orc_pcg32_seq": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, _P]),
+    "orc_strat2_seq": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "orc_sincos_2pi": (None, [C.c_uint32, _P, _P, _P]),
     "orc_bsdf_eval": (None, [_P, _P, _P, _P, _P]),
     "orc_bsdf_sample": (None, [_P, _P, _P, C.c_float, C.c_float, C.c_float, _P]),
@@ -114,30 +115,31 @@ class Scene:
         lib().orc_scene_bvh_info(self._h, C.byref(n), C.byref(d))
         return n.value, d.value
 
-    def render(self, camera16, spp, max_depth, rr_depth=3, seed=0, frame=0, y0=0, y1=None, threads=0):
+    def render(self, camera16, spp, max_depth, rr_depth=3, seed=0, frame=0, y0=0, y1=None, threads=0, sampler=0):
         cam = f32(camera16)
         w = int(cam[14:15].view(np.uint32)[0])
         h = int(cam[15:16].view(np.uint32)[0])
         y1 = h if y1 is None else y1
         out = np.zeros((y1 - y0, w, 3), np.float32)
         st = Stats()
-        cfg = Cfg(max_depth, rr_depth, seed, threads)
+        cfg = Cfg(max_depth, rr_depth, seed, threads, sampler)
         rc = lib().orc_render(self._h, _p(cam), C.byref(cfg), frame, spp, y0, y1, _p(out), C.byref(st))
         if rc != 0:
             raise ValueError("orc_render: bad arguments")
         return out, st
 
-    def render64(self, camera16, spp, max_depth, rr_depth=3, seed=0, frame=0, threads=0):
+    def render64(self, camera16, spp, max_depth, rr_depth=3, seed=0, frame=0, threads=0, sampler=0):
         cam = f32(camera16)
         w = int(cam[14:15].view(np.uint32)[0])
         h = int(cam[15:16].view(np.uint32)[0])
         out = np.zeros((h, w, 3), np.float64)
-        cfg = Cfg(max_depth, rr_depth, seed, threads)
+        cfg = Cfg(max_depth, rr_depth, seed, threads, sampler)
         if lib().orc_render64(self._h, _p(cam), C.byref(cfg), frame, spp, _p(out)) != 0:
             raise ValueError("orc_render64: bad arguments")
         return out
 
-    def primary_hits(self, camera16, sample=0, frame=0, max_depth=5, seed=0, y0=0, y1=None, threads=0):
+    def primary_hits(self, camera16, sample=0, frame=0, max_depth=5, seed=0, y0=0, y1=None, threads=0,
+                     sampler=0):
         """First-hit ids and distances of rows [y0, y1) (all rows by default)."""
         cam = f32(camera16)
         w = int(cam[14:15].view(np.uint32)[0])
@@ -145,7 +147,7 @@ class Scene:
         y1 = h if y1 is None else y1
         ids = np.zeros((y1 - y0) * w, np.uint32)
         t = np.zeros((y1 - y0) * w, np.float32)
-        cfg = Cfg(max_depth, 3, seed, threads)
+        cfg = Cfg(max_depth, 3, seed, threads, sampler)
         if lib().orc_primary_hits_rows(self._h, _p(cam), C.byref(cfg), frame, sample, y0, y1, _p(ids), _p(t)):
             raise ValueError("orc_primary_hits_rows: bad arguments")
         return ids.reshape(y1 - y0, w), t.reshape(y1 - y0, w)
@@ -171,6 +173,14 @@ class Scene:
         mp = np.zeros(len(self.materials), np.float32)
         n = lib().orc_light_table(self._h, _p(pmf), _p(cdf), _p(area), _p(mp))
         return n, pmf, cdf, area, mp
+
+
+def strat2(seed, frame, pixel, pair, n, first=0):
+    """Points first .. first + n - 1 of the stratified sampler's 2-D pattern
+    `pair` of one pixel and frame, as an (n, 2) float32 array."""
+    out = np.zeros((n, 2), np.float32)
+    lib().orc_strat2_seq(seed, frame, pixel, pair, first, n, _p(out))
+    return out
 
 
 def pcg32(initstate, initseq, n, advance=0):

@@ -36,11 +36,12 @@ def round_f32(r: Fraction) -> np.float32:
     return best
 
 
-def cfg(spp=4, max_depth=5, seed=0):
+def cfg(spp=4, max_depth=5, seed=0, sampler=0):
     c = aqua.integrator_default()
     c.spp = spp
     c.max_depth = max_depth
     c.seed = seed
+    c.sampler = sampler
     return c
 
 
@@ -134,11 +135,12 @@ def test_trace_occluded_sizes_and_mixed_segments(gpu, box_room, box_room_oracle)
         np.testing.assert_array_equal(occ_g, ((ids_b != aqua.MISS) & past).astype(np.uint8))
 
 
-def _render_match(scene, orc_scene, cam, spp, max_depth, seed=0, frame=0):
-    r = aqua.Renderer(scene, cfg(spp, max_depth, seed))
+def _render_match(scene, orc_scene, cam, spp, max_depth, seed=0, frame=0, sampler=0):
+    r = aqua.Renderer(scene, cfg(spp, max_depth, seed, sampler))
     st = r.render(spp, frame=frame, camera=cam)
     img_g = r.film(cam)
-    img_c, ost = orc_scene.render(aqua.camera_to_array(cam), spp, max_depth, seed=seed, frame=frame)
+    img_c, ost = orc_scene.render(aqua.camera_to_array(cam), spp, max_depth, seed=seed, frame=frame,
+                                  sampler=sampler)
     err = relrmse(img_g, img_c)
     assert err <= RELRMSE_TOL, f"relRMSE {err:.3e}"
     assert st.camera_rays == ost.camera_rays
@@ -152,6 +154,27 @@ def test_render_box_room_parity(gpu, box_room, box_room_oracle):
     g, c, st = _render_match(box_room, box_room_oracle, cam, spp=16, max_depth=5)
     np.testing.assert_array_equal(g, c)
     assert g.mean() > 0.05
+
+
+def test_stratified_sampler_film_and_hits_match_oracle(gpu, box_room, box_room_oracle):
+    # stratified camera / first-hit dimensions: the same integer sampler on
+    # both sides, so films and primary hits stay bit-exact; samples 8.. of
+    # frame 3 (a later pass) use the same patterns
+    cam = small_camera(box_room.camera(), 80, 64)
+    g, c, _ = _render_match(box_room, box_room_oracle, cam, spp=16, max_depth=5, seed=7, frame=3, sampler=1)
+    np.testing.assert_array_equal(g, c)
+    g0, _, _ = _render_match(box_room, box_room_oracle, cam, spp=16, max_depth=5, seed=7, frame=3)
+    assert not np.array_equal(g, g0)
+    r = aqua.Renderer(box_room, cfg(sampler=1, seed=7))
+    r.set_opts(samples_per_pass=4)
+    r.render(16, frame=3, camera=cam)
+    np.testing.assert_array_equal(r.film(cam), c)
+    cam16 = aqua.camera_to_array(cam)
+    for s in (0, 5, 1023):
+        ids_g, t_g = r.primary_hits(sample=s, frame=3, camera=cam)
+        ids_c, t_c = box_room_oracle.primary_hits(cam16, sample=s, frame=3, seed=7, sampler=1)
+        np.testing.assert_array_equal(ids_g, ids_c)
+        np.testing.assert_array_equal(t_g, t_c)
 
 
 def test_render_shipped_cbox_point_light_parity(gpu):
