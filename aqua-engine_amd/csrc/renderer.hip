@@ -1050,14 +1050,13 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
   DevScene sc = r->dev_scene();
   sc.bvh.spill = lane.spill.p;
   Queues q = r->queues(li);
-  int g = grid_for(n);
-  int gp = r->trav_grid(n);
+  const int g_pass = grid_for(n);
   const uint32_t n_slots = pp.nrc ? pp.n_train * (pp.max_depth + 1u) : 0u;
   // the lane's previous film may still be reading its queues on s2
   HIPCHK(hipStreamWaitEvent(s, lane.film_done, 0));
   k_reset<<<64, kBlock, 0, s>>>(q.counters, n, q.cursors, cursor_set(pp.max_depth + 1, 0));
   if (n_slots) k_clear_taux<<<int((n_slots + kBlock - 1) / kBlock), kBlock, 0, s>>>(q.taux, n_slots);
-  k_camera<<<g, kBlock, 0, s>>>(pp, r->cfg.sampler, q);
+  k_camera<<<g_pass, kBlock, 0, s>>>(pp, r->cfg.sampler, q);
   bool timing = r->opts.timing != 0;
   const bool sort_mats = r->opts.sort_materials != 0;
   if (sort_mats)
@@ -1094,8 +1093,19 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
   const bool qfork = overlap && pp.nrc && pp.query_depth <= pp.max_depth;
   auto shadow_on_s2 = [&](uint32_t b) { return overlap && !(qfork && b >= pp.query_depth); };
   bool queries_forked = false;
+  // Past the query vertex only the training paths are alive (query paths
+  // end there and shoot no shadow ray): those bounces launch grids sized for
+  // them, not for the whole pass (the kernels loop over the device counts, so
+  // a grid's size changes no result).
+  auto alive_bound = [&](uint32_t b, bool shadow) -> uint32_t {
+    const bool tail = pp.nrc && (shadow ? b >= pp.query_depth : b > pp.query_depth);
+    return tail ? std::min(n, pp.n_train) : n;
+  };
   for (uint32_t b = 0; b <= pp.max_depth; ++b) {
     uint32_t c = b & 1u;
+    const int g = grid_for(alive_bound(b, false));
+    const int gp = r->trav_grid(alive_bound(b, false));
+    const int gps = r->trav_grid(alive_bound(b, true));
     const uint32_t* cnt = q.counters + cnt_rays(b);
     uint32_t* cur = q.cursors + cursor_set(b, 0);
     hipEvent_t* ev = &r->kev[4 * b];
@@ -1133,11 +1143,11 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
         ss = s2;
       }
       if (count)
-        k_shadow<true><<<gp, kBlock, 0, ss>>>(shadow_bvh, ShadowIO{q, (b & 1u) ^ 1u}, q.counters + cnt_shadow(b),
-                                              q.cursors + cursor_set(b, 1), q.stats);
-      else
-        k_shadow<false><<<gp, kBlock, 0, ss>>>(shadow_bvh, ShadowIO{q, (b & 1u) ^ 1u}, q.counters + cnt_shadow(b),
+        k_shadow<true><<<gps, kBlock, 0, ss>>>(shadow_bvh, ShadowIO{q, (b & 1u) ^ 1u}, q.counters + cnt_shadow(b),
                                                q.cursors + cursor_set(b, 1), q.stats);
+      else
+        k_shadow<false><<<gps, kBlock, 0, ss>>>(shadow_bvh, ShadowIO{q, (b & 1u) ^ 1u}, q.counters + cnt_shadow(b),
+                                                q.cursors + cursor_set(b, 1), q.stats);
       if (shadow_on_s2(b)) HIPCHK(hipEventRecord(lane.join_ev[b], s2));
     }
     if (timing) HIPCHK(hipEventRecord(ev[3], s));
