@@ -9,10 +9,11 @@ namespace aqd {
 // With the integrator's "stratified" sampler the camera ray's two dimensions
 // and the first hit's six come from shuffled, Owen-scrambled 2-D Sobol
 // points over the sample index (Burley 2020, "Practical Hash-based Owen
-// Scrambling", JCGT 9(4)); the PCG32 stream still supplies, and advances
-// past, every other draw. A pixel's first 2^m samples are then a (0, m, 2)-net
-// in each of its patterns: 0 = camera (u0, u1); first hit: 1 = light point
-// (u1, u2), 2 = BSDF direction (u4, u5), 3 = light pick and lobe (u0, u3).
+// Scrambling", JCGT 9(4)); the PCG32 stream still supplies all the other
+// draws and advances past the replaced ones. Every aligned block of 2^m of a
+// pixel's samples is then a (0, m, 2)-net in each of its patterns:
+// 0 = camera (u0, u1); first hit: 1 = light point (u1, u2), 2 = BSDF
+// direction (u4, u5), 3 = light pick and lobe (u0, u3).
 
 // Laine & Karras 2011 hash (the constants of Burley 2020, listing 2): each
 // output bit depends only on the input bits at or below it.
