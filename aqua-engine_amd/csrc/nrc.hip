@@ -1104,6 +1104,13 @@ int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* 
   return AQUA_OK;
 }
 
+// The last training step's loss sum (over its rows x 3 channels) copied to
+// host memory on stream s (pinned memory keeps the copy asynchronous).
+int nrc_loss_async(aqua_nrc* nrc, float* host_dst, hipStream_t s) {
+  HIPCHK(hipMemcpyAsync(host_dst, nrc->loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+  return AQUA_OK;
+}
+
 // `steps` Adam steps on n samples already on the device, stream-ordered on
 // `s`. With stats, the call waits for the steps and reports their timing and
 // the last loss; without, it returns at once (the loss stays on the device).

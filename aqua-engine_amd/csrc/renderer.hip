@@ -854,6 +854,8 @@ struct aqua_renderer {
   DevBuf<unsigned long long> stats;
   // pinned host copies of each pass's counters (read after the frame)
   uint32_t* pass_counters = nullptr;
+  float* loss_host = nullptr;  // pinned: the frame's last training loss (a sum over loss_rows x 3)
+  uint32_t loss_rows = 0;
   size_t pass_counters_cap = 0;
   // films
   uint32_t film_w = 0, film_h = 0, film_spp = 0, oracle_spp = 0;
@@ -880,6 +882,7 @@ struct aqua_renderer {
     if (lane[1].s) (void)hipStreamSynchronize(lane[1].s);
     if (stream) (void)hipStreamSynchronize(stream);
     if (pass_counters) (void)hipHostFree(pass_counters);
+    if (loss_host) (void)hipHostFree(loss_host);
     if (pass0_done) (void)hipEventDestroy(pass0_done);
     if (nrc) aqua_nrc_destroy(nrc);
     for (hipEvent_t e : kev) (void)hipEventDestroy(e);
@@ -958,6 +961,7 @@ int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* 
                      hipStream_t s, const uint32_t* d_count, const uint32_t* d_perm);
 int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
                      aqua_nrc_stats* stats, hipStream_t s, hipEvent_t before_update = nullptr);
+int nrc_loss_async(aqua_nrc* nrc, float* host_dst, hipStream_t s);
 }  // namespace aqua
 
 namespace {
@@ -1233,8 +1237,10 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
 // first 128 steps) four steps per frame, each on a quarter of the samples,
 // so it converges within the first frames; after that one per frame on all
 // of them (measured: the same image error as four, at a third of the cost).
-// Only the last step waits, for its loss. A non-finite loss switches the
-// cache off: later frames render as the plain tracer.
+// No step waits: the last step's loss is copied to pinned host memory on the
+// stream, and the frame reads it once its end event has passed. A
+// non-finite loss switches the cache off: later frames render as the plain
+// tracer.
 int nrc_train_frame(aqua_renderer* r, uint32_t nt, aqua_frame_stats* st, hipEvent_t before_update) {
   hipStream_t s = r->stream;
   const bool timing = r->opts.timing != 0;
@@ -1245,22 +1251,25 @@ int nrc_train_frame(aqua_renderer* r, uint32_t nt, aqua_frame_stats* st, hipEven
     steps = uint32_t(std::min<uint64_t>(steps, r->cfg.training_iters > r->nrc_steps
                                                    ? r->cfg.training_iters - r->nrc_steps : 0u));
   if (steps == 0) return AQUA_OK;
+  if (!r->loss_host)
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&r->loss_host), sizeof(float), hipHostMallocDefault));
   uint32_t per = (nt + steps - 1) / steps;
   uint32_t last = steps;
   for (uint32_t k = 0; k < steps && per > 0; ++k) {
     uint32_t lo = k * per, cnt = std::min(per, nt - std::min(nt, lo));
     if (cnt >= 32) last = k;
   }
-  float loss = 0.0f;
   for (uint32_t k = 0; k <= last && last < steps; ++k) {
     uint32_t lo = k * per, cnt = std::min(per, nt - std::min(nt, lo));
     if (cnt < 32) continue;
-    aqua_nrc_stats ns;
-    if (int e = aqua::nrc_train_device(r->nrc, cnt, r->brec.p + 4 * size_t(lo), r->btgt.p + lo, 1,
-                                       k == last ? &ns : nullptr, s, before_update))
+    if (int e = aqua::nrc_train_device(r->nrc, cnt, r->brec.p + 4 * size_t(lo), r->btgt.p + lo, 1, nullptr, s,
+                                       before_update))
       return e;
     before_update = nullptr;  // later steps follow the first update on s
-    if (k == last) loss = ns.loss;
+    if (k == last) {
+      if (int e = aqua::nrc_loss_async(r->nrc, r->loss_host, s)) return e;
+      r->loss_rows = cnt;
+    }
     ++r->nrc_steps;
   }
   if (timing) {
@@ -1271,8 +1280,6 @@ int nrc_train_frame(aqua_renderer* r, uint32_t nt, aqua_frame_stats* st, hipEven
     st->ms_nrc += ms;
   }
   st->nrc_train_records += nt;
-  st->nrc_loss = loss;
-  if (!std::isfinite(loss)) r->nrc_enabled = false;  // a diverged cache is not used again
   return AQUA_OK;
 }
 
@@ -1598,6 +1605,7 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
   memset(&st, 0, sizeof st);
   bool count = r->opts.count_traversal != 0;
   k_zero_stats<<<1, 64, 0, r->stream>>>(r->stats.p);
+  r->loss_rows = 0;
   HIPCHK(hipEventRecord(r->ev0, r->stream));
   // Two lanes: consecutive passes run concurrently on their own queues and
   // streams (cache queries included), their films chained in pass order;
@@ -1665,6 +1673,13 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
   HIPCHK(hipStreamWaitEvent(r->stream, last_film, 0));
   HIPCHK(hipEventRecord(r->ev1, r->stream));
   HIPCHK(hipEventSynchronize(r->ev1));
+  if (r->loss_rows) {
+    // the frame's last Adam step: its loss arrived before ev1
+    const float loss = *r->loss_host / float(3.0 * double(r->loss_rows));
+    r->loss_rows = 0;
+    st.nrc_loss = loss;
+    if (!std::isfinite(loss)) r->nrc_enabled = false;  // a diverged cache is not used again
+  }
   for (size_t k = 0; k < passes.size(); ++k) add_pass_counts(&st, r->pass_counters + k * kCntWords, passes[k]);
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
