@@ -958,6 +958,12 @@ struct aqua_nrc {
   Buf<float2> fgrad;         // [level][npad] feature-pair gradients, forward/backward -> scatter
   Buf<uint32_t> skey, sidx;  // training batch sort: Morton keys, row order (double buffers)
   Buf<uint8_t> stemp;
+  // a sort queued ahead of the training step by nrc_presort_rows: the rows it
+  // ordered (pre_rec) and their order, for the next nrc_train_device call
+  const float4* pre_rec = nullptr;
+  const uint32_t* pre_perm = nullptr;
+  const uint32_t* pre_sidx = nullptr;  // the order buffer it lives in (a regrown one drops it)
+  uint32_t pre_cap = 0;
   uint32_t cap_rows = 0;
   Buf<float4> rec_buf, tgt_buf;
   Buf<float> out_buf;
@@ -1104,6 +1110,30 @@ int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* 
   return AQUA_OK;
 }
 
+// Queues the Morton sort of the training rows at d_rec (at most n_max; the
+// real count *d_count stays on the device) on stream s, so that it runs right
+// after the work that wrote them instead of after the host has read the
+// count. The next nrc_train_device call on the same rows (d_rec, n =
+// *d_count) uses this order instead of sorting; any other call drops it.
+int nrc_presort_rows(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, const uint32_t* d_count, hipStream_t s) {
+  nrc->pre_rec = nullptr;
+  if (n_max == 0 || !d_count) return AQUA_OK;
+  if (int e = nrc->skey.alloc(size_t(2) * n_max)) return e;
+  if (int e = nrc->sidx.alloc(size_t(2) * n_max)) return e;
+  size_t need = 0;
+  HIPCHK(aqua::query_sort_storage(n_max, &need));
+  if (int e = nrc->stemp.alloc(need)) return e;
+  HIPCHK(aqua::query_sort(nrc->stemp.p, nrc->stemp.n, d_rec, d_count, n_max, nrc->skey.p, nrc->sidx.p, s,
+                          &nrc->pre_perm));
+  nrc->pre_rec = d_rec;
+  nrc->pre_sidx = nrc->sidx.p;
+  nrc->pre_cap = n_max;
+  return AQUA_OK;
+}
+
+// Forgets a queued-ahead sort (its rows are about to change).
+void nrc_presort_drop(aqua_nrc* nrc) { nrc->pre_rec = nullptr; }
+
 // The last training step's loss sum (over its rows x 3 channels) copied to
 // host memory on stream s (pinned memory keeps the copy asynchronous).
 int nrc_loss_async(aqua_nrc* nrc, float* host_dst, hipStream_t s) {
@@ -1119,19 +1149,26 @@ int nrc_loss_async(aqua_nrc* nrc, float* host_dst, hipStream_t s) {
 // backward passes only read them).
 int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
                      aqua_nrc_stats* stats, hipStream_t s, hipEvent_t before_update) {
-  if (n == 0 || steps == 0) return AQUA_OK;
+  if (n == 0 || steps == 0) {
+    nrc->pre_rec = nullptr;
+    return AQUA_OK;
+  }
   if (int e = nrc->ensure_rows(n)) return e;
   FwdArgs a = nrc->args(n, d_rec, d_tgt);
   a.pack_fwd = nrc->pack_fwd.p;
   a.pack_bwd = nrc->pack_bwd.p;
   // rows in Morton order: the scatter merges rows that share coarse entries
-  {
+  // (sorted here unless nrc_presort_rows already queued these rows' sort)
+  if (nrc->pre_rec == d_rec && n <= nrc->pre_cap && nrc->pre_sidx == nrc->sidx.p) {
+    a.perm = nrc->pre_perm;
+  } else {
     size_t need = 0;
     HIPCHK(aqua::query_sort_storage(n, &need));
     if (int e = nrc->stemp.alloc(need)) return e;
     HIPCHK(aqua::query_sort(nrc->stemp.p, nrc->stemp.n, d_rec, nullptr, n, nrc->skey.p, nrc->sidx.p, s,
                             &a.perm));
   }
+  nrc->pre_rec = nullptr;
   uint32_t npad = a.npad;
   uint32_t nchunks = n_chunks(npad);
   double t_fb = 0, t_wg = 0, t_ad = 0, t_sc = 0;

@@ -856,6 +856,7 @@ struct aqua_renderer {
   uint32_t* pass_counters = nullptr;
   float* loss_host = nullptr;  // pinned: the frame's last training loss (a sum over loss_rows x 3)
   uint32_t loss_rows = 0;
+  bool presort = false;  // the pass queues the sort of its training rows (one step on all of them)
   size_t pass_counters_cap = 0;
   // films
   uint32_t film_w = 0, film_h = 0, film_spp = 0, oracle_spp = 0;
@@ -962,6 +963,8 @@ int nrc_infer_device(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, float* 
 int nrc_train_device(aqua_nrc* nrc, uint32_t n, const float4* d_rec, const float4* d_tgt, uint32_t steps,
                      aqua_nrc_stats* stats, hipStream_t s, hipEvent_t before_update = nullptr);
 int nrc_loss_async(aqua_nrc* nrc, float* host_dst, hipStream_t s);
+int nrc_presort_rows(aqua_nrc* nrc, uint32_t n_max, const float4* d_rec, const uint32_t* d_count, hipStream_t s);
+void nrc_presort_drop(aqua_nrc* nrc);
 }  // namespace aqua
 
 namespace {
@@ -1188,6 +1191,10 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
     k_nrc_targets<<<std::min(grid_for(n_slots), 4096), kBlock, 0, s>>>(
         q.trec, q.taux, n_slots, pp.max_depth + 1u, pp.train_stride, pp.train_offset, q.Lbuf, r->brec.p,
         r->btgt.p, q.counters + kCntTrain);
+    // the frame's single Adam step takes all these rows: their sort goes in
+    // stream order now, ahead of the count read below
+    if (r->presort)
+      if (int e = aqua::nrc_presort_rows(r->nrc, n_slots, r->brec.p, q.counters + kCntTrain, s)) return e;
   }
   HIPCHK(hipGetLastError());
   if (counters_out) {
@@ -1241,15 +1248,22 @@ int run_pass(aqua_renderer* r, int li, const PassParams& pp, bool fp64, aqua_fra
 // stream, and the frame reads it once its end event has passed. A
 // non-finite loss switches the cache off: later frames render as the plain
 // tracer.
+// Adam steps the next frame takes: four while the cache is young (its first
+// 128), then one, never past training_iters (0: no limit).
+uint32_t frame_steps(const aqua_renderer* r) {
+  uint32_t steps = r->nrc_steps < 128 ? 4u : 1u;
+  if (r->cfg.training_iters > 0)
+    steps = uint32_t(std::min<uint64_t>(steps, r->cfg.training_iters > r->nrc_steps
+                                                   ? r->cfg.training_iters - r->nrc_steps : 0u));
+  return steps;
+}
+
 int nrc_train_frame(aqua_renderer* r, uint32_t nt, aqua_frame_stats* st, hipEvent_t before_update) {
   hipStream_t s = r->stream;
   const bool timing = r->opts.timing != 0;
   r->batch_n = nt;
   if (timing) HIPCHK(hipEventRecord(r->nev[2], s));
-  uint32_t steps = r->nrc_steps < 128 ? 4u : 1u;
-  if (r->cfg.training_iters > 0)  // the frame takes no step past training_iters
-    steps = uint32_t(std::min<uint64_t>(steps, r->cfg.training_iters > r->nrc_steps
-                                                   ? r->cfg.training_iters - r->nrc_steps : 0u));
+  const uint32_t steps = frame_steps(r);
   if (steps == 0) return AQUA_OK;
   if (!r->loss_host)
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&r->loss_host), sizeof(float), hipHostMallocDefault));
@@ -1279,6 +1293,7 @@ int nrc_train_frame(aqua_renderer* r, uint32_t nt, aqua_frame_stats* st, hipEven
     HIPCHK(hipEventElapsedTime(&ms, r->nev[2], r->nev[3]));
     st->ms_nrc += ms;
   }
+  aqua::nrc_presort_drop(r->nrc);  // used by the step above, or not at all
   st->nrc_train_records += nt;
   return AQUA_OK;
 }
@@ -1637,6 +1652,8 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
   }
   std::vector<PassParams> passes;
   bool train = false;  // a pass collected training samples
+  if (r->nrc) aqua::nrc_presort_drop(r->nrc);
+  r->presort = cache && n_pass == 1 && !r->opts.timing && frame_steps(r) == 1;
   for (uint32_t k = 0, s0 = 0; s0 < spp; s0 += S, ++k) {
     pp.sample0 = spp0 + s0;
     pp.spp_in_pass = std::min(S, spp - s0);
@@ -1670,6 +1687,8 @@ static int render_samples(aqua_renderer* r, const aqua_camera* cam_in, uint32_t 
     }
     if (int e = nrc_train_frame(r, nt, &st, last_film)) return e;
   }
+  r->presort = false;
+  if (r->nrc) aqua::nrc_presort_drop(r->nrc);
   HIPCHK(hipStreamWaitEvent(r->stream, last_film, 0));
   HIPCHK(hipEventRecord(r->ev1, r->stream));
   HIPCHK(hipEventSynchronize(r->ev1));
