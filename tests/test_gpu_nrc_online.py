@@ -261,3 +261,35 @@ def test_training_stops_after_training_iters(gpu, box_room):
     r.render(1, frame=7, camera=cam)
     r.render(1, frame=50, camera=cam)
     np.testing.assert_array_equal(r.film(cam), before)
+
+
+def test_queued_training_sort_matches_the_in_step_sort(gpu, box_room):
+    # Once the cache takes one Adam step per frame (after 128), a frame queues
+    # its training rows' sort in the pass, ahead of the host's count read;
+    # with per-kernel timing on it sorts them inside the step instead. From
+    # one state both must train on the same rows: equal loss up to the float
+    # atomics' summation order, and next frames that agree just as closely.
+    cam = small_camera(box_room.camera(), 64, 48)
+    c = aqua.integrator_default()
+    c.type = aqua.INTEGRATOR_NRC
+    c.spp, c.max_depth, c.learning_rate, c.nrc_train_paths = 1, 6, 1e-2, 2048
+    r = aqua.Renderer(box_room, c)
+    r.set_opts(timing=False)
+    for f in range(34):  # 4 steps a frame: past the first 128
+        r.render(1, frame=f, camera=cam)
+    r.nrc_checkpoint()
+    out = {}
+    for timing in (False, True, False):
+        r.nrc_checkpoint(restore=True)
+        r.set_opts(timing=timing)
+        st = r.render(2, frame=40, camera=cam)
+        assert st.nrc_train_records > 0
+        r.set_opts(timing=False)
+        r.render(2, frame=41, camera=cam)
+        out.setdefault(timing, []).append((st.nrc_loss, r.film(cam)))
+    (l_q, f_q), (l_q2, f_q2) = out[False]
+    (l_s, f_s), = out[True]
+    assert np.isfinite(l_q) and l_q > 0
+    for l, f in ((l_s, f_s), (l_q2, f_q2)):
+        assert abs(l - l_q) <= 1e-4 * abs(l_q), (l, l_q)
+        assert np.sqrt(np.sum((f - f_q) ** 2) / np.sum(f_q ** 2)) < 1e-3
