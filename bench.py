@@ -216,9 +216,19 @@ def box_room_leg(aqua, oracle_spp=4096, steps=10):
     wall = time.perf_counter() - t0
     r.render_oracle(0, oracle_spp, frame=777)
     e, _ = r.film_error()
+    # the same frame with the stratified sampler (camera and first-hit
+    # dimensions), against the same FP64 render
+    import numpy as np
+    ref = r.oracle_film()
+    cfg.sampler = aqua.SAMPLER_STRATIFIED
+    rs = aqua.Renderer(scene, cfg)
+    rs.set_opts(timing=False)
+    rs.render(16, frame=0)
+    x = rs.film().astype(np.float64)
+    e_s = float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
     return {"scene": "box room, 36 triangles, 512x512, 16 spp, max depth 5",
             "ms_per_frame": round(wall * 1e3 / steps, 3), "mrays_per_s": round(rays / wall / 1e6, 1),
-            "relmse_vs_fp64": round(e, 6), "oracle_spp": oracle_spp}
+            "relmse_vs_fp64": round(e, 6), "relmse_vs_fp64_stratified": round(e_s, 6), "oracle_spp": oracle_spp}
 
 
 def material_sort_leg(aqua, spp=16, steps=5, width=1920, height=1080):
