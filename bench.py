@@ -678,6 +678,13 @@ def main():
             print("bench: cache evaluation leg, stratified sampler", file=sys.stderr, flush=True)
             out["nrc_eval_stratified"] = nrc_eval_leg(aqua, sampler=aqua.SAMPLER_STRATIFIED, oracles=oracles)
             del oracles
+            ev, es = out["nrc_eval"], out["nrc_eval_stratified"]
+            out["north_star"] = {
+                "metric": "cache ms/frame to relMSE 1e-2 over the plain tracer's (query at vertex 2, 24 frames)",
+                "target": 0.25,
+                "stratified": {"f16": es["nrc_v2_f16_over_pt"], "bf16": es["nrc_v2_bf16_over_pt"]},
+                "independent": {"f16": ev["nrc_v2_f16_over_pt"], "bf16": ev["nrc_v2_bf16_over_pt"]},
+            }
         if "large" in legs:
             print("bench: large-scene leg", file=sys.stderr, flush=True)
             out["large"] = large_leg(aqua)
