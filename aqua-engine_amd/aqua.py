@@ -101,7 +101,8 @@ class FrameStats(C.Structure):
 
 class RendererOpts(C.Structure):
     _fields_ = [("samples_per_pass", C.c_uint32), ("count_traversal", C.c_uint32),
-                ("timing", C.c_uint32), ("sort_materials", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("timing", C.c_uint32), ("sort_materials", C.c_uint32), ("train_ahead", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
 
 
 NRC_F16, NRC_BF16, NRC_FP8 = 0, 1, 2
@@ -418,8 +419,9 @@ class Renderer:
             pass
 
     def set_opts(self, samples_per_pass: int = 0, count_traversal: bool = False, timing: bool = True,
-                 sort_materials: bool = False):
-        o = RendererOpts(samples_per_pass, int(count_traversal), int(timing), int(sort_materials))
+                 sort_materials: bool = False, train_ahead: bool = False):
+        o = RendererOpts(samples_per_pass, int(count_traversal), int(timing), int(sort_materials),
+                         int(train_ahead))
         _check(lib().aqua_renderer_set_opts(self._h, C.byref(o)))
 
     def render(self, spp: int, frame: int = 0, camera: Optional[Camera] = None) -> FrameStats:

@@ -167,7 +167,8 @@ pub mod ffi {
         pub count_traversal: u32,
         pub timing: u32,
         pub sort_materials: u32,
-        pub reserved: [u32; 4],
+        pub train_ahead: u32,
+        pub reserved: [u32; 3],
     }
 
     #[repr(C)]

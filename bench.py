@@ -410,7 +410,7 @@ def nrc_infer_leg(aqua, warm_frames=40, frames=5, spp=4, width=1920, height=1080
 
 
 def nrc_eval_leg(aqua, frames=tuple(range(5, 240, 10)), n_frames=240, oracle_spp=4096, width=1920,
-                 height=1080, max_spp=256, sampler=0, oracles=None):
+                 height=1080, max_spp=256, sampler=0, oracles=None, train_ahead=False):
     """ms/frame to relMSE <= 1e-2 against an FP64 render of `oracle_spp` samples,
     plain path tracer vs cache, on the box room fly-through: max depth 8, 2^16
     training suffix paths per frame, online training at 1 spp through every
@@ -430,7 +430,9 @@ def nrc_eval_leg(aqua, frames=tuple(range(5, 240, 10)), n_frames=240, oracle_spp
     `sampler` is the tracer's and the caches' sample generator (independent
     PCG32 draws, or stratified camera / first-hit dimensions). The FP64
     renders are filled into `oracles` (frame -> film) on the first call and
-    reused by a later one."""
+    reused by a later one. With `train_ahead` the caches train each frame's
+    batch at the start of the next frame, beside its first bounces (the
+    checkpoint keeps the batch that waits, so every rung trains it)."""
     with tempfile.TemporaryDirectory() as d:
         scene = aqua.Scene.load(aqua.procgen(aqua.PROCGEN_BOX_ROOM, d, width=width, height=height))
     pt = aqua.integrator_default()
@@ -450,7 +452,7 @@ def nrc_eval_leg(aqua, frames=tuple(range(5, 240, 10)), n_frames=240, oracle_spp
         c.nrc_query_vertex = qv
         c.sampler = sampler
         caches[key] = aqua.Renderer(scene, c)
-        caches[key].set_opts(timing=False)
+        caches[key].set_opts(timing=False, train_ahead=train_ahead)
     out = []
     next_frame = 0
     t_oracle = 0.0

@@ -260,7 +260,12 @@ typedef struct aqua_renderer_opts {
   uint32_t timing;             /* 1 = per-kernel HIP-event timing             */
   uint32_t sort_materials;     /* 1 = order each bounce's hits by material
                                   before shading (same film)                 */
-  uint32_t reserved[4];
+  uint32_t train_ahead;        /* "nrc": 1 = train a frame's batch at the start
+                                  of the next frame, beside its first bounces
+                                  (its queries wait for the update, so they see
+                                  the same weights as with 0); the batch waiting
+                                  is part of the cache's checkpoint           */
+  uint32_t reserved[3];
 } aqua_renderer_opts;
 
 int aqua_renderer_create(const aqua_scene* scene, const aqua_integrator_cfg* cfg, int device,

@@ -293,3 +293,78 @@ def test_queued_training_sort_matches_the_in_step_sort(gpu, box_room):
     for l, f in ((l_s, f_s), (l_q2, f_q2)):
         assert abs(l - l_q) <= 1e-4 * abs(l_q), (l, l_q)
         assert np.sqrt(np.sum((f - f_q) ** 2) / np.sum(f_q ** 2)) < 1e-3
+
+
+def _rel(a, b):
+    return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+def test_train_ahead_queries_see_the_same_weights(gpu, box_room):
+    # train_ahead trains a frame's batch at the start of the next frame (its
+    # queries wait for the update), so every frame queries the weights it
+    # would have queried with the step at the end of the previous frame.
+    # Two runs of the same renderer already differ a little (training rows
+    # are appended in no fixed order, and the young cache's four steps take
+    # a quarter of them each; float atomics): train_ahead must stay within
+    # that spread. Frame 0 (no training before it) is the same bit for bit.
+    cam = small_camera(box_room.camera(), 64, 48)
+    c = nrc_cfg(max_depth=5, train_paths=2048)
+    a, a2, b = (aqua.Renderer(box_room, c) for _ in range(3))
+    for r in (a, a2):
+        r.set_opts(timing=False)
+    b.set_opts(timing=False, train_ahead=True)
+    spread, gap = [], []
+    for f in list(range(8)) + [20]:
+        spp = 2 if f == 20 else 1  # frame 20: the last batch is trained first
+        sa, _, sb = (r.render(spp, frame=f, camera=cam) for r in (a, a2, b))
+        fa, fa2, fb = a.film(cam), a2.film(cam), b.film(cam)
+        if f == 0:
+            np.testing.assert_array_equal(fa, fb)
+        assert sb.nrc_queries == sa.nrc_queries and sb.nrc_train_records > 0
+        spread.append(_rel(fa2, fa))
+        gap.append(_rel(fb, fa))
+    assert max(gap) < max(4 * max(spread), 1e-4), (gap, spread)
+
+
+def test_train_ahead_checkpoint_keeps_the_waiting_batch(gpu, box_room):
+    # the batch that waits for the next frame is part of the checkpoint: after
+    # a restore the next frame trains it again before its queries (float
+    # atomics: equal up to the spread of two such restores)
+    cam = small_camera(box_room.camera(), 64, 48)
+    r = aqua.Renderer(box_room, nrc_cfg(max_depth=5, train_paths=2048))
+    r.set_opts(timing=False, train_ahead=True)
+    for f in range(5):
+        r.render(1, frame=f, camera=cam)
+    r.nrc_checkpoint()
+    r.render(2, frame=100, camera=cam)
+    at_ckpt = r.film(cam)
+    for f in range(5, 9):
+        r.render(1, frame=f, camera=cam)
+    r.render(2, frame=100, camera=cam)
+    moved = r.film(cam)
+    backs = []
+    for _ in range(2):
+        r.nrc_checkpoint(restore=True)
+        r.render(2, frame=100, camera=cam)
+        backs.append(r.film(cam))
+    spread = _rel(backs[1], backs[0])
+    assert _rel(backs[0], at_ckpt) < max(5 * spread, 1e-4), (_rel(backs[0], at_ckpt), spread)
+    assert _rel(moved, at_ckpt) > 5 * max(_rel(backs[0], at_ckpt), 1e-4)
+
+
+def test_train_ahead_diverged_cache_falls_back(gpu, box_room):
+    cam = small_camera(box_room.camera(), 64, 64)
+    r = aqua.Renderer(box_room, nrc_cfg(max_depth=5, train_paths=1024, lr=1e30))
+    r.set_opts(timing=False, train_ahead=True)
+    for f in range(6):
+        st = r.render(1, camera=cam, frame=f)
+        if st.nrc_queries == 0:
+            break
+    assert st.nrc_queries == 0
+    pt = aqua.integrator_default()
+    pt.max_depth = 5
+    a = aqua.Renderer(box_room, pt)
+    a.render(4, camera=cam, frame=9)
+    st = r.render(4, camera=cam, frame=9)
+    assert st.nrc_queries == 0 and st.nrc_train_records == 0
+    np.testing.assert_array_equal(r.film(cam), a.film(cam))
